@@ -1,0 +1,167 @@
+/*
+ * mpres-hip: MI355X (gfx950) implementation of the RNS multiple-precision hot
+ * path of MPRES-BLAS, as a C ABI.
+ *
+ * Data types are layout-identical to the reference (src/types.cuh:46-113), so
+ * buffers built for the reference can be handed over unchanged:
+ *   er_float_t       {double frac; long exp}                       16 bytes
+ *   mp_float_t<n>    {int digits[n]; int sign; int exp; er_float_t eval[2]}  4n+40 bytes
+ *   mp_array_t       structure of arrays with buf and a device-side len
+ *   mp_collection_t  structure of arrays without buf/len
+ *   csr_t            {double *as; int *ja; int *irp}
+ * All pointers inside these structs are device pointers. The number of moduli
+ * n is a property of the context (one of the reference params files), not of
+ * the type, so mp_float_t arrays are passed as void pointers with a stride of
+ * mpres_mp_float_size(ctx) bytes.
+ *
+ * Every entry point returns an int status (MPRES_OK or a negative error) and
+ * enqueues its kernels on the caller's HIP stream (NULL = default stream).
+ * Results are bit-identical to the reference's CUDA algorithm on the same
+ * inputs and summation order: operations whose result depends on the order of
+ * multiple-precision additions take the reference's launch parameters that
+ * fix that order (the "order" arguments below) and reproduce it.
+ *
+ * Workspace: entry points that need scratch space take (ws, ws_bytes). Pass
+ * ws == NULL to use a buffer owned by the context (allocated on first use and
+ * grown when needed); the matching mpres_*_workspace() call returns the size.
+ */
+#ifndef MPRES_HIP_H
+#define MPRES_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define MPRES_OK 0
+#define MPRES_ERR_ARG (-1)        /* invalid argument (size, stride, leading dimension, order) */
+#define MPRES_ERR_HIP (-2)        /* a HIP runtime call failed; see mpres_last_error() */
+#define MPRES_ERR_WORKSPACE (-3)  /* caller workspace too small */
+#define MPRES_ERR_UNSUPPORTED (-4)/* valid call the library does not implement (e.g. transposed v1 gemm) */
+#define MPRES_ERR_NO_DEVICE (-5)  /* no gfx950 device */
+
+/* ---- reference types (src/types.cuh) ---- */
+typedef struct { double frac; long exp; } er_float_t;                 /* src/types.cuh:46-49 */
+typedef struct { int x, y, z, w; } mpres_int4_t;                      /* CUDA int4 */
+typedef struct {                                                      /* src/types.cuh:85-92 */
+    int *digits;
+    int *sign;
+    int *exp;
+    er_float_t *eval;
+    mpres_int4_t *buf;
+    int *len;
+} mp_array_t;
+typedef struct {                                                      /* src/types.cuh:99-104 */
+    int *digits;
+    int *sign;
+    int *exp;
+    er_float_t *eval;
+} mp_collection_t;
+typedef struct { double *as; int *ja; int *irp; } csr_t;              /* src/types.cuh:109-113 */
+
+/* mblas_trans_type, src/blas/mblas_enum.cuh:25-31 */
+#define MPRES_NO_TRANS 111
+#define MPRES_TRANS 112
+#define MPRES_CONJ_TRANS 113
+
+typedef struct mpres_ctx mpres_ctx_t;
+
+/* ---- library / context ----
+ * Replaces rns_const_init() + mp_const_init() (src/rns.cuh:324-442,
+ * src/arith/arith_utils.cuh:44-85): the constants of one moduli set are
+ * computed on the host and placed in device memory owned by the context. */
+int mpres_num_moduli_sets(void);
+const char *mpres_moduli_set_name(int set_id);   /* "8_2double", "16_4double", "32_8double", ... */
+int mpres_moduli_set_id(const char *name);        /* -1 if unknown */
+int mpres_ctx_create(mpres_ctx_t **ctx, int set_id, int device);
+int mpres_ctx_destroy(mpres_ctx_t *ctx);
+/* n, MP_PRECISION, MP_H, MP_J, RNS_MODULI_PRODUCT_LOG2 */
+int mpres_ctx_info(const mpres_ctx_t *ctx, int *n, int *precision, int *mp_h, int *mp_j, int *log2m);
+size_t mpres_mp_float_size(const mpres_ctx_t *ctx);  /* 4n + 40 */
+const char *mpres_last_error(void);                  /* thread-local message of the last failure */
+/* copies of the device constant tables, for checking against an independent recomputation:
+ * what = 0 moduli (n ints), 1 (M/m_i)^-1 mod m_i (n), 2 RNS_POW2 ((log2m+1)*n), 3 M mod 2^(j+1) (30),
+ * 4 M_i mod 2^(j+1) (30*n), 5 2^-(j+1) mod m_i (30*n), 6 MRC inverses (n*n);
+ * what = 7 doubles: recip_rd (n), recip_ru (n), acc; what = 8 er_float_t: unit_low, unit_upp, inv_unit_low, inv_unit_upp;
+ * what = 9 ints: MP_PRECISION, MP_H, MP_J, RNS_EVAL_REF_FACTOR, RNS_MODULI_PRODUCT_LOG2 */
+int mpres_ctx_table(const mpres_ctx_t *ctx, int what, void *out, size_t bytes);
+/* the same tables computed on the host for a moduli set, without a device */
+int mpres_moduli_set_table(int set_id, int what, void *out, size_t bytes);
+
+/* ---- mp_array_t management, src/mparray.cuh:35-165 ----
+ * host2device / device2host take host arrays of mp_float_t (AoS, 4n+40 bytes each)
+ * and move them in one transfer per field instead of one per element. */
+int mpres_array_init(mpres_ctx_t *ctx, mp_array_t *a, size_t size);
+int mpres_array_clear(mpres_ctx_t *ctx, mp_array_t *a);
+int mpres_array_host2device(mpres_ctx_t *ctx, mp_array_t *dst, const void *host_src, size_t size, void *stream);
+int mpres_array_device2host(mpres_ctx_t *ctx, void *host_dst, const mp_array_t *src, size_t size, void *stream);
+
+/* ---- double -> mp conversion on the GPU: mp_set_d, src/arith/assign.cuh:54-81 ----
+ * (the reference converts on the CPU only). x and dst are device pointers. */
+int mpres_array_set_d(mpres_ctx_t *ctx, mp_array_t *dst, const double *x, int count, void *stream);
+int mpres_set_d(mpres_ctx_t *ctx, void *dst_aos, const double *x, int count, void *stream);
+/* AoS <-> SoA on the device (mp_float_t[] <-> mp_array_t, first count elements) */
+int mpres_aos_to_array(mpres_ctx_t *ctx, mp_array_t *dst, const void *src_aos, int count, void *stream);
+int mpres_array_to_aos(mpres_ctx_t *ctx, void *dst_aos, const mp_array_t *src, int count, void *stream);
+
+/* ---- element-wise arithmetic on AoS arrays (cuda::mp_add / mp_sub / mp_mul / mp_mul_d,
+ * src/arith/{add,sub,mul,muld}.cuh); r[i] = x[i] op y[i] ---- */
+#define MPRES_OP_ADD 0
+#define MPRES_OP_SUB 1
+#define MPRES_OP_MUL 2
+#define MPRES_OP_MUL_D 3   /* y is a device array of double */
+#define MPRES_OP_VADD 4    /* the v1 vector-add kernels: src/mpvector.cuh:234-472 + round :669-714 */
+int mpres_elementwise(mpres_ctx_t *ctx, int op, int count, const void *x, const void *y, void *r, void *stream);
+
+/* ---- BLAS, v1 interface over mp_array_t ---- */
+/* r = x . y ; cuda::mp_dot<gridDim1,blockDim1,gridDim2,gridDim3,blockDim3>, src/blas/dot.cuh:84-107.
+ * The sum order is fixed by (gridDim3, blockDim3); the other template arguments do not change the result. */
+size_t mpres_dot_workspace(const mpres_ctx_t *ctx, int n, int grid3, int block3);
+int mpres_dot(mpres_ctx_t *ctx, int n, const mp_array_t *x, int incx, const mp_array_t *y, int incy, mp_array_t *r,
+              int grid3, int block3, void *ws, size_t ws_bytes, void *stream);
+/* y = alpha*x + y ; cuda::mp_axpy<gridDim1,blockDim1,gridDim2>, src/blas/axpy.cuh:45-74 (order-free) */
+int mpres_axpy(mpres_ctx_t *ctx, int n, const mp_array_t *alpha, const mp_array_t *x, int incx, mp_array_t *y,
+               int incy, void *stream);
+/* y = alpha*op(A)*x + beta*y ; cuda::mp_gemv<gridDim1,blockDim1,gridDim2,blockDim3>, src/blas/gemv.cuh:150-268.
+ * The sum order of each output is fixed by blockDim3. */
+size_t mpres_gemv_workspace(const mpres_ctx_t *ctx, int trans, int m, int n, int block3);
+int mpres_gemv(mpres_ctx_t *ctx, int trans, int m, int n, const mp_array_t *alpha, const mp_array_t *A, int lda,
+               const mp_array_t *x, int incx, const mp_array_t *beta, mp_array_t *y, int incy, int block3, void *ws,
+               size_t ws_bytes, void *stream);
+/* C = alpha*A*B + beta*C ; cuda::mp_gemm<...>, src/blas/gemm.cuh:69-167 (no-trans only, as the reference) */
+size_t mpres_gemm_workspace(const mpres_ctx_t *ctx, int m, int n, int k);
+int mpres_gemm(mpres_ctx_t *ctx, int transa, int transb, int m, int n, int k, const mp_array_t *alpha,
+               const mp_array_t *A, int lda, const mp_array_t *B, int ldb, const mp_array_t *beta, mp_array_t *C,
+               int ldc, void *ws, size_t ws_bytes, void *stream);
+
+/* ---- BLAS, v2 interface over mp_float_t arrays (src/blas/v2) ---- */
+/* cuda::mp_dot<blocks,threads>(n,x,y,r), src/blas/v2/dot_v2.cuh:121-129 */
+size_t mpres_dot_v2_workspace(const mpres_ctx_t *ctx, int n, int blocks, int threads);
+int mpres_dot_v2(mpres_ctx_t *ctx, int n, const void *x, const void *y, void *r, int blocks, int threads, void *ws,
+                 size_t ws_bytes, void *stream);
+/* cuda::mp_axpy(n,alpha,x,y,r), src/blas/v2/axpy_v2.cuh:39-52 */
+int mpres_axpy_v2(mpres_ctx_t *ctx, int n, const void *alpha, const void *x, const void *y, void *r, void *stream);
+/* cuda::mp_gemm(transa,transb,...), src/blas/v2/gemm_v2.cuh:55-81 */
+int mpres_gemm_v2(mpres_ctx_t *ctx, int transa, int transb, int m, int n, int k, const void *alpha, const void *A,
+                  int lda, const void *B, int ldb, const void *beta, void *C, int ldc, void *stream);
+
+/* ---- sparse: double CSR matrix times mp vector (x, y: mp_float_t arrays) ---- */
+/* cuda::mp_spmv_csr<threads>, src/sparse/spmv/spmv_csr.cuh:48-64 */
+int mpres_spmv_csr(mpres_ctx_t *ctx, int m, const csr_t *csr, const void *x, void *y, void *stream);
+/* cuda::mp_spmv_csrv<threads,threadsPerRow>, src/sparse/spmv/spmv_csrv.cuh:48-87; threads_per_row in {1,2,4,8,16,32} */
+int mpres_spmv_csrv(mpres_ctx_t *ctx, int m, const csr_t *csr, int threads_per_row, const void *x, void *y,
+                    void *stream);
+
+/* ---- tuning ----
+ * Lanes per multiple-precision number used by the kernels of an operation
+ * (0 = the built-in choice). op: 0 elementwise, 1 dot, 2 gemv, 3 gemm, 4 spmv, 5 axpy.
+ * Only values compiled into the library are accepted (mpres_lanes_supported). */
+int mpres_lanes_supported(const mpres_ctx_t *ctx, int lanes);
+int mpres_set_lanes(mpres_ctx_t *ctx, int op, int lanes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,569 @@
+// C ABI of the MI355X RNS multiple-precision path (include/mpres_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mpres_hip.h"
+#include "internal.hpp"
+#include "ops.hpp"
+
+using namespace mph;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return fail(MPRES_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+enum OpKind { OP_ELEM = 0, OP_DOT = 1, OP_GEMV = 2, OP_GEMM = 3, OP_SPMV = 4, OP_AXPY = 5, OP_COUNT = 6 };
+
+const Ops *find_ops(int n, int L) {
+    switch (n) {
+        case 8: return ops_n8(L);
+        case 16: return ops_n16(L);
+        case 32: return ops_n32(L);
+        default: return nullptr;
+    }
+}
+
+// Built-in lanes-per-number choice for each (moduli count, operation):
+// one lane per modulus everywhere unless a narrower group is compiled and
+// measured to be faster (the tuning tables override these).
+int default_lanes(int n, int op) {
+    (void)op;
+    return n;
+}
+
+}  // namespace
+
+struct mpres_ctx {
+    int device = 0;
+    int set_id = -1;
+    DevConsts host{};
+    DevConsts *dev = nullptr;
+    int *dpow2 = nullptr;
+    const Ops *ops[OP_COUNT] = {};
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+};
+
+namespace {
+
+size_t mpsize(const mpres_ctx *c) { return (size_t)(4 * c->host.n + 40); }
+
+SoA soa_of(const mp_array_t *a) {
+    SoA s;
+    s.d = a->digits;
+    s.s = a->sign;
+    s.e = a->exp;
+    s.ev = (ErMem *)a->eval;
+    s.lenp = a->len;
+    s.len = 0;
+    return s;
+}
+
+// workspace: caller's buffer or the context pool
+int get_ws(mpres_ctx *c, void *ws, size_t ws_bytes, size_t need, char **out) {
+    if (need == 0) {
+        *out = nullptr;
+        return MPRES_OK;
+    }
+    if (ws) {
+        if (ws_bytes < need) return fail(MPRES_ERR_WORKSPACE, "workspace of %zu bytes, %zu needed", ws_bytes, need);
+        *out = (char *)ws;
+        return MPRES_OK;
+    }
+    if (c->ws_bytes < need) {
+        if (c->ws) {
+            HIPCHK(hipDeviceSynchronize());
+            HIPCHK(hipFree(c->ws));
+            c->ws = nullptr;
+            c->ws_bytes = 0;
+        }
+        HIPCHK(hipMalloc(&c->ws, need));
+        c->ws_bytes = need;
+    }
+    *out = (char *)c->ws;
+    return MPRES_OK;
+}
+
+unsigned next_pow2(unsigned v) {  // src/common.cuh:92-101
+    v--;
+    v |= v >> 1;
+    v |= v >> 2;
+    v |= v >> 4;
+    v |= v >> 8;
+    v |= v >> 16;
+    return v + 1;
+}
+
+hipStream_t strm(void *st) { return (hipStream_t)st; }
+
+}  // namespace
+
+extern "C" {
+
+const char *mpres_last_error(void) { return g_err.c_str(); }
+int mpres_num_moduli_sets(void) { return num_sets(); }
+const char *mpres_moduli_set_name(int id) {
+    const ModuliSet *s = get_set(id);
+    return s ? s->name : nullptr;
+}
+int mpres_moduli_set_id(const char *name) {
+    if (!name) return -1;
+    for (int i = 0; i < num_sets(); i++)
+        if (std::strcmp(get_set(i)->name, name) == 0) return i;
+    return -1;
+}
+
+int mpres_ctx_create(mpres_ctx_t **out, int set_id, int device) {
+    if (!out) return fail(MPRES_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    const ModuliSet *s = get_set(set_id);
+    if (!s) return fail(MPRES_ERR_ARG, "unknown moduli set %d", set_id);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0)
+        return fail(MPRES_ERR_NO_DEVICE, "no HIP device %d", device);
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(MPRES_ERR_NO_DEVICE, "device %d is %s; this library is built for gfx950", device, prop.gcnArchName);
+    HIPCHK(hipSetDevice(device));
+    auto *c = new mpres_ctx();
+    c->device = device;
+    c->set_id = set_id;
+    std::vector<int> pow2;
+    if (!compute_consts(*s, c->host, pow2)) {
+        delete c;
+        return fail(MPRES_ERR_ARG, "moduli set %s: inconsistent RNS_MODULI_PRODUCT_LOG2", s->name);
+    }
+    for (int op = 0; op < OP_COUNT; op++) {
+        c->ops[op] = find_ops(s->n, default_lanes(s->n, op));
+        if (!c->ops[op]) {
+            delete c;
+            return fail(MPRES_ERR_UNSUPPORTED, "no kernels compiled for %d moduli", s->n);
+        }
+    }
+    if (hipMalloc(&c->dpow2, pow2.size() * sizeof(int)) != hipSuccess ||
+        hipMalloc(&c->dev, sizeof(DevConsts)) != hipSuccess) {
+        (void)hipFree(c->dpow2);
+        delete c;
+        return fail(MPRES_ERR_HIP, "allocating the constant tables failed");
+    }
+    c->host.pow2 = c->dpow2;
+    HIPCHK(hipMemcpy(c->dpow2, pow2.data(), pow2.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->dev, &c->host, sizeof(DevConsts), hipMemcpyHostToDevice));
+    *out = c;
+    return MPRES_OK;
+}
+
+int mpres_ctx_destroy(mpres_ctx_t *c) {
+    if (!c) return MPRES_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipDeviceSynchronize();
+    if (c->ws) (void)hipFree(c->ws);
+    (void)hipFree(c->dev);
+    (void)hipFree(c->dpow2);
+    delete c;
+    return MPRES_OK;
+}
+
+int mpres_ctx_info(const mpres_ctx_t *c, int *n, int *precision, int *mp_h, int *mp_j, int *log2m) {
+    if (!c) return fail(MPRES_ERR_ARG, "null context");
+    if (n) *n = c->host.n;
+    if (precision) *precision = c->host.mp_precision;
+    if (mp_h) *mp_h = c->host.mp_h;
+    if (mp_j) *mp_j = c->host.mp_j;
+    if (log2m) *log2m = c->host.log2m;
+    return MPRES_OK;
+}
+
+size_t mpres_mp_float_size(const mpres_ctx_t *c) { return c ? mpsize(c) : 0; }
+
+static int table_of(const DevConsts &h, const std::vector<int> &pow2, int what, void *out, size_t bytes) {
+    const int n = h.n;
+    std::vector<char> buf;
+    auto put = [&](const void *p, size_t sz) { buf.insert(buf.end(), (const char *)p, (const char *)p + sz); };
+    switch (what) {
+        case 0: put(h.m, n * sizeof(int)); break;
+        case 1: put(h.w, n * sizeof(int)); break;
+        case 2: put(pow2.data(), pow2.size() * sizeof(int)); break;
+        case 3: put(h.mpow2, sizeof h.mpow2); break;
+        case 4:
+            for (int j = 0; j < P2T; j++) put(h.h[j], n * sizeof(int));
+            break;
+        case 5:
+            for (int j = 0; j < P2T; j++) put(h.pow2inv[j], n * sizeof(int));
+            break;
+        case 6:
+            for (int j = 0; j < n; j++) put(h.mrcinv[j], n * sizeof(int));
+            break;
+        case 7:
+            put(h.rrd, n * sizeof(double));
+            put(h.rru, n * sizeof(double));
+            put(&h.acc, sizeof(double));
+            break;
+        case 8: {
+            const ErC *u[4] = {&h.unit_low, &h.unit_upp, &h.inv_unit_low, &h.inv_unit_upp};
+            for (auto *e : u) {
+                er_float_t v{e->frac, (long)e->exp};
+                put(&v, sizeof v);
+            }
+            break;
+        }
+        case 9: {
+            const int v[5] = {h.mp_precision, h.mp_h, h.mp_j, h.ref_factor, h.log2m};
+            put(v, sizeof v);
+            break;
+        }
+        default: return fail(MPRES_ERR_ARG, "unknown table %d", what);
+    }
+    if (bytes < buf.size()) return fail(MPRES_ERR_ARG, "table %d needs %zu bytes", what, buf.size());
+    std::memcpy(out, buf.data(), buf.size());
+    return MPRES_OK;
+}
+
+int mpres_ctx_table(const mpres_ctx_t *c, int what, void *out, size_t bytes) {
+    if (!c || !out) return fail(MPRES_ERR_ARG, "null argument");
+    std::vector<int> p2((size_t)(c->host.log2m + 1) * c->host.n);
+    if (hipMemcpy(p2.data(), c->dpow2, p2.size() * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(MPRES_ERR_HIP, "reading RNS_POW2");
+    return table_of(c->host, p2, what, out, bytes);
+}
+
+int mpres_moduli_set_table(int set_id, int what, void *out, size_t bytes) {
+    const ModuliSet *s = get_set(set_id);
+    if (!s || !out) return fail(MPRES_ERR_ARG, "invalid argument");
+    DevConsts h;
+    std::vector<int> p2;
+    if (!compute_consts(*s, h, p2)) return fail(MPRES_ERR_ARG, "inconsistent moduli set");
+    return table_of(h, p2, what, out, bytes);
+}
+
+// ---------------------------------------------------------------- arrays
+int mpres_array_init(mpres_ctx_t *c, mp_array_t *a, size_t size) {
+    if (!c || !a || size == 0) return fail(MPRES_ERR_ARG, "invalid array request");
+    std::memset(a, 0, sizeof *a);
+    const size_t n = (size_t)c->host.n;
+    HIPCHK(hipMalloc(&a->digits, n * size * sizeof(int)));
+    HIPCHK(hipMalloc(&a->sign, size * sizeof(int)));
+    HIPCHK(hipMalloc(&a->exp, size * sizeof(int)));
+    HIPCHK(hipMalloc(&a->eval, 2 * size * sizeof(er_float_t)));
+    HIPCHK(hipMalloc(&a->buf, size * sizeof(mpres_int4_t)));
+    HIPCHK(hipMalloc(&a->len, sizeof(int)));
+    const int len = (int)size;
+    HIPCHK(hipMemcpy(a->len, &len, sizeof(int), hipMemcpyHostToDevice));
+    return MPRES_OK;
+}
+
+int mpres_array_clear(mpres_ctx_t *c, mp_array_t *a) {
+    (void)c;
+    if (!a) return MPRES_OK;
+    (void)hipFree(a->digits);
+    (void)hipFree(a->sign);
+    (void)hipFree(a->exp);
+    (void)hipFree(a->eval);
+    (void)hipFree(a->buf);
+    (void)hipFree(a->len);
+    std::memset(a, 0, sizeof *a);
+    return MPRES_OK;
+}
+
+int mpres_array_host2device(mpres_ctx_t *c, mp_array_t *dst, const void *src, size_t size, void *st) {
+    if (!c || !dst || (!src && size)) return fail(MPRES_ERR_ARG, "null argument");
+    const int n = c->host.n;
+    const size_t S = mpsize(c);
+    std::vector<int> dig(size * n), sg(size), ex(size);
+    std::vector<er_float_t> ev(2 * size);
+    const char *p = (const char *)src;
+    for (size_t i = 0; i < size; i++) {
+        std::memcpy(&dig[i * n], p + i * S, n * sizeof(int));
+        std::memcpy(&sg[i], p + i * S + 4 * n, sizeof(int));
+        std::memcpy(&ex[i], p + i * S + 4 * n + 4, sizeof(int));
+        std::memcpy(&ev[i], p + i * S + 4 * n + 8, sizeof(er_float_t));
+        std::memcpy(&ev[size + i], p + i * S + 4 * n + 24, sizeof(er_float_t));
+    }
+    // upper bounds live at eval + len; len is the array's actual length
+    int len = 0;
+    HIPCHK(hipMemcpy(&len, dst->len, sizeof(int), hipMemcpyDeviceToHost));
+    if ((size_t)len < size) return fail(MPRES_ERR_ARG, "array of length %d, %zu elements given", len, size);
+    HIPCHK(hipMemcpyAsync(dst->digits, dig.data(), dig.size() * sizeof(int), hipMemcpyHostToDevice, strm(st)));
+    HIPCHK(hipMemcpyAsync(dst->sign, sg.data(), size * sizeof(int), hipMemcpyHostToDevice, strm(st)));
+    HIPCHK(hipMemcpyAsync(dst->exp, ex.data(), size * sizeof(int), hipMemcpyHostToDevice, strm(st)));
+    HIPCHK(hipMemcpyAsync(dst->eval, ev.data(), size * sizeof(er_float_t), hipMemcpyHostToDevice, strm(st)));
+    HIPCHK(hipMemcpyAsync(dst->eval + len, ev.data() + size, size * sizeof(er_float_t), hipMemcpyHostToDevice, strm(st)));
+    HIPCHK(hipStreamSynchronize(strm(st)));
+    return MPRES_OK;
+}
+
+int mpres_array_device2host(mpres_ctx_t *c, void *dst, const mp_array_t *src, size_t size, void *st) {
+    if (!c || !src || (!dst && size)) return fail(MPRES_ERR_ARG, "null argument");
+    const int n = c->host.n;
+    const size_t S = mpsize(c);
+    int len = 0;
+    HIPCHK(hipMemcpy(&len, src->len, sizeof(int), hipMemcpyDeviceToHost));
+    if ((size_t)len < size) return fail(MPRES_ERR_ARG, "array of length %d, %zu elements asked", len, size);
+    std::vector<int> dig(size * n), sg(size), ex(size);
+    std::vector<er_float_t> ev(2 * size);
+    HIPCHK(hipMemcpyAsync(dig.data(), src->digits, dig.size() * sizeof(int), hipMemcpyDeviceToHost, strm(st)));
+    HIPCHK(hipMemcpyAsync(sg.data(), src->sign, size * sizeof(int), hipMemcpyDeviceToHost, strm(st)));
+    HIPCHK(hipMemcpyAsync(ex.data(), src->exp, size * sizeof(int), hipMemcpyDeviceToHost, strm(st)));
+    HIPCHK(hipMemcpyAsync(ev.data(), src->eval, size * sizeof(er_float_t), hipMemcpyDeviceToHost, strm(st)));
+    HIPCHK(hipMemcpyAsync(ev.data() + size, src->eval + len, size * sizeof(er_float_t), hipMemcpyDeviceToHost, strm(st)));
+    HIPCHK(hipStreamSynchronize(strm(st)));
+    char *p = (char *)dst;
+    for (size_t i = 0; i < size; i++) {
+        std::memcpy(p + i * S, &dig[i * n], n * sizeof(int));
+        std::memcpy(p + i * S + 4 * n, &sg[i], sizeof(int));
+        std::memcpy(p + i * S + 4 * n + 4, &ex[i], sizeof(int));
+        std::memcpy(p + i * S + 4 * n + 8, &ev[i], sizeof(er_float_t));
+        std::memcpy(p + i * S + 4 * n + 24, &ev[size + i], sizeof(er_float_t));
+    }
+    return MPRES_OK;
+}
+
+int mpres_array_set_d(mpres_ctx_t *c, mp_array_t *dst, const double *x, int count, void *st) {
+    if (!c || !dst || (!x && count > 0)) return fail(MPRES_ERR_ARG, "null argument");
+    if (count <= 0) return MPRES_OK;
+    HIPCHK(c->ops[OP_ELEM]->set_d(c->dev, x, count, soa_of(dst), nullptr, true, strm(st)));
+    return MPRES_OK;
+}
+
+int mpres_set_d(mpres_ctx_t *c, void *dst, const double *x, int count, void *st) {
+    if (!c || (!dst && count > 0) || (!x && count > 0)) return fail(MPRES_ERR_ARG, "null argument");
+    if (count <= 0) return MPRES_OK;
+    SoA none{};
+    HIPCHK(c->ops[OP_ELEM]->set_d(c->dev, x, count, none, (char *)dst, false, strm(st)));
+    return MPRES_OK;
+}
+
+int mpres_aos_to_array(mpres_ctx_t *c, mp_array_t *dst, const void *src, int count, void *st) {
+    if (!c || !dst || (!src && count > 0)) return fail(MPRES_ERR_ARG, "null argument");
+    HIPCHK(c->ops[OP_ELEM]->aos_to_soa(c->dev, (const char *)src, soa_of(dst), count, strm(st)));
+    return MPRES_OK;
+}
+
+int mpres_array_to_aos(mpres_ctx_t *c, void *dst, const mp_array_t *src, int count, void *st) {
+    if (!c || !src || (!dst && count > 0)) return fail(MPRES_ERR_ARG, "null argument");
+    HIPCHK(c->ops[OP_ELEM]->soa_to_aos(c->dev, soa_of(src), (char *)dst, count, strm(st)));
+    return MPRES_OK;
+}
+
+int mpres_elementwise(mpres_ctx_t *c, int op, int count, const void *x, const void *y, void *r, void *st) {
+    if (!c || op < 0 || op > 4) return fail(MPRES_ERR_ARG, "invalid operation %d", op);
+    if (count <= 0) return MPRES_OK;
+    if (!x || !y || !r) return fail(MPRES_ERR_ARG, "null argument");
+    HIPCHK(c->ops[OP_ELEM]->elem(c->dev, op, count, (const char *)x, (const char *)y, (const double *)y, (char *)r,
+                                 strm(st)));
+    return MPRES_OK;
+}
+
+// ---------------------------------------------------------------- dot
+size_t mpres_dot_workspace(const mpres_ctx_t *c, int n, int g3, int b3) {
+    (void)n;
+    if (!c || g3 <= 0 || b3 <= 0) return 0;
+    return mpsize(c) * ((size_t)g3 * b3 + (size_t)g3 + (size_t)b3 + 1);
+}
+
+static int dot_common(mpres_ctx_t *c, int mode, int n, SoA xs, int incx, SoA ys, int incy, const char *xa,
+                      const char *ya, int g3, int b3, void *ws, size_t wsb, hipStream_t st, char **result) {
+    const size_t S = mpsize(c);
+    char *w;
+    int rc = get_ws(c, ws, wsb, mpres_dot_workspace(c, n, g3, b3), &w);
+    if (rc) return rc;
+    const long long G = (long long)g3 * b3;
+    char *p1 = w, *p2 = p1 + S * G, *p3 = p2 + S * g3, *p4 = p3 + S * b3;
+    const Ops *o = c->ops[OP_DOT];
+    const unsigned P = next_pow2((unsigned)b3);
+    // pass 1: strided fold over the products, halving tree per block of b3 (src/mpreduct.cuh:38-74)
+    HIPCHK(o->fold(c->dev, mode, n, G, xs, incx, ys, incy, xa, ya, p1, st));
+    HIPCHK(o->tree(c->dev, p1, b3, P, g3, p2, st));
+    // pass 2: one block of b3 over the g3 block results (src/mpreduct.cuh:81-112)
+    SoA none{};
+    HIPCHK(o->fold(c->dev, 2, g3, b3, none, 1, none, 1, p2, nullptr, p3, st));
+    HIPCHK(o->tree(c->dev, p3, b3, P, 1, p4, st));
+    *result = p4;
+    return MPRES_OK;
+}
+
+int mpres_dot(mpres_ctx_t *c, int n, const mp_array_t *x, int incx, const mp_array_t *y, int incy, mp_array_t *r,
+              int g3, int b3, void *ws, size_t wsb, void *st) {
+    if (!c || !x || !y || !r) return fail(MPRES_ERR_ARG, "null argument");
+    if (n <= 0) return MPRES_OK;  // src/blas/dot.cuh:88-90
+    if (g3 <= 0 || b3 <= 0) return fail(MPRES_ERR_ARG, "invalid reduction shape %d x %d", g3, b3);
+    char *res;
+    int rc = dot_common(c, 0, n, soa_of(x), incx, soa_of(y), incy, nullptr, nullptr, g3, b3, ws, wsb, strm(st), &res);
+    if (rc) return rc;
+    HIPCHK(c->ops[OP_DOT]->aos_to_soa(c->dev, res, soa_of(r), 1, strm(st)));
+    return MPRES_OK;
+}
+
+size_t mpres_dot_v2_workspace(const mpres_ctx_t *c, int n, int blocks, int threads) {
+    return mpres_dot_workspace(c, n, blocks, threads);
+}
+
+int mpres_dot_v2(mpres_ctx_t *c, int n, const void *x, const void *y, void *r, int blocks, int threads, void *ws,
+                 size_t wsb, void *st) {
+    if (!c || !x || !y || !r) return fail(MPRES_ERR_ARG, "null argument");
+    if (n <= 0) return MPRES_OK;
+    if (blocks <= 0 || threads <= 0) return fail(MPRES_ERR_ARG, "invalid launch shape %d x %d", blocks, threads);
+    char *res;
+    SoA none{};
+    int rc = dot_common(c, 1, n, none, 1, none, 1, (const char *)x, (const char *)y, blocks, threads, ws, wsb, strm(st),
+                        &res);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(r, res, mpsize(c), hipMemcpyDeviceToDevice, strm(st)));
+    return MPRES_OK;
+}
+
+// ---------------------------------------------------------------- axpy
+int mpres_axpy(mpres_ctx_t *c, int n, const mp_array_t *alpha, const mp_array_t *x, int incx, mp_array_t *y, int incy,
+               void *st) {
+    if (!c || !alpha || !x || !y) return fail(MPRES_ERR_ARG, "null argument");
+    if (n <= 0) return MPRES_OK;  // src/blas/axpy.cuh:49-51
+    HIPCHK(c->ops[OP_AXPY]->axpy(c->dev, n, soa_of(alpha), soa_of(x), incx, soa_of(y), incy, strm(st)));
+    return MPRES_OK;
+}
+
+int mpres_axpy_v2(mpres_ctx_t *c, int n, const void *alpha, const void *x, const void *y, void *r, void *st) {
+    if (!c || !alpha || !x || !y || !r) return fail(MPRES_ERR_ARG, "null argument");
+    if (n <= 0) return MPRES_OK;
+    HIPCHK(c->ops[OP_AXPY]->axpy_v2(c->dev, n, (const char *)alpha, (const char *)x, (const char *)y, (char *)r,
+                                     strm(st)));
+    return MPRES_OK;
+}
+
+// ---------------------------------------------------------------- gemv
+size_t mpres_gemv_workspace(const mpres_ctx_t *c, int trans, int m, int n, int b3) {
+    if (!c || m <= 0 || n <= 0 || b3 <= 0) return 0;
+    const bool tr = trans != MPRES_NO_TRANS;
+    const size_t lx = tr ? m : n, ly = tr ? n : m;
+    return mpsize(c) * (lx + ly * (size_t)b3 + ly);
+}
+
+int mpres_gemv(mpres_ctx_t *c, int trans, int m, int n, const mp_array_t *alpha, const mp_array_t *A, int lda,
+               const mp_array_t *x, int incx, const mp_array_t *beta, mp_array_t *y, int incy, int b3, void *ws,
+               size_t wsb, void *st) {
+    if (!c || !alpha || !A || !x || !beta || !y) return fail(MPRES_ERR_ARG, "null argument");
+    if (m <= 0 || n <= 0) return MPRES_OK;  // src/blas/gemv.cuh:155-157
+    if (incx == 0 || incy == 0 || lda < std::max(1, m))
+        return fail(MPRES_ERR_ARG, "invalid increments or leading dimension");  // :159-161
+    if (b3 <= 0) return fail(MPRES_ERR_ARG, "invalid blockDim3 %d", b3);
+    if (trans != MPRES_NO_TRANS && trans != MPRES_TRANS && trans != MPRES_CONJ_TRANS)
+        return fail(MPRES_ERR_ARG, "invalid trans %d", trans);
+    const bool tr = trans != MPRES_NO_TRANS;
+    const int lx = tr ? m : n, ly = tr ? n : m;
+    const size_t S = mpsize(c);
+    char *w;
+    int rc = get_ws(c, ws, wsb, mpres_gemv_workspace(c, trans, m, n, b3), &w);
+    if (rc) return rc;
+    char *bx = w, *part = bx + S * lx, *sums = part + S * (size_t)ly * b3;
+    const Ops *o = c->ops[OP_GEMV];
+    SoA ys = soa_of(y);
+    // buffer1 = alpha * x, y = beta * y (src/blas/gemv.cuh:175-190)
+    HIPCHK(o->scal(c->dev, lx, soa_of(alpha), soa_of(x), incx, bx, false, strm(st)));
+    HIPCHK(o->scal(c->dev, ly, soa_of(beta), ys, incy, nullptr, true, strm(st)));
+    // per-output strided sums of A .* buffer1 and the halving tree (:199-218, :44-76)
+    HIPCHK(o->gemv_fold(c->dev, tr, m, n, soa_of(A), lda, bx, b3, (long long)ly * b3, part, strm(st)));
+    HIPCHK(o->tree(c->dev, part, b3, next_pow2((unsigned)b3), ly, sums, strm(st)));
+    HIPCHK(o->gemv_final(c->dev, ly, ys, incy, sums, strm(st)));
+    return MPRES_OK;
+}
+
+// ---------------------------------------------------------------- gemm
+size_t mpres_gemm_workspace(const mpres_ctx_t *c, int m, int n, int k) {
+    (void)c;
+    (void)m;
+    (void)n;
+    (void)k;
+    return 0;
+}
+
+int mpres_gemm(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const mp_array_t *alpha, const mp_array_t *A,
+               int lda, const mp_array_t *B, int ldb, const mp_array_t *beta, mp_array_t *C, int ldc, void *ws,
+               size_t wsb, void *st) {
+    (void)ws;
+    (void)wsb;
+    if (!c || !alpha || !A || !B || !beta || !C) return fail(MPRES_ERR_ARG, "null argument");
+    if (m <= 0 || n <= 0 || k <= 0) return MPRES_OK;  // src/blas/gemm.cuh:75-77
+    if ((ta == MPRES_NO_TRANS && lda < std::max(1, m)) || (ta != MPRES_NO_TRANS && lda < std::max(1, k)) ||
+        (tb == MPRES_NO_TRANS && ldb < std::max(1, k)) || (tb != MPRES_NO_TRANS && ldb < std::max(1, n)) ||
+        ldc < std::max(1, m))
+        return fail(MPRES_ERR_ARG, "invalid leading dimension");  // :82-96
+    if (ta != MPRES_NO_TRANS || tb != MPRES_NO_TRANS)
+        return fail(MPRES_ERR_UNSUPPORTED, "only non-transposed matrices are supported");  // :114-125
+    HIPCHK(c->ops[OP_GEMM]->gemm(c->dev, m, n, k, soa_of(alpha), soa_of(A), lda, soa_of(B), ldb, soa_of(beta),
+                                 soa_of(C), ldc, strm(st)));
+    return MPRES_OK;
+}
+
+int mpres_gemm_v2(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const void *alpha, const void *A, int lda,
+                  const void *B, int ldb, const void *beta, void *C, int ldc, void *st) {
+    if (!c || !alpha || !A || !B || !beta || !C) return fail(MPRES_ERR_ARG, "null argument");
+    if (m <= 0 || n <= 0) return MPRES_OK;
+    if (k < 0 || lda <= 0 || ldb <= 0 || ldc < std::max(1, m)) return fail(MPRES_ERR_ARG, "invalid dimensions");
+    HIPCHK(c->ops[OP_GEMM]->gemm_v2(c->dev, ta == MPRES_TRANS || ta == MPRES_CONJ_TRANS,
+                                    tb == MPRES_TRANS || tb == MPRES_CONJ_TRANS, m, n, k, (const char *)alpha,
+                                    (const char *)A, lda, (const char *)B, ldb, (const char *)beta, (char *)C, ldc,
+                                    strm(st)));
+    return MPRES_OK;
+}
+
+// ---------------------------------------------------------------- SpMV
+int mpres_spmv_csrv(mpres_ctx_t *c, int m, const csr_t *csr, int tpr, const void *x, void *y, void *st) {
+    if (!c || !csr || !x || !y) return fail(MPRES_ERR_ARG, "null argument");
+    if (m <= 0) return MPRES_OK;
+    if (tpr < 1 || tpr > 32 || (tpr & (tpr - 1))) return fail(MPRES_ERR_ARG, "threadsPerRow %d not in 1..32", tpr);
+    const Ops *o = c->ops[OP_SPMV];
+    if (tpr > 256 / o->L) {
+        // the tree of a row must fit one workgroup: fall back to a narrower group
+        for (int L = o->L / 2; L >= 1; L /= 2) {
+            const Ops *q = find_ops(c->host.n, L);
+            if (q && tpr <= 256 / L) {
+                o = q;
+                break;
+            }
+        }
+        if (tpr > 256 / o->L) return fail(MPRES_ERR_UNSUPPORTED, "threadsPerRow %d with %d lanes", tpr, o->L);
+    }
+    HIPCHK(o->spmv(c->dev, m, csr->irp, csr->ja, csr->as, (const char *)x, (char *)y, tpr, strm(st)));
+    return MPRES_OK;
+}
+
+int mpres_spmv_csr(mpres_ctx_t *c, int m, const csr_t *csr, const void *x, void *y, void *st) {
+    return mpres_spmv_csrv(c, m, csr, 1, x, y, st);
+}
+
+// ---------------------------------------------------------------- tuning
+int mpres_lanes_supported(const mpres_ctx_t *c, int lanes) {
+    return (c && find_ops(c->host.n, lanes) != nullptr) ? 1 : 0;
+}
+
+int mpres_set_lanes(mpres_ctx_t *c, int op, int lanes) {
+    if (!c || op < 0 || op >= OP_COUNT) return fail(MPRES_ERR_ARG, "invalid operation %d", op);
+    const Ops *o = find_ops(c->host.n, lanes == 0 ? default_lanes(c->host.n, op) : lanes);
+    if (!o) return fail(MPRES_ERR_UNSUPPORTED, "%d lanes per number not compiled for %d moduli", lanes, c->host.n);
+    c->ops[op] = o;
+    return MPRES_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,450 @@
+// Kernel templates of the multiple-precision path, parameterised by the
+// number of moduli N and the lanes per number L (see group.hpp).
+//
+// Thread mapping: blocks of BLK threads; every L consecutive lanes form one
+// group, i.e. one multiple-precision number. The reference's "one thread per
+// element" launch shapes are replaced by this mapping; the reference's launch
+// parameters survive only where they fix a summation order (dot: gridDim3 x
+// blockDim3, gemv: blockDim3, SpMV CSR-vector: threadsPerRow), and there the
+// same order is reproduced.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "group.hpp"
+#include "ops.hpp"
+
+namespace mph {
+
+constexpr int BLK = 256;
+
+
+// mp_float_t arrays (array of structures, 4N+40 bytes per element)
+template <int N>
+struct AoSRef {
+    static constexpr int STRIDE = 4 * N + 40;
+    char *base;
+    __device__ __forceinline__ int *digits(long long i) const { return (int *)(base + i * STRIDE); }
+    __device__ __forceinline__ int *sign(long long i) const { return (int *)(base + i * STRIDE + 4 * N); }
+    __device__ __forceinline__ int *exp(long long i) const { return (int *)(base + i * STRIDE + 4 * N + 4); }
+    __device__ __forceinline__ ErMem *eval(long long i) const { return (ErMem *)(base + i * STRIDE + 4 * N + 8); }
+};
+
+template <int N, int L>
+__device__ __forceinline__ void load_soa(const Grp<N, L> &g, MP<N / L> &a, const SoA &v, long long i) {
+    constexpr int R = N / L;
+    const int *p = v.d + i * N + g.lane * R;
+#pragma unroll
+    for (int r = 0; r < R; r++) a.d[r] = p[r];
+    a.sign = v.s[i];
+    a.exp = v.e[i];
+    const ErMem lo = v.ev[i], up = v.ev[i + v.len];
+    a.ev0 = Er{lo.frac, (int)lo.exp};
+    a.ev1 = Er{up.frac, (int)up.exp};
+}
+template <int N, int L>
+__device__ __forceinline__ void store_soa(const Grp<N, L> &g, const MP<N / L> &a, const SoA &v, long long i) {
+    constexpr int R = N / L;
+    int *p = v.d + i * N + g.lane * R;
+#pragma unroll
+    for (int r = 0; r < R; r++) p[r] = a.d[r];
+    if (g.lane == 0) {
+        v.s[i] = a.sign;
+        v.e[i] = a.exp;
+        v.ev[i] = ErMem{a.ev0.frac, (long long)a.ev0.exp};
+        v.ev[i + v.len] = ErMem{a.ev1.frac, (long long)a.ev1.exp};
+    }
+}
+template <int N, int L>
+__device__ __forceinline__ void load_aos(const Grp<N, L> &g, MP<N / L> &a, const AoSRef<N> &v, long long i) {
+    constexpr int R = N / L;
+    const int *p = v.digits(i) + g.lane * R;
+#pragma unroll
+    for (int r = 0; r < R; r++) a.d[r] = p[r];
+    a.sign = *v.sign(i);
+    a.exp = *v.exp(i);
+    const ErMem *e = v.eval(i);
+    const ErMem lo = e[0], up = e[1];
+    a.ev0 = Er{lo.frac, (int)lo.exp};
+    a.ev1 = Er{up.frac, (int)up.exp};
+}
+template <int N, int L>
+__device__ __forceinline__ void store_aos(const Grp<N, L> &g, const MP<N / L> &a, const AoSRef<N> &v, long long i) {
+    constexpr int R = N / L;
+    int *p = v.digits(i) + g.lane * R;
+#pragma unroll
+    for (int r = 0; r < R; r++) p[r] = a.d[r];
+    if (g.lane == 0) {
+        *v.sign(i) = a.sign;
+        *v.exp(i) = a.exp;
+        ErMem *e = v.eval(i);
+        e[0] = ErMem{a.ev0.frac, (long long)a.ev0.exp};
+        e[1] = ErMem{a.ev1.frac, (long long)a.ev1.exp};
+    }
+}
+
+// BLAS index of element k of an n-vector with increment inc (e.g. src/mpvector.cuh:65-67)
+__device__ __forceinline__ long long bidx(long long k, long long n, int inc) {
+    return inc > 0 ? k * inc : (-n + k + 1) * (long long)inc;
+}
+
+// read the device-side length of an mp_array_t argument
+__device__ __forceinline__ SoA rd(SoA v) {
+    v.len = (v.lenp != nullptr) ? *v.lenp : v.len;
+    return v;
+}
+
+template <int L>
+__device__ __forceinline__ long long group_id() {
+    return ((long long)blockIdx.x * BLK + threadIdx.x) / L;
+}
+
+// ---------------------------------------------------------------- conversion
+// mp_set_d for count doubles; out either SoA (soa_out) or AoS
+template <int N, int L, bool SOA_OUT>
+__global__ __launch_bounds__(BLK) void k_set_d(const DevConsts *__restrict__ C, const double *__restrict__ x, int count,
+                                               SoA so, AoSRef<N> ao) {
+    so = rd(so);
+    const long long i = group_id<L>();
+    if (i >= count) return;
+    Grp<N, L> g(C);
+    MP<N / L> a;
+    g.set_d(a, x[i]);
+    if (SOA_OUT)
+        store_soa(g, a, so, i);
+    else
+        store_aos(g, a, ao, i);
+}
+
+// ---------------------------------------------------------------- element-wise
+template <int N, int L, int OP>
+__global__ __launch_bounds__(BLK) void k_elem(const DevConsts *__restrict__ C, int count, AoSRef<N> x, AoSRef<N> y,
+                                              const double *__restrict__ yd, AoSRef<N> r) {
+    const long long i = group_id<L>();
+    if (i >= count) return;
+    Grp<N, L> g(C);
+    MP<N / L> a, b, c;
+    load_aos(g, a, x, i);
+    if (OP == 3) {
+        g.mul_d(c, a, yd[i]);
+    } else {
+        load_aos(g, b, y, i);
+        if (OP == 0) g.add(c, a, b);
+        if (OP == 1) {
+            b.sign ^= 1;  // mp_sub, src/arith/sub.cuh:41-53
+            g.add(c, a, b);
+        }
+        if (OP == 2) g.mul(c, a, b);
+        if (OP == 4) g.vadd(c, a, b);
+    }
+    store_aos(g, c, r, i);
+}
+
+// ---------------------------------------------------------------- reductions
+// Strided fold of the reduction kernels (src/mpreduct.cuh:38-56, dot_v2.cuh:37-55):
+// virtual thread v accumulates terms v, v + nvt, v + 2 nvt, ... from zero.
+// MODE 0: terms are x_i * y_i with SoA x, y and BLAS strides (v1 dot)
+// MODE 1: terms are x_i * y_i with AoS x, y (v2 dot)
+// MODE 2: terms are AoS partials t_i
+template <int N, int L, int MODE>
+__global__ __launch_bounds__(BLK) void k_fold(const DevConsts *__restrict__ C, long long cnt, long long nvt, SoA xs,
+                                              int incx, SoA ys, int incy, AoSRef<N> xa, AoSRef<N> ya, AoSRef<N> out) {
+    xs = rd(xs);
+    ys = rd(ys);
+    const long long v = group_id<L>();
+    if (v >= nvt) return;
+    Grp<N, L> g(C);
+    MP<N / L> acc, a, b, p;
+    g.zero(acc);
+    for (long long i = v; i < cnt; i += nvt) {
+        if (MODE == 0) {
+            load_soa(g, a, xs, bidx(i, cnt, incx));
+            load_soa(g, b, ys, bidx(i, cnt, incy));
+            g.mul(p, a, b);
+        } else if (MODE == 1) {
+            load_aos(g, a, xa, i);
+            load_aos(g, b, ya, i);
+            g.mul(p, a, b);
+        } else {
+            load_aos(g, p, xa, i);
+        }
+        g.add(acc, acc, p);
+    }
+    store_aos(g, acc, out, v);
+}
+
+// Halving tree of src/mpreduct.cuh:60-68 over each block of b partials:
+// for i = nextPow2(b)/2 .. 1: s[t] = s[t] + s[t+i] for t < i, t + i < b.
+// One workgroup per block; the block's partials stay in (cached) global memory.
+template <int N, int L>
+__global__ __launch_bounds__(BLK) void k_tree(const DevConsts *__restrict__ C, AoSRef<N> part, int b, unsigned p2,
+                                              AoSRef<N> out) {
+    const long long base = (long long)blockIdx.x * b;
+    const int W = BLK / L;
+    const int gl = threadIdx.x / L;
+    Grp<N, L> g(C);
+    for (unsigned s = p2 >> 1; s >= 1; s >>= 1) {
+        for (int t = gl; t < (int)s; t += W) {
+            if (t + (int)s < b) {
+                MP<N / L> a, c, r;
+                load_aos(g, a, part, base + t);
+                load_aos(g, c, part, base + t + s);
+                g.add(r, a, c);
+                store_aos(g, r, part, base + t);
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+    }
+    if (gl == 0) {
+        MP<N / L> a;
+        load_aos(g, a, part, base);
+        store_aos(g, a, out, blockIdx.x);
+    }
+}
+
+// AoS -> SoA copy of count numbers (also used to write reduction results into mp_array_t)
+template <int N, int L>
+__global__ __launch_bounds__(BLK) void k_aos_to_soa(const DevConsts *__restrict__ C, AoSRef<N> src, SoA dst, int count) {
+    dst = rd(dst);
+    const long long i = group_id<L>();
+    if (i >= count) return;
+    Grp<N, L> g(C);
+    MP<N / L> a;
+    load_aos(g, a, src, i);
+    store_soa(g, a, dst, i);
+}
+template <int N, int L>
+__global__ __launch_bounds__(BLK) void k_soa_to_aos(const DevConsts *__restrict__ C, SoA src, AoSRef<N> dst, int count) {
+    src = rd(src);
+    const long long i = group_id<L>();
+    if (i >= count) return;
+    Grp<N, L> g(C);
+    MP<N / L> a;
+    load_soa(g, a, src, i);
+    store_aos(g, a, dst, i);
+}
+
+// ---------------------------------------------------------------- axpy
+// v1: buffer = x * alpha (+round), y = buffer (+) y with the v1 vector add (+round),
+// src/blas/axpy.cuh:62-72 fused into one pass
+template <int N, int L>
+__global__ __launch_bounds__(BLK) void k_axpy(const DevConsts *__restrict__ C, int n, SoA alpha, SoA x, int incx, SoA y,
+                                              int incy) {
+    alpha = rd(alpha);
+    x = rd(x);
+    y = rd(y);
+    const long long i = group_id<L>();
+    if (i >= n) return;
+    Grp<N, L> g(C);
+    MP<N / L> a, xv, yv, b, r;
+    load_soa(g, a, alpha, 0);
+    load_soa(g, xv, x, bidx(i, n, incx));
+    const long long iy = bidx(i, n, incy);
+    load_soa(g, yv, y, iy);
+    g.mul(b, xv, a);
+    g.vadd(r, b, yv);
+    store_soa(g, r, y, iy);
+}
+// v2: r = alpha * x (+) y, src/blas/v2/axpy_v2.cuh:39-52
+template <int N, int L>
+__global__ __launch_bounds__(BLK) void k_axpy_v2(const DevConsts *__restrict__ C, int n, AoSRef<N> alpha, AoSRef<N> x,
+                                                 AoSRef<N> y, AoSRef<N> r) {
+    const long long i = group_id<L>();
+    if (i >= n) return;
+    Grp<N, L> g(C);
+    MP<N / L> a, xv, yv, ax, s;
+    load_aos(g, a, alpha, 0);
+    load_aos(g, xv, x, i);
+    load_aos(g, yv, y, i);
+    g.mul(ax, a, xv);
+    g.add(s, ax, yv);
+    store_aos(g, s, r, i);
+}
+
+// ---------------------------------------------------------------- gemv
+// out[k] = v[k] * s (vec2scal esi/digits/round, src/mpvector.cuh:139-216), v with BLAS stride;
+// out is AoS (ws) or the SoA v itself (in place)
+template <int N, int L, bool INPLACE>
+__global__ __launch_bounds__(BLK) void k_scal(const DevConsts *__restrict__ C, int len, SoA s, SoA v, int inc,
+                                              AoSRef<N> out) {
+    s = rd(s);
+    v = rd(v);
+    const long long k = group_id<L>();
+    if (k >= len) return;
+    Grp<N, L> g(C);
+    MP<N / L> a, x, r;
+    load_soa(g, a, s, 0);
+    const long long iv = bidx(k, len, inc);
+    load_soa(g, x, v, iv);
+    g.mul(r, x, a);
+    if (INPLACE)
+        store_soa(g, r, v, iv);
+    else
+        store_aos(g, r, out, k);
+}
+// virtual thread (o, t): o-th output, t-th of b3 strided partial sums of A-entries times bx
+// (matrix_row_sum_kernel / matrix_col_sum_kernel, src/blas/gemv.cuh:44-120, with the
+// products of the mat2vec scaling kernels formed on the fly)
+template <int N, int L, bool TRANS>
+__global__ __launch_bounds__(BLK) void k_gemv_fold(const DevConsts *__restrict__ C, int m, int n, SoA A, int lda,
+                                                   AoSRef<N> bx, int b3, long long nvt, AoSRef<N> part) {
+    A = rd(A);
+    const long long v = group_id<L>();
+    if (v >= nvt) return;
+    const long long o = v / b3;
+    const int t = (int)(v % b3);
+    Grp<N, L> g(C);
+    MP<N / L> acc, a, xb, p;
+    g.zero(acc);
+    const int len = TRANS ? m : n;
+    for (int j = t; j < len; j += b3) {
+        load_soa(g, a, A, TRANS ? (o * lda + j) : ((long long)j * lda + o));
+        load_aos(g, xb, bx, j);
+        g.mul(p, a, xb);
+        g.add(acc, acc, p);
+    }
+    store_aos(g, acc, part, v);
+}
+// y[o] = y[o] (+) rowsum[o]  (src/blas/gemv.cuh:72-75 with mp_add rounding, add.cuh:223-248)
+template <int N, int L>
+__global__ __launch_bounds__(BLK) void k_gemv_final(const DevConsts *__restrict__ C, int len, SoA y, int incy,
+                                                    AoSRef<N> sums) {
+    y = rd(y);
+    const long long o = group_id<L>();
+    if (o >= len) return;
+    Grp<N, L> g(C);
+    MP<N / L> a, s, r;
+    const long long iy = bidx(o, len, incy);
+    load_soa(g, a, y, iy);
+    load_aos(g, s, sums, o);
+    g.add(r, a, s);
+    store_soa(g, r, y, iy);
+}
+
+// ---------------------------------------------------------------- gemm
+// v1 (src/blas/gemm.cuh:39-58 then the element-wise epilogue :127-166), fused per entry:
+// sum_k A*B in order, buffer = sum*alpha (+round), C = C*beta (+round), C = C (+) buffer (v1 add, +round)
+template <int N, int L>
+__global__ __launch_bounds__(BLK) void k_gemm(const DevConsts *__restrict__ C_, int m, int n, int k, SoA alpha, SoA A,
+                                              int lda, SoA B, int ldb, SoA beta, SoA Cm, int ldc) {
+    alpha = rd(alpha);
+    A = rd(A);
+    B = rd(B);
+    beta = rd(beta);
+    Cm = rd(Cm);
+    const long long e = group_id<L>();
+    if (e >= (long long)m * n) return;
+    const int row = (int)(e % m), col = (int)(e / m);
+    Grp<N, L> g(C_);
+    MP<N / L> sum, a, b, p;
+    g.zero(sum);
+    for (int i = 0; i < k; i++) {
+        load_soa(g, a, A, (long long)lda * i + row);
+        load_soa(g, b, B, (long long)col * ldb + i);
+        g.mul(p, a, b);
+        g.add(sum, sum, p);
+    }
+    MP<N / L> al, be, cv, s2, c2, r;
+    load_soa(g, al, alpha, 0);
+    load_soa(g, be, beta, 0);
+    const long long ic = (long long)col * ldc + row;
+    load_soa(g, cv, Cm, ic);
+    g.mul(s2, sum, al);
+    g.mul(c2, cv, be);
+    g.vadd(r, c2, s2);
+    store_soa(g, r, Cm, ic);
+}
+// v2 kernel (src/blas/v2/gemm_v2.cuh:55-81): C = (sum*alpha) (+) (C*beta)
+template <int N, int L>
+__global__ __launch_bounds__(BLK) void k_gemm_v2(const DevConsts *__restrict__ C_, int ta, int tb, int m, int n, int k,
+                                                 AoSRef<N> alpha, AoSRef<N> A, int lda, AoSRef<N> B, int ldb,
+                                                 AoSRef<N> beta, AoSRef<N> Cm, int ldc) {
+    const long long e = group_id<L>();
+    if (e >= (long long)m * n) return;
+    const int row = (int)(e % m), col = (int)(e / m);
+    Grp<N, L> g(C_);
+    MP<N / L> sum, a, b, p;
+    g.zero(sum);
+    for (int i = 0; i < k; i++) {
+        const long long ia = ta ? (long long)i + (long long)lda * row : (long long)row + (long long)lda * i;
+        const long long ib = tb ? (long long)col + (long long)ldb * i : (long long)i + (long long)ldb * col;
+        load_aos(g, a, A, ia);
+        load_aos(g, b, B, ib);
+        g.mul(p, a, b);
+        g.add(sum, sum, p);
+    }
+    MP<N / L> al, be, cv, s2, c2, r;
+    load_aos(g, al, alpha, 0);
+    load_aos(g, be, beta, 0);
+    const long long ic = (long long)row + (long long)col * ldc;
+    load_aos(g, cv, Cm, ic);
+    g.mul(s2, sum, al);
+    g.mul(c2, cv, be);
+    g.add(r, s2, c2);
+    store_aos(g, r, Cm, ic);
+}
+
+// ---------------------------------------------------------------- SpMV
+// Row partial sums of tpr strided lanes (src/sparse/spmv/spmv_csrv.cuh:61-65; tpr = 1
+// is the scalar kernel spmv_csr.cuh:53-62), combined by the 16/8/4/2/1 tree
+// (:66-80) through LDS. One workgroup holds BLK/L groups = (BLK/L)/tpr rows.
+template <int N, int L>
+__global__ __launch_bounds__(BLK) void k_spmv(const DevConsts *__restrict__ C, int m, const int *__restrict__ irp,
+                                              const int *__restrict__ ja, const double *__restrict__ as, AoSRef<N> x,
+                                              AoSRef<N> y, int tpr) {
+    constexpr int R = N / L;
+    constexpr int W = BLK / L;
+    __shared__ int sd[W][N];
+    __shared__ int ss[W], se[W];
+    __shared__ double sf0[W], sf1[W];
+    __shared__ int sx0[W], sx1[W];
+    const int gl = threadIdx.x / L;
+    const long long v = group_id<L>();
+    const long long row = v / tpr;
+    const int lane = (int)(v % tpr);
+    const bool live = row < m;
+    Grp<N, L> g(C);
+    MP<R> acc, a, p;
+    g.zero(acc);
+    if (live) {
+        const int re = irp[row + 1];
+        for (int i = irp[row] + lane; i < re; i += tpr) {
+            load_aos(g, a, x, ja[i]);
+            g.mul_d(p, a, as[i]);
+            g.add(acc, acc, p);
+        }
+    }
+    if (tpr > 1) {
+        for (int h = tpr >> 1; h >= 1; h >>= 1) {
+            // publish partials of lanes h..2h-1, then lanes < h add them
+            if (live && lane >= h && lane < 2 * h) {
+#pragma unroll
+                for (int r = 0; r < R; r++) sd[gl][g.ix(r)] = acc.d[r];
+                if (g.lane == 0) {
+                    ss[gl] = acc.sign;
+                    se[gl] = acc.exp;
+                    sf0[gl] = acc.ev0.frac;
+                    sx0[gl] = acc.ev0.exp;
+                    sf1[gl] = acc.ev1.frac;
+                    sx1[gl] = acc.ev1.exp;
+                }
+            }
+            __syncthreads();
+            if (live && lane < h) {
+                const int o = gl + h;
+#pragma unroll
+                for (int r = 0; r < R; r++) p.d[r] = sd[o][g.ix(r)];
+                p.sign = ss[o];
+                p.exp = se[o];
+                p.ev0 = Er{sf0[o], sx0[o]};
+                p.ev1 = Er{sf1[o], sx1[o]};
+                g.add(acc, acc, p);
+            }
+            __syncthreads();
+        }
+    }
+    if (live && lane == 0) store_aos(g, acc, y, row);
+}
+
+}  // namespace mph

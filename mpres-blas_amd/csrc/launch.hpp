@@ -1,0 +1,166 @@
+// Host-side launchers of the kernel templates for one (N, L) pair, exposed
+// as a table of plain function pointers so that api.cpp (the C ABI) can
+// dispatch at run time on the context's moduli count and the chosen lanes
+// per number.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "ops.hpp"
+
+#if defined(__HIPCC__) && defined(MPH_KERNELS)
+#include "kernels.hpp"
+
+namespace mph {
+
+inline unsigned grid_for(long long groups, int L) {
+    const long long threads = groups * L;
+    return (unsigned)((threads + BLK - 1) / BLK);
+}
+
+template <int N, int L>
+struct Launch {
+    static hipError_t set_d(const DevConsts *c, const double *x, int count, SoA so, char *aos, bool soa,
+                            hipStream_t st) {
+        if (count <= 0) return hipSuccess;
+        if (soa)
+            hipLaunchKernelGGL((k_set_d<N, L, true>), dim3(grid_for(count, L)), dim3(BLK), 0, st, c, x, count, so,
+                               AoSRef<N>{aos});
+        else
+            hipLaunchKernelGGL((k_set_d<N, L, false>), dim3(grid_for(count, L)), dim3(BLK), 0, st, c, x, count, so,
+                               AoSRef<N>{aos});
+        return hipGetLastError();
+    }
+    static hipError_t elem(const DevConsts *c, int op, int count, const char *x, const char *y, const double *yd,
+                           char *r, hipStream_t st) {
+        if (count <= 0) return hipSuccess;
+        const dim3 gr(grid_for(count, L)), bl(BLK);
+        AoSRef<N> X{(char *)x}, Y{(char *)y}, Rr{r};
+        switch (op) {
+            case 0: hipLaunchKernelGGL((k_elem<N, L, 0>), gr, bl, 0, st, c, count, X, Y, yd, Rr); break;
+            case 1: hipLaunchKernelGGL((k_elem<N, L, 1>), gr, bl, 0, st, c, count, X, Y, yd, Rr); break;
+            case 2: hipLaunchKernelGGL((k_elem<N, L, 2>), gr, bl, 0, st, c, count, X, Y, yd, Rr); break;
+            case 3: hipLaunchKernelGGL((k_elem<N, L, 3>), gr, bl, 0, st, c, count, X, Y, yd, Rr); break;
+            case 4: hipLaunchKernelGGL((k_elem<N, L, 4>), gr, bl, 0, st, c, count, X, Y, yd, Rr); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
+    static hipError_t fold(const DevConsts *c, int mode, long long cnt, long long nvt, SoA xs, int incx, SoA ys,
+                           int incy, const char *xa, const char *ya, char *out, hipStream_t st) {
+        if (nvt <= 0) return hipSuccess;
+        const dim3 gr(grid_for(nvt, L)), bl(BLK);
+        AoSRef<N> XA{(char *)xa}, YA{(char *)ya}, O{out};
+        if (mode == 0)
+            hipLaunchKernelGGL((k_fold<N, L, 0>), gr, bl, 0, st, c, cnt, nvt, xs, incx, ys, incy, XA, YA, O);
+        else if (mode == 1)
+            hipLaunchKernelGGL((k_fold<N, L, 1>), gr, bl, 0, st, c, cnt, nvt, xs, incx, ys, incy, XA, YA, O);
+        else
+            hipLaunchKernelGGL((k_fold<N, L, 2>), gr, bl, 0, st, c, cnt, nvt, xs, incx, ys, incy, XA, YA, O);
+        return hipGetLastError();
+    }
+    static hipError_t tree(const DevConsts *c, char *part, int b, unsigned p2, int nblocks, char *out, hipStream_t st) {
+        if (nblocks <= 0) return hipSuccess;
+        hipLaunchKernelGGL((k_tree<N, L>), dim3(nblocks), dim3(BLK), 0, st, c, AoSRef<N>{part}, b, p2, AoSRef<N>{out});
+        return hipGetLastError();
+    }
+    static hipError_t aos_to_soa(const DevConsts *c, const char *src, SoA dst, int count, hipStream_t st) {
+        if (count <= 0) return hipSuccess;
+        hipLaunchKernelGGL((k_aos_to_soa<N, L>), dim3(grid_for(count, L)), dim3(BLK), 0, st, c,
+                           AoSRef<N>{(char *)src}, dst, count);
+        return hipGetLastError();
+    }
+    static hipError_t soa_to_aos(const DevConsts *c, SoA src, char *dst, int count, hipStream_t st) {
+        if (count <= 0) return hipSuccess;
+        hipLaunchKernelGGL((k_soa_to_aos<N, L>), dim3(grid_for(count, L)), dim3(BLK), 0, st, c, src, AoSRef<N>{dst},
+                           count);
+        return hipGetLastError();
+    }
+    static hipError_t axpy(const DevConsts *c, int n, SoA alpha, SoA x, int incx, SoA y, int incy, hipStream_t st) {
+        if (n <= 0) return hipSuccess;
+        hipLaunchKernelGGL((k_axpy<N, L>), dim3(grid_for(n, L)), dim3(BLK), 0, st, c, n, alpha, x, incx, y, incy);
+        return hipGetLastError();
+    }
+    static hipError_t axpy_v2(const DevConsts *c, int n, const char *alpha, const char *x, const char *y, char *r,
+                              hipStream_t st) {
+        if (n <= 0) return hipSuccess;
+        hipLaunchKernelGGL((k_axpy_v2<N, L>), dim3(grid_for(n, L)), dim3(BLK), 0, st, c, n, AoSRef<N>{(char *)alpha},
+                           AoSRef<N>{(char *)x}, AoSRef<N>{(char *)y}, AoSRef<N>{r});
+        return hipGetLastError();
+    }
+    static hipError_t scal(const DevConsts *c, int len, SoA s, SoA v, int inc, char *out, bool inplace,
+                           hipStream_t st) {
+        if (len <= 0) return hipSuccess;
+        if (inplace)
+            hipLaunchKernelGGL((k_scal<N, L, true>), dim3(grid_for(len, L)), dim3(BLK), 0, st, c, len, s, v, inc,
+                               AoSRef<N>{out});
+        else
+            hipLaunchKernelGGL((k_scal<N, L, false>), dim3(grid_for(len, L)), dim3(BLK), 0, st, c, len, s, v, inc,
+                               AoSRef<N>{out});
+        return hipGetLastError();
+    }
+    static hipError_t gemv_fold(const DevConsts *c, bool trans, int m, int n, SoA A, int lda, const char *bx, int b3,
+                                long long nvt, char *part, hipStream_t st) {
+        if (nvt <= 0) return hipSuccess;
+        if (trans)
+            hipLaunchKernelGGL((k_gemv_fold<N, L, true>), dim3(grid_for(nvt, L)), dim3(BLK), 0, st, c, m, n, A, lda,
+                               AoSRef<N>{(char *)bx}, b3, nvt, AoSRef<N>{part});
+        else
+            hipLaunchKernelGGL((k_gemv_fold<N, L, false>), dim3(grid_for(nvt, L)), dim3(BLK), 0, st, c, m, n, A, lda,
+                               AoSRef<N>{(char *)bx}, b3, nvt, AoSRef<N>{part});
+        return hipGetLastError();
+    }
+    static hipError_t gemv_final(const DevConsts *c, int len, SoA y, int incy, const char *sums, hipStream_t st) {
+        if (len <= 0) return hipSuccess;
+        hipLaunchKernelGGL((k_gemv_final<N, L>), dim3(grid_for(len, L)), dim3(BLK), 0, st, c, len, y, incy,
+                           AoSRef<N>{(char *)sums});
+        return hipGetLastError();
+    }
+    static hipError_t gemm(const DevConsts *c, int m, int n, int k, SoA alpha, SoA A, int lda, SoA B, int ldb,
+                           SoA beta, SoA Cm, int ldc, hipStream_t st) {
+        if ((long long)m * n <= 0) return hipSuccess;
+        hipLaunchKernelGGL((k_gemm<N, L>), dim3(grid_for((long long)m * n, L)), dim3(BLK), 0, st, c, m, n, k, alpha, A,
+                           lda, B, ldb, beta, Cm, ldc);
+        return hipGetLastError();
+    }
+    static hipError_t gemm_v2(const DevConsts *c, int ta, int tb, int m, int n, int k, const char *alpha,
+                              const char *A, int lda, const char *B, int ldb, const char *beta, char *Cm, int ldc,
+                              hipStream_t st) {
+        if ((long long)m * n <= 0) return hipSuccess;
+        hipLaunchKernelGGL((k_gemm_v2<N, L>), dim3(grid_for((long long)m * n, L)), dim3(BLK), 0, st, c, ta, tb, m, n,
+                           k, AoSRef<N>{(char *)alpha}, AoSRef<N>{(char *)A}, lda, AoSRef<N>{(char *)B}, ldb,
+                           AoSRef<N>{(char *)beta}, AoSRef<N>{Cm}, ldc);
+        return hipGetLastError();
+    }
+    static hipError_t spmv(const DevConsts *c, int m, const int *irp, const int *ja, const double *as, const char *x,
+                           char *y, int tpr, hipStream_t st) {
+        if (m <= 0) return hipSuccess;
+        if (tpr > BLK / L) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_spmv<N, L>), dim3(grid_for((long long)m * tpr, L)), dim3(BLK), 0, st, c, m, irp, ja, as,
+                           AoSRef<N>{(char *)x}, AoSRef<N>{y}, tpr);
+        return hipGetLastError();
+    }
+    static Ops table() {
+        Ops o;
+        o.N = N;
+        o.L = L;
+        o.set_d = &set_d;
+        o.elem = &elem;
+        o.fold = &fold;
+        o.tree = &tree;
+        o.aos_to_soa = &aos_to_soa;
+        o.soa_to_aos = &soa_to_aos;
+        o.axpy = &axpy;
+        o.axpy_v2 = &axpy_v2;
+        o.scal = &scal;
+        o.gemv_fold = &gemv_fold;
+        o.gemv_final = &gemv_final;
+        o.gemm = &gemm;
+        o.gemm_v2 = &gemm_v2;
+        o.spmv = &spmv;
+        return o;
+    }
+};
+
+}  // namespace mph
+#endif

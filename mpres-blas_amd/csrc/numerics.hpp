@@ -1,0 +1,196 @@
+// Scalar numerics shared by every kernel of the RNS multiple-precision path:
+// IEEE-754 directed rounding, extended-range floats, exact modular arithmetic.
+//
+// Everything here is __host__ __device__ so the same code is checked on the
+// CPU (tests/test_numerics_host.py builds a host-only harness of this header)
+// before it runs on the GPU.
+//
+// Directed rounding. The reference's CUDA path uses __dadd_rd / __dmul_ru /
+// __ddiv_rd ... (true IEEE directed rounding, e.g. src/extrange.cuh:458-641,
+// src/rns.cuh:807-862). gfx950 has no per-instruction rounding mode, so each
+// directed operation is derived from the round-to-nearest result and its
+// exact error term:
+//   a + b = s + e   (Knuth's TwoSum, exact for finite inputs)
+//   a * b = p + e   (e = fma(a, b, -p), exact without underflow)
+//   a / b = q + r/b (r = fma(-q, b, a), exact without underflow)
+// and the rounded-to-nearest value is stepped one ulp down (up) when the
+// error shows it lies above (below) the exact result. Exact zero sums follow
+// IEEE-754: -0 under round-down unless both operands are +0.
+//
+// Modular arithmetic. Residues are < 2^31 and products < 2^62. Products are
+// split into a round-to-nearest double p and its exact low part; the quotient
+// floor(p / m) is estimated from p * (1/m) and the remainder p - q*m is
+// formed exactly by an fma, then corrected into [0, m). The result is the
+// canonical residue, i.e. what the reference's 64-bit '%' gives
+// (src/modular.cuh:118-174, 250-314). All of this is full-rate FP64 on CDNA4.
+#pragma once
+
+#include <stdint.h>
+#include <math.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define MPH_HD __host__ __device__ __forceinline__
+#else
+#define MPH_HD inline __attribute__((always_inline))
+#endif
+
+namespace mph {
+
+MPH_HD uint64_t d2u(double x) {
+    union { double d; uint64_t u; } v;
+    v.d = x;
+    return v.u;
+}
+MPH_HD double u2d(uint64_t x) {
+    union { double d; uint64_t u; } v;
+    v.u = x;
+    return v.d;
+}
+
+// next double toward -inf / +inf of a finite nonzero x
+MPH_HD double step_down(double x) { return u2d(x > 0 ? d2u(x) - 1 : d2u(x) + 1); }
+MPH_HD double step_up(double x) { return u2d(x > 0 ? d2u(x) + 1 : d2u(x) - 1); }
+
+MPH_HD double add_rd(double a, double b) {
+    double s = a + b;
+    double bb = s - a;
+    double e = (a - (s - bb)) + (b - bb);
+    if (e < 0) return step_down(s);
+    if (s == 0) return (d2u(a) == 0 && d2u(b) == 0) ? 0.0 : -0.0;
+    return s;
+}
+MPH_HD double add_ru(double a, double b) {
+    double s = a + b;
+    double bb = s - a;
+    double e = (a - (s - bb)) + (b - bb);
+    return (e > 0) ? step_up(s) : s;
+}
+MPH_HD double sub_rd(double a, double b) { return add_rd(a, -b); }
+MPH_HD double sub_ru(double a, double b) { return add_ru(a, -b); }
+MPH_HD double mul_rd(double a, double b) {
+    double p = a * b;
+    double e = fma(a, b, -p);
+    return (e < 0) ? step_down(p) : p;
+}
+MPH_HD double mul_ru(double a, double b) {
+    double p = a * b;
+    double e = fma(a, b, -p);
+    return (e > 0) ? step_up(p) : p;
+}
+MPH_HD double div_rd(double a, double b) {
+    double q = a / b;
+    double r = fma(-q, b, a);
+    bool below = (r != 0) && ((r < 0) != (b < 0));  // exact a/b < q
+    return below ? step_down(q) : q;
+}
+MPH_HD double div_ru(double a, double b) {
+    double q = a / b;
+    double r = fma(-q, b, a);
+    bool above = (r != 0) && ((r < 0) == (b < 0));  // exact a/b > q
+    return above ? step_up(q) : q;
+}
+
+// ceil(log2(x)) where log2 is rounded to nearest, for a positive normal x.
+// log2(x) = e + log2(f), f in [1,2). It rounds to e exactly when f = 1 or
+// log2(f) is below half the spacing of doubles just above e (toward e+1).
+MPH_HD int ceil_log2_rn(double x) {
+    const uint64_t u = d2u(x);
+    const int e = (int)((u >> 52) & 2047) - 1023;
+    const uint64_t mant = u & ((((uint64_t)1) << 52) - 1);
+    if (mant == 0) return e;
+    if (e == 0) return 1;
+    const int a = e > 0 ? e : -e;
+    int fl = 31 - __builtin_clz((unsigned)a);         // floor(log2 |e|)
+    if (e < 0 && (a & (a - 1)) == 0) fl -= 1;         // just above -2^k lies the binade below
+    // half spacing = 2^(fl-53); log2(f) < 2^(fl-53)  <=>  mant < 2^(fl-1) * ln 2  (to first order)
+    const double t = ldexp(0.69314718055994530942, fl - 1);
+    return ((double)mant < t) ? e : e + 1;
+}
+
+// ---------------------------------------------------------------------------
+// Extended-range floats (er_float_t: frac in [1,2) and an integer exponent),
+// src/extrange.cuh. Exponents are held in int registers.
+// ---------------------------------------------------------------------------
+struct Er {
+    double frac;
+    int exp;
+};
+
+// er_adjust, src/extrange.cuh:412-421
+MPH_HD Er er_adjust(double frac, int exp) {
+    const uint64_t u = d2u(frac);
+    Er r;
+    const bool nz = frac != 0;
+    r.exp = nz ? (exp + (int)((u & ~((uint64_t)1 << 63)) >> 52) - 1023) : 0;
+    const uint64_t v = (u & ((((uint64_t)1) << 52) - 1)) | (u & ((uint64_t)1 << 63)) | ((uint64_t)1023 << 52);
+    r.frac = u2d(v) * (double)nz;
+    return r;
+}
+// er_set_d, src/extrange.cuh:46-58 (CUDA :94-106)
+MPH_HD Er er_set_d(double x) {
+    Er r;
+    if (x != 0) {
+        const uint64_t u = d2u(x);
+        r.exp = (int)((u & ~((uint64_t)1 << 63)) >> 52) - 1023;
+        r.frac = u2d((u & ((((uint64_t)1) << 52) - 1)) | (u & ((uint64_t)1 << 63)) | ((uint64_t)1023 << 52));
+    } else {
+        r.exp = 0;
+        r.frac = 0;
+    }
+    return r;
+}
+// er_add_rd / er_add_ru (CUDA), src/extrange.cuh:450-490
+MPH_HD Er er_add_dir(Er x, Er y, bool up) {
+    x.exp = (x.frac != 0) ? x.exp : 0;
+    y.exp = (y.frac != 0) ? y.exp : 0;
+    const int dexp = (x.frac != 0 && y.frac != 0) ? x.exp - y.exp : 0;
+    double a, b;
+    int e;
+    if (dexp > 0) {
+        e = x.exp;
+        a = x.frac;
+        b = ldexp(y.frac, -dexp);
+    } else if (dexp < 0) {
+        e = y.exp;
+        a = y.frac;
+        b = ldexp(x.frac, dexp);
+    } else {
+        e = (x.exp == 0) ? y.exp : x.exp;
+        a = x.frac;
+        b = y.frac;
+    }
+    const double f = up ? add_ru(a, b) : add_rd(a, b);
+    return er_adjust(f, e);
+}
+// er_md_rd / er_md_ru: x*y/z, src/extrange.cuh:627-644
+MPH_HD Er er_md_rd(Er x, Er y, Er z) { return er_adjust(div_rd(mul_rd(x.frac, y.frac), z.frac), x.exp + y.exp - z.exp); }
+MPH_HD Er er_md_ru(Er x, Er y, Er z) { return er_adjust(div_ru(mul_ru(x.frac, y.frac), z.frac), x.exp + y.exp - z.exp); }
+
+// ---------------------------------------------------------------------------
+// Exact modular arithmetic for moduli m < 2^31 (values held as int).
+// ---------------------------------------------------------------------------
+// x in (-2m, 3m) -> [0, m)
+MPH_HD double mod_fix(double r, double m) {
+    r = (r < 0) ? r + m : r;
+    r = (r < 0) ? r + m : r;
+    r = (r >= m) ? r - m : r;
+    r = (r >= m) ? r - m : r;
+    return r;
+}
+// (a * b) mod m for 0 <= a, b < 2^31
+MPH_HD int mulmod(int a, int b, double m, double minv) {
+    const double da = (double)a, db = (double)b;
+    const double p = da * db;
+    const double pl = fma(da, db, -p);
+    const double q = floor(p * minv);
+    double r = fma(-q, m, p) + pl;
+    return (int)mod_fix(r, m);
+}
+// v mod m for an integer-valued |v| < 2^53
+MPH_HD int modd(double v, double m, double minv) {
+    const double q = floor(v * minv);
+    return (int)mod_fix(fma(-q, m, v), m);
+}
+
+}  // namespace mph

@@ -1,0 +1,54 @@
+// Type-erased launcher table shared between the kernel translation units and
+// the C ABI (api.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "devconsts.hpp"
+
+namespace mph {
+
+struct ErMem {  // er_float_t in memory: {double frac; long exp}
+    double frac;
+    long long exp;
+};
+
+// mp_array_t / mp_collection_t (structure of arrays), src/types.cuh:85-104
+struct SoA {
+    int *d;
+    int *s;
+    int *e;
+    ErMem *ev;
+    const int *lenp;  // device pointer to the actual length (mp_array_t.len)
+    int len;          // filled from *lenp inside the kernels: eval upper bounds start at ev + len
+};
+
+struct Ops {
+    int N, L;
+    hipError_t (*set_d)(const DevConsts *, const double *, int, SoA, char *, bool, hipStream_t);
+    hipError_t (*elem)(const DevConsts *, int, int, const char *, const char *, const double *, char *, hipStream_t);
+    hipError_t (*fold)(const DevConsts *, int, long long, long long, SoA, int, SoA, int, const char *,
+                       const char *, char *, hipStream_t);
+    hipError_t (*tree)(const DevConsts *, char *, int, unsigned, int, char *, hipStream_t);
+    hipError_t (*aos_to_soa)(const DevConsts *, const char *, SoA, int, hipStream_t);
+    hipError_t (*soa_to_aos)(const DevConsts *, SoA, char *, int, hipStream_t);
+    hipError_t (*axpy)(const DevConsts *, int, SoA, SoA, int, SoA, int, hipStream_t);
+    hipError_t (*axpy_v2)(const DevConsts *, int, const char *, const char *, const char *, char *, hipStream_t);
+    hipError_t (*scal)(const DevConsts *, int, SoA, SoA, int, char *, bool, hipStream_t);
+    hipError_t (*gemv_fold)(const DevConsts *, bool, int, int, SoA, int, const char *, int, long long, char *,
+                            hipStream_t);
+    hipError_t (*gemv_final)(const DevConsts *, int, SoA, int, const char *, hipStream_t);
+    hipError_t (*gemm)(const DevConsts *, int, int, int, SoA, SoA, int, SoA, int, SoA,
+                       SoA, int, hipStream_t);
+    hipError_t (*gemm_v2)(const DevConsts *, int, int, int, int, int, const char *, const char *, int, const char *,
+                          int, const char *, char *, int, hipStream_t);
+    hipError_t (*spmv)(const DevConsts *, int, const int *, const int *, const double *, const char *, char *, int,
+                       hipStream_t);
+};
+
+// lanes-per-number values compiled for each moduli count
+const Ops *ops_n8(int L);
+const Ops *ops_n16(int L);
+const Ops *ops_n32(int L);
+
+}  // namespace mph

@@ -1,0 +1,253 @@
+"""GPU parity: every entry point of the HIP library against the oracle's
+restatement of the reference CUDA algorithm (policy D), bit for bit in every
+field (residues, sign, exponent, both interval bounds), on seeded inputs at
+sizes the oracle finishes in seconds, for every compiled lanes-per-number
+variant."""
+import ctypes as C
+import importlib.util
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import exact
+import gen
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+D = po.POLICY_D
+
+
+def _load_mpres():
+    spec = importlib.util.spec_from_file_location("mpres", os.path.join(ROOT, "mpres-blas_amd", "mpres.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+mp = _load_mpres()
+_ctx = {}
+_orc = {}
+
+
+def ctx(name):
+    if name not in _ctx:
+        _ctx[name] = mp.Context(name)
+    return _ctx[name]
+
+
+def orc(name):
+    if name not in _orc:
+        _orc[name] = po.Oracle(name)
+    return _orc[name]
+
+
+def lanes_of(c):
+    return [L for L in (1, 2, 4, 8, 16, 32) if c.L.mpres_lanes_supported(c.h, L)]
+
+
+def note(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def same(got, want, what):
+    got = np.ascontiguousarray(got)
+    want = np.ascontiguousarray(want)
+    assert got.dtype == want.dtype and got.shape == want.shape
+    if got.tobytes() == want.tobytes():
+        return
+    bad = [i for i in range(len(got)) if got[i].tobytes() != want[i].tobytes()]
+    i = bad[0]
+    raise AssertionError(f"{what}: {len(bad)} of {len(got)} records differ; first {i}:\n gpu {got[i]}\n ref {want[i]}")
+
+
+SETS = ["8_2double", "16_4double", "32_8double", "8_32bit"]
+
+
+def inputs(o, seed, count, elo=-40, ehi=40):
+    x = gen.uexp(seed, count, elo, ehi)
+    x[:6] = [0.0, -0.0, 1.0, -1.0, 0.75, -3.0]
+    return x
+
+
+@pytest.mark.parametrize("name", SETS)
+def test_set_d(name):
+    c, o = ctx(name), orc(name)
+    x = np.concatenate([inputs(o, 1, 3000, -300, 300), [2.0**-1000, -2.0**900, 5e-324]])
+    for L in lanes_of(c):
+        note("set_d", name, L)
+        c.set_lanes(mp.K_ELEM, L)
+        got = c.aos_to_host(c.set_d(x), len(x))
+        same(got, o.set_d(x, D), f"set_d {name} L={L}")
+
+
+def _operands(o, seed, cnt):
+    a = o.set_d(inputs(o, seed, cnt), D)
+    b = o.set_d(inputs(o, seed + 1, cnt), D)
+    # products carry 106-bit significands: at 8 moduli their sums round
+    pa = o.mul(a, b, D)
+    pb = o.mul(b[::-1].copy(), a, D)
+    # cancellations: exact (x + -x) and near-exact, to reach the sign estimate (MRC) path
+    neg = pa.copy()
+    neg["sign"] ^= 1
+    near = o.mul(pa, o.set_d(np.full(cnt, 1.0 + 2.0**-40), D), D)
+    near["sign"] ^= 1
+    X = np.concatenate([a, pa, pa, pa, pa])
+    Y = np.concatenate([b, pb, neg, near, pb[::-1]])
+    return X, Y
+
+
+@pytest.mark.parametrize("name", SETS)
+def test_elementwise(name):
+    c, o = ctx(name), orc(name)
+    X, Y = _operands(o, 11, 600)
+    yd = inputs(o, 13, len(X), -30, 30)
+    dX, dY, dYd = c.aos(X), c.aos(Y), mp.DevBuf(host=yd)
+    want = {mp.OP_ADD: o.add(X, Y, D), mp.OP_SUB: o.sub(X, Y, D), mp.OP_MUL: o.mul(X, Y, D),
+            mp.OP_MUL_D: o.mul_d(X, yd, D), mp.OP_VADD: o.vadd(X, Y)}
+    for L in lanes_of(c):
+        c.set_lanes(mp.K_ELEM, L)
+        for op, w in want.items():
+            note("elementwise", name, L, op)
+            r = c.aos(count=len(X))
+            mp.check(c.L.mpres_elementwise(c.h, op, len(X), dX.ptr, dYd.ptr if op == mp.OP_MUL_D else dY.ptr, r.ptr,
+                                           None))
+            same(c.aos_to_host(r, len(X)), w, f"op {op} {name} L={L}")
+
+
+@pytest.mark.parametrize("name,n,g3,b3", [("8_2double", 5000, 64, 64), ("8_2double", 3001, 256, 64),
+                                          ("8_2double", 700, 7, 33), ("16_4double", 2000, 64, 64),
+                                          ("32_8double", 1500, 32, 64)])
+def test_dot(name, n, g3, b3):
+    c, o = ctx(name), orc(name)
+    x = o.set_d(gen.uexp(21, n, -20, 20), D)
+    y = o.set_d(gen.uexp(22, n, -20, 20), D)
+    ax, ay = c.array_from_host(x), c.array_from_host(y)
+    r = c.array(1)
+    dx, dy, dr = c.aos(x), c.aos(y), c.aos(count=1)
+    want = o.dot(x, y, g3, b3, D)
+    want_s = o.dot(x, y, g3, b3, D, incx=-1, incy=2, n=n // 2)
+    for L in lanes_of(c):
+        c.set_lanes(mp.K_DOT, L)
+        note("dot", name, n, g3, b3, L)
+        mp.check(c.L.mpres_dot(c.h, n, C.byref(ax), 1, C.byref(ay), 1, C.byref(r), g3, b3, None, 0, None))
+        same(c.array_to_host(r, 1), want, f"dot {name} L={L}")
+        mp.check(c.L.mpres_dot(c.h, n // 2, C.byref(ax), -1, C.byref(ay), 2, C.byref(r), g3, b3, None, 0, None))
+        same(c.array_to_host(r, 1), want_s, f"dot strided {name} L={L}")
+        mp.check(c.L.mpres_dot_v2(c.h, n, dx.ptr, dy.ptr, dr.ptr, g3, b3, None, 0, None))
+        same(c.aos_to_host(dr, 1), want, f"dot_v2 {name} L={L}")
+    # the result is an exactly rounded-down sum of exactly rounded products: compare with the exact value
+    m = [int(v) for v in o.moduli()]
+    ex = sum(exact.mp_value(p, m) for p in o.mul(x, y, D))
+    got = exact.mp_value(want[0], m)
+    if ex != 0:
+        assert abs(got - ex) / abs(ex) < 2.0 ** (-c.precision + 12)
+
+
+@pytest.mark.parametrize("name", ["8_2double", "32_8double"])
+def test_axpy(name):
+    c, o = ctx(name), orc(name)
+    n = 3000
+    X, Y = _operands(o, 31, n // 5)
+    alpha = o.set_d([-0.625], D)
+    want1 = o.axpy(alpha, X, Y, D)
+    want2 = o.axpy_v2(alpha, X, Y, D)
+    wants = o.axpy(alpha, X, Y, D, incx=2, incy=-1, n=len(X) // 2)
+    for L in lanes_of(c):
+        c.set_lanes(mp.K_AXPY, L)
+        note("axpy", name, L)
+        ax, ay, aa = c.array_from_host(X), c.array_from_host(Y), c.array_from_host(alpha)
+        mp.check(c.L.mpres_axpy(c.h, len(X), C.byref(aa), C.byref(ax), 1, C.byref(ay), 1, None))
+        same(c.array_to_host(ay, len(X)), want1, f"axpy {name} L={L}")
+        ay = c.array_from_host(Y)
+        mp.check(c.L.mpres_axpy(c.h, len(X) // 2, C.byref(aa), C.byref(ax), 2, C.byref(ay), -1, None))
+        same(c.array_to_host(ay, len(X)), wants, f"axpy strided {name} L={L}")
+        dX, dY, dA, dR = c.aos(X), c.aos(Y), c.aos(alpha), c.aos(count=len(X))
+        mp.check(c.L.mpres_axpy_v2(c.h, len(X), dA.ptr, dX.ptr, dY.ptr, dR.ptr, None))
+        same(c.aos_to_host(dR, len(X)), want2, f"axpy_v2 {name} L={L}")
+
+
+@pytest.mark.parametrize("name,m,n,b3", [("32_8double", 70, 50, 32), ("32_8double", 33, 64, 5),
+                                         ("8_2double", 64, 40, 32), ("16_4double", 20, 90, 16)])
+def test_gemv(name, m, n, b3):
+    c, o = ctx(name), orc(name)
+    A = o.set_d(gen.uexp(41, m * n, -30, 30), D)
+    alpha = o.set_d(gen.uexp(42, 1, -30, 30), D)
+    beta = o.set_d(gen.uexp(43, 1, -30, 30), D)
+    for trans in (0, 1):
+        lx, ly = (m, n) if trans else (n, m)
+        x = o.set_d(gen.uexp(44, lx, -30, 30), D)
+        y = o.set_d(gen.uexp(45, ly, -30, 30), D)
+        want = o.gemv(trans, m, n, alpha, A, m, x, beta, y, b3, D)
+        for L in lanes_of(c):
+            c.set_lanes(mp.K_GEMV, L)
+            note("gemv", name, m, n, b3, trans, L)
+            aA, ax, ay = c.array_from_host(A), c.array_from_host(x), c.array_from_host(y)
+            aa, ab = c.array_from_host(alpha), c.array_from_host(beta)
+            mp.check(c.L.mpres_gemv(c.h, mp.TRANS if trans else mp.NO_TRANS, m, n, C.byref(aa), C.byref(aA), m,
+                                    C.byref(ax), 1, C.byref(ab), C.byref(ay), 1, b3, None, 0, None))
+            same(c.array_to_host(ay, ly), want, f"gemv {name} trans={trans} L={L}")
+
+
+@pytest.mark.parametrize("name,m,n,k", [("32_8double", 13, 11, 17), ("8_2double", 9, 7, 40), ("16_4double", 5, 6, 7)])
+def test_gemm(name, m, n, k):
+    c, o = ctx(name), orc(name)
+    A = o.set_d(gen.uexp(51, m * k, -30, 30), D)
+    B = o.set_d(gen.uexp(52, k * n, -30, 30), D)
+    Cm = o.set_d(gen.uexp(53, m * n, -30, 30), D)
+    alpha = o.set_d(gen.uexp(54, 1, -30, 30), D)
+    beta = o.set_d(gen.uexp(55, 1, -30, 30), D)
+    want = o.gemm(m, n, k, alpha, A, m, B, k, beta, Cm, m, D)
+    for L in lanes_of(c):
+        c.set_lanes(mp.K_GEMM, L)
+        note("gemm", name, m, n, k, L)
+        aA, aB, aC = c.array_from_host(A), c.array_from_host(B), c.array_from_host(Cm)
+        aa, ab = c.array_from_host(alpha), c.array_from_host(beta)
+        mp.check(c.L.mpres_gemm(c.h, mp.NO_TRANS, mp.NO_TRANS, m, n, k, C.byref(aa), C.byref(aA), m, C.byref(aB), k,
+                                C.byref(ab), C.byref(aC), m, None, 0, None))
+        same(c.array_to_host(aC, m * n), want, f"gemm {name} L={L}")
+        assert c.L.mpres_gemm(c.h, mp.TRANS, mp.NO_TRANS, m, n, k, C.byref(aa), C.byref(aA), k, C.byref(aB), k,
+                              C.byref(ab), C.byref(aC), m, None, 0, None) == -4
+    for ta, tb in ((0, 0), (1, 0), (0, 1), (1, 1)):
+        lda = k if ta else m
+        ldb = n if tb else k
+        want2 = o.gemm_v2(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, Cm, m, D)
+        for L in lanes_of(c):
+            c.set_lanes(mp.K_GEMM, L)
+            note("gemm_v2", name, ta, tb, L)
+            dA, dB, dC, da, db = c.aos(A), c.aos(B), c.aos(Cm), c.aos(alpha), c.aos(beta)
+            mp.check(c.L.mpres_gemm_v2(c.h, mp.TRANS if ta else mp.NO_TRANS, mp.TRANS if tb else mp.NO_TRANS, m, n, k,
+                                       da.ptr, dA.ptr, lda, dB.ptr, ldb, db.ptr, dC.ptr, m, None))
+            same(c.aos_to_host(dC, m * n), want2, f"gemm_v2 {name} {ta}{tb} L={L}")
+
+
+def _spmv_case(name, m, irp, ja, as_, xs, tprs):
+    c, o = ctx(name), orc(name)
+    x = o.set_d(xs, D)
+    dirp, dja, das, dx = mp.DevBuf(host=irp), mp.DevBuf(host=ja), mp.DevBuf(host=as_), c.aos(x)
+    csr = mp.Csr(das.ptr.value, dja.ptr.value, dirp.ptr.value)
+    for tpr in tprs:
+        want = o.spmv(m, irp, ja, as_, x, tpr, D)
+        for L in lanes_of(c):
+            c.set_lanes(mp.K_SPMV, L)
+            note("spmv", name, m, tpr, L)
+            dy = c.aos(count=m)
+            mp.check(c.L.mpres_spmv_csrv(c.h, m, C.byref(csr), tpr, dx.ptr, dy.ptr, None))
+            same(c.aos_to_host(dy, m), want, f"spmv {name} tpr={tpr} L={L}")
+
+
+def test_spmv_lf10():
+    """tests/sparse/matrices/LF10.mtx, the reference's SpMV fixture."""
+    m, n, irp, ja, as_ = gen.read_mtx(os.path.join(ROOT, "tests", "golden", "LF10.mtx"))
+    assert (m, n, irp[-1]) == (18, 18, 82)
+    _spmv_case("16_4double", m, irp, ja, as_, gen.uexp(61, n, -10, 10), (1, 2, 4, 8, 16, 32))
+
+
+@pytest.mark.parametrize("name", ["16_4double", "8_2double"])
+def test_spmv_powerlaw(name):
+    m = 600
+    irp, ja, as_ = gen.powerlaw_csr(62, m, kmax=200)
+    _spmv_case(name, m, irp, ja, as_, gen.uexp(63, m, -10, 10), (1, 4, 32))

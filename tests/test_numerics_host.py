@@ -1,0 +1,104 @@
+"""The GPU path's scalar numerics (mpres-blas_amd/csrc/numerics.hpp), built
+for the host, against the oracle's IEEE directed rounding (fesetround), glibc's
+ceil(log2()) and exact integer arithmetic."""
+import ctypes as C
+import math
+import os
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def bits(x):
+    return struct.unpack("<Q", struct.pack("<d", x))[0]
+
+
+@pytest.fixture(scope="module")
+def nh():
+    d = tempfile.mkdtemp()
+    so = os.path.join(d, "nh.so")
+    subprocess.run(["g++", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-o", so,
+                    os.path.join(ROOT, "tests", "host", "numerics_host.cpp")], check=True)
+    L = C.CDLL(so)
+    for f in ("nh_add", "nh_sub", "nh_mul", "nh_div"):
+        getattr(L, f).restype = C.c_double
+        getattr(L, f).argtypes = [C.c_int, C.c_double, C.c_double]
+    L.nh_ceil_log2.argtypes = [C.c_double]
+    L.nh_mulmod.argtypes = [C.c_int, C.c_int, C.c_int]
+    L.nh_modd.argtypes = [C.c_double, C.c_int]
+    L.nh_er_add.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_double),
+                            C.POINTER(C.c_int)]
+    return L
+
+
+def operands(rng, count):
+    u = rng.uniform(-1, 1, count)
+    e = rng.integers(-80, 80, count)
+    a = np.ldexp(u, e)
+    out = list(a)
+    # edge cases: zeros of both signs, exact cancellation, powers of two, ulp neighbours
+    out += [0.0, -0.0, 1.0, -1.0, 2.0**-60, 3.0, 1.5, math.nextafter(1.0, 2), math.nextafter(1.0, 0)]
+    return out
+
+
+@pytest.mark.parametrize("op", ["add", "sub", "mul", "div"])
+def test_directed_ops_match_ieee(nh, oracle_lib, op):
+    L = oracle_lib.lib()
+    ref = {"add": L.orc_dadd, "sub": L.orc_dsub, "mul": L.orc_dmul, "div": L.orc_ddiv}[op]
+    mine = {"add": nh.nh_add, "sub": nh.nh_sub, "mul": nh.nh_mul, "div": nh.nh_div}[op]
+    rng = np.random.default_rng(11)
+    xs = operands(rng, 4000)
+    ys = operands(rng, 4000)
+    ys[:200] = [-v for v in xs[:200]]  # exact cancellation
+    ys[200:400] = xs[200:400]
+    for x, y in zip(xs, ys):
+        if op == "div" and y == 0:
+            continue
+        for up in (0, 1):
+            a, b = mine(up, x, y), ref(po.POLICY_D, up, x, y)
+            assert bits(a) == bits(b), (op, up, x, y, a, b)
+
+
+def test_ceil_log2_matches_libm(nh, oracle_lib):
+    L = oracle_lib.lib()
+    rng = np.random.default_rng(12)
+    xs = []
+    for e in list(range(-1020, -1)) + list(range(1, 60)):
+        base = 2.0**e
+        xs.append(base)
+        v = base
+        for _ in range(12):  # the first doubles above a power of two
+            v = math.nextafter(v, math.inf)
+            xs.append(v)
+        # around the rounding threshold of log2 near 2^e
+        for j in range(1, 600, 7):
+            xs.append(base * (1 + j * 2.0**-52))
+        xs.append(math.nextafter(base, 0))
+    xs += list(np.ldexp(rng.uniform(0.5, 1, 3000), rng.integers(-1000, -20, 3000)))
+    bad = [x for x in xs if nh.nh_ceil_log2(x) != L.orc_ceil_log2_lib(x)]
+    assert not bad, bad[:5]
+
+
+@pytest.mark.parametrize("m", [113812103, 99312287, 1283742851, 1122512269, 39051, 34571])
+def test_mulmod_exact(nh, m):
+    rng = np.random.default_rng(m)
+    vals = list(rng.integers(0, m, 3000)) + [0, 1, m - 1, m - 2, m // 2, (m + 1) // 2]
+    ws = list(rng.integers(0, m, 3000)) + [m - 1, m - 1, m - 1, 2, 2, 2]
+    for a, b in zip(vals, ws):
+        assert nh.nh_mulmod(int(a), int(b), m) == (int(a) * int(b)) % m
+    # products that land near multiples of m (where a quotient estimate is off by one)
+    for k in range(1, 3000):
+        a = int(rng.integers(1, m))
+        if math.gcd(a, m) != 1:
+            continue
+        b = (k * m + int(rng.integers(-3, 4))) * pow(a, -1, m) % m
+        assert nh.nh_mulmod(a, b, m) == (a * b) % m
+    for v in list(rng.integers(-2**52, 2**52, 3000)) + [0, 2**53 - 1, -(2**53 - 1)]:
+        assert nh.nh_modd(float(v), m) == int(v) % m
