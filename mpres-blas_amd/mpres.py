@@ -1,8 +1,8 @@
 """Python binding of libmpres_hip.so (the C ABI in include/mpres_hip.h).
 
 Thin ctypes layer used by the tests and the benchmark. Device memory comes
-from torch (plain byte tensors) or from the library's own mp_array_t
-allocator; every computation runs in the HIP library. Loading fails loudly
+from hipMalloc (through the HIP runtime the library links) or from the
+library's own mp_array_t allocator; every computation runs in the HIP library. Loading fails loudly
 when the library has not been built: there is no fallback path.
 """
 import ctypes as C
@@ -103,36 +103,98 @@ def set_names():
     return [L.mpres_moduli_set_name(i).decode() for i in range(L.mpres_num_moduli_sets())]
 
 
-def _torch():
-    import torch
+_hip = None
 
-    return torch
+
+def hip():
+    """The HIP runtime that libmpres_hip.so itself links (same process-wide instance).
+
+    Device buffers and timing events are created through it directly; torch is
+    not used here because it carries its own copy of the HIP runtime."""
+    global _hip
+    if _hip is None:
+        lib()
+        H = C.CDLL("libamdhip64.so.7")
+        vp, sz = C.c_void_p, C.c_size_t
+        H.hipMalloc.argtypes = [C.POINTER(vp), sz]
+        H.hipFree.argtypes = [vp]
+        H.hipMemcpy.argtypes = [vp, vp, sz, C.c_int]
+        H.hipMemset.argtypes = [vp, C.c_int, sz]
+        H.hipEventCreate.argtypes = [C.POINTER(vp)]
+        H.hipEventRecord.argtypes = [vp, vp]
+        H.hipEventSynchronize.argtypes = [vp]
+        H.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), vp, vp]
+        H.hipEventDestroy.argtypes = [vp]
+        H.hipGetErrorString.restype = C.c_char_p
+        _hip = H
+    return _hip
+
+
+def hipchk(rc):
+    if rc != 0:
+        raise MpresError(f"HIP error {rc}: {hip().hipGetErrorString(rc).decode()}")
+
+
+def synchronize():
+    hipchk(hip().hipDeviceSynchronize())
 
 
 class DevBuf:
-    """A device byte buffer (torch uint8 tensor) holding numpy data."""
+    """A device byte buffer (hipMalloc), optionally filled from a numpy array."""
 
     def __init__(self, nbytes=None, host=None):
-        torch = _torch()
+        H = hip()
         if host is not None:
             host = np.ascontiguousarray(host)
-            raw = torch.from_numpy(host.view(np.uint8).reshape(-1).copy())
-            self.t = raw.to("cuda")
-            self.nbytes = host.nbytes
+            nbytes = host.nbytes
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        hipchk(H.hipMalloc(C.byref(p), max(self.nbytes, 1)))
+        self.p = p
+        if host is not None:
+            hipchk(H.hipMemcpy(p, host.ctypes.data_as(C.c_void_p), self.nbytes, 1))
         else:
-            self.nbytes = int(nbytes)
-            self.t = torch.zeros(max(self.nbytes, 1), dtype=torch.uint8, device="cuda")
+            hipchk(H.hipMemset(p, 0, max(self.nbytes, 1)))
 
     @property
     def ptr(self):
-        return C.c_void_p(self.t.data_ptr())
+        return self.p
 
     def to_numpy(self, dtype, count=None):
-        torch = _torch()
-        torch.cuda.synchronize()
-        b = self.t.cpu().numpy()[: self.nbytes]
-        a = b.view(dtype)
+        out = np.empty(self.nbytes, np.uint8)
+        hipchk(hip().hipMemcpy(out.ctypes.data_as(C.c_void_p), self.p, self.nbytes, 2))
+        a = out.view(dtype)
         return a if count is None else a[:count]
+
+    def __del__(self):
+        try:
+            if self.p:
+                hip().hipFree(self.p)
+                self.p = None
+        except Exception:
+            pass
+
+
+class Timer:
+    """HIP events on the null stream (the stream the library's kernels use when no stream is given)."""
+
+    def __init__(self):
+        H = hip()
+        self.a, self.b = C.c_void_p(), C.c_void_p()
+        hipchk(H.hipEventCreate(C.byref(self.a)))
+        hipchk(H.hipEventCreate(C.byref(self.b)))
+
+    def start(self, stream=None):
+        hipchk(hip().hipEventRecord(self.a, stream))
+
+    def stop(self, stream=None):
+        hipchk(hip().hipEventRecord(self.b, stream))
+
+    def ms(self):
+        hipchk(hip().hipEventSynchronize(self.b))
+        t = C.c_float()
+        hipchk(hip().hipEventElapsedTime(C.byref(t), self.a, self.b))
+        return t.value
 
 
 class Context:
@@ -213,7 +275,7 @@ class Context:
         dx = DevBuf(host=x)
         a = self.array(len(x))
         check(self.L.mpres_array_set_d(self.h, C.byref(a), dx.ptr, len(x), None))
-        _torch().cuda.synchronize()
+        synchronize()
         return a
 
     def elementwise(self, op, x, y, count):
@@ -222,4 +284,4 @@ class Context:
         return r
 
     def sync(self):
-        _torch().cuda.synchronize()
+        synchronize()

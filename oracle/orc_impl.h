@@ -622,7 +622,10 @@ static void F(vadd)(const orc_ctx *c, MPF *r, MPF x, MPF y) {
         const double d = 1.0 / m;
         long rr = (long)(fx * x.digits[i]) * (long)c->pow2[pg * N + i] + (long)(fy * y.digits[i]) * (long)c->pow2[pt * N + i];
         double q = (double)rr * d;
-        int qi = (int)q;
+        /* with ~31-bit moduli the quotient can leave the int range; the CUDA
+         * conversion saturates (PTX cvt clamps float-to-integer results), where
+         * a plain C cast on x86 would give INT_MIN */
+        int qi = (q >= 2147483648.0) ? 2147483647 : (q <= -2147483649.0) ? (-2147483647 - 1) : (int)q;
         int res = (int)(rr - (long)qi * (long)m);
         if (r->sign == 1) res = (int)(((long)m - (long)res) % (long)m);
         r->digits[i] = res < 0 ? res + m : res;
