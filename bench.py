@@ -1,0 +1,315 @@
+#!/usr/bin/env python3
+"""Throughput of the plain RNS multiple-precision path at 424 bits on one MI355X.
+
+Workload (one step): mp_gemv y = alpha*A*x + beta*y with A 4096 x 4096, then
+mp_gemm C = alpha*A*B + beta*C with 512 x 512 x 512, both with the 32-modulus
+set 32_8double (424-bit precision). Inputs are u*2^e with u uniform in [-1,1)
+and e uniform in [-30,30], converted to the mp format on the GPU before the
+timed region. gemv uses the reference harness's row-sum block size 32, which
+fixes its summation order.
+
+Prints one JSON line: whole-step throughput in mp-FMA/s (one mp-FMA = one
+multiply-accumulate of the operation: m*n for gemv, M*N*K for gemm), the
+roofline position of the dominant kernel (its average duration measured with
+HIP events around its launches), the relative error of sampled outputs
+against the exact (dyadic) result, and the CPU reference algorithm timed on a
+bounded sample of the same workload on this host.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from fractions import Fraction
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "tests")]
+
+import gen  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 (vector and matrix)
+
+SET = "32_8double"
+M_GEMV = N_GEMV = 4096
+M_GEMM = N_GEMM = K_GEMM = 512
+B3 = 32
+ELO, EHI = -30, 30
+
+
+def load_mpres():
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("mpres", os.path.join(ROOT, "mpres-blas_amd", "mpres.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def exact_value(rec, m):
+    M = 1
+    for v in m:
+        M *= int(v)
+    X = 0
+    for d, mi in zip(rec["digits"], m):
+        mi = int(mi)
+        Mi = M // mi
+        X += int(d) * Mi * pow(Mi, -1, mi)
+    X %= M
+    v = Fraction(X) * (Fraction(2) ** int(rec["exp"]))
+    return -v if int(rec["sign"]) else v
+
+
+def cpu_baseline(threads):
+    """The reference's CPU mp_float_t algorithm (the oracle's policy-E restatement),
+    composed in the GPU's order, on a bounded sample: 2048 full rows of the gemv
+    and a 256 x 256 block of the gemm (full K)."""
+    sys.path[:0] = [os.path.join(ROOT, "oracle")]
+    import pyoracle as po
+
+    po.set_threads(threads)
+    o = po.Oracle(SET)
+    rows = 2048
+    bm = bn = 256
+    A = o.set_d(gen.uexp(301, rows * N_GEMV, ELO, EHI), po.POLICY_E)
+    x = o.set_d(gen.uexp(302, N_GEMV, ELO, EHI), po.POLICY_E)
+    y = o.set_d(gen.uexp(303, rows, ELO, EHI), po.POLICY_E)
+    al = o.set_d(gen.uexp(304, 1, ELO, EHI), po.POLICY_E)
+    be = o.set_d(gen.uexp(305, 1, ELO, EHI), po.POLICY_E)
+    Ag = o.set_d(gen.uexp(311, bm * K_GEMM, ELO, EHI), po.POLICY_E)
+    Bg = o.set_d(gen.uexp(312, K_GEMM * bn, ELO, EHI), po.POLICY_E)
+    Cg = o.set_d(gen.uexp(313, bm * bn, ELO, EHI), po.POLICY_E)
+    t0 = time.perf_counter()
+    o.gemv(0, rows, N_GEMV, al, A, rows, x, be, y, B3, policy=po.POLICY_E)
+    t1 = time.perf_counter()
+    o.gemm(bm, bn, K_GEMM, al, Ag, bm, Bg, K_GEMM, be, Cg, bm, policy=po.POLICY_E)
+    t2 = time.perf_counter()
+    fmas = rows * N_GEMV + bm * bn * K_GEMM
+    return {
+        "value": fmas / (t2 - t0),
+        "unit": "mp-FMA/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{rows} rows of the 4096x4096 gemv ({rows * N_GEMV} mp-FMA, {t1 - t0:.2f} s) + a "
+                  f"{bm}x{bn} block of the 512^3 gemm ({bm * bn * K_GEMM} mp-FMA, {t2 - t1:.2f} s), "
+                  f"CPU mp_float_t algorithm (emulated directed rounding), OpenMP over rows/entries",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    mp = load_mpres()
+    ctx = mp.Context(SET)
+    nmod = ctx.n
+    S = ctx.size
+    print(f"[bench] context {SET}: n={nmod}, precision={ctx.precision} bits", file=sys.stderr, flush=True)
+
+    # ---- inputs resident in HBM (converted on the GPU) ----
+    A = ctx.array_set_d(gen.uexp(101, M_GEMV * N_GEMV, ELO, EHI))
+    x = ctx.array_set_d(gen.uexp(102, N_GEMV, ELO, EHI))
+    y0 = ctx.array_set_d(gen.uexp(103, M_GEMV, ELO, EHI))
+    y = ctx.array_set_d(gen.uexp(103, M_GEMV, ELO, EHI))
+    alpha = ctx.array_set_d(gen.uexp(104, 1, ELO, EHI))
+    beta = ctx.array_set_d(gen.uexp(105, 1, ELO, EHI))
+    Ag = ctx.array_set_d(gen.uexp(201, M_GEMM * K_GEMM, ELO, EHI))
+    Bg = ctx.array_set_d(gen.uexp(202, K_GEMM * N_GEMM, ELO, EHI))
+    C0 = ctx.array_set_d(gen.uexp(203, M_GEMM * N_GEMM, ELO, EHI))
+    Cg = ctx.array_set_d(gen.uexp(203, M_GEMM * N_GEMM, ELO, EHI))
+    alpha_g = ctx.array_set_d(gen.uexp(204, 1, ELO, EHI))
+    beta_g = ctx.array_set_d(gen.uexp(205, 1, ELO, EHI))
+    ws_bytes = ctx.L.mpres_gemv_workspace(ctx.h, mp.NO_TRANS, M_GEMV, N_GEMV, B3)
+    ws = mp.DevBuf(nbytes=ws_bytes)
+    mp.synchronize()
+    print("[bench] inputs converted", file=sys.stderr, flush=True)
+
+    L = ctx.L
+    ev = [mp.Timer() for _ in range(2)]
+    t_gemv = [0.0]
+    t_gemm = [0.0]
+
+    def step(timed):
+        if timed:
+            ev[0].start()
+        ctx.copy_array(y, y0, M_GEMV)
+        mp.check(L.mpres_gemv(ctx.h, mp.NO_TRANS, M_GEMV, N_GEMV, C.byref(alpha), C.byref(A), M_GEMV, C.byref(x), 1,
+                              C.byref(beta), C.byref(y), 1, B3, ws.ptr, ws_bytes, None))
+        if timed:
+            ev[0].stop()
+            ev[1].start()
+        ctx.copy_array(Cg, C0, M_GEMM * N_GEMM)
+        mp.check(L.mpres_gemm(ctx.h, mp.NO_TRANS, mp.NO_TRANS, M_GEMM, N_GEMM, K_GEMM, C.byref(alpha_g), C.byref(Ag),
+                              M_GEMM, C.byref(Bg), K_GEMM, C.byref(beta_g), C.byref(Cg), M_GEMM, None, 0, None))
+        if timed:
+            ev[1].stop()
+            t_gemv[0] += ev[0].ms()
+            t_gemm[0] += ev[1].ms()
+
+    for _ in range(args.warmup):
+        step(False)
+    mp.synchronize()
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+
+    # ---- timed region: K steps, no per-op events, kernel-level events inside the library ----
+    ctx.profile(True)
+    ctx.profile_read(mp.K_GEMM)
+    ctx.profile_read(mp.K_GEMV)
+    mp.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(False)
+    mp.synchronize()
+    wall = time.perf_counter() - t0
+    gemm_ms, gemm_n = ctx.profile_read(mp.K_GEMM)
+    gemv_ms, gemv_n = ctx.profile_read(mp.K_GEMV)
+    ctx.profile(False)
+    ms_per_step = wall * 1e3 / args.steps
+    print(f"[bench] timed {args.steps} steps: {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
+
+    # per-operation split (separate pass with events between the operations)
+    for _ in range(max(1, min(args.steps, 3))):
+        step(True)
+    nsplit = max(1, min(args.steps, 3))
+    gemv_op_ms, gemm_op_ms = t_gemv[0] / nsplit, t_gemm[0] / nsplit
+
+    fma_gemv = M_GEMV * N_GEMV
+    fma_gemm = M_GEMM * N_GEMM * K_GEMM
+    value = (fma_gemv + fma_gemm) / (ms_per_step / 1e3)
+
+    # ---- roofline of the dominant kernel ----
+    gemm_avg = gemm_ms / max(gemm_n, 1)
+    gemv_avg = gemv_ms / max(gemv_n, 1)
+    if gemm_avg >= gemv_avg:
+        # residue multiply-accumulates M*N*K*n, counted as one FP64 FMA (2 flops) each
+        ach = 2.0 * fma_gemm * nmod / (gemm_avg / 1e3) / 1e12
+        roof = {"kernel": "k_gemm (fused per-entry k-loop + epilogue)", "bound": "mfma", "achieved": ach,
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
+                "avg_ms": gemm_avg, "work": "M*N*K*n residue multiply-accumulates, 2 FP64 flops each"}
+    else:
+        bytes_ = (M_GEMV * N_GEMV + N_GEMV + 2 * M_GEMV) * S
+        ach = bytes_ / (gemv_avg / 1e3) / 1e9
+        roof = {"kernel": "k_gemv_fold", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS, "traffic": None, "avg_ms": gemv_avg,
+                "work": "(m*n + n + 2m) * (4n+40) bytes"}
+
+    # ---- accuracy: sampled outputs against the exact dyadic result ----
+    m_ = [int(v) for v in ctx_moduli(ctx)]
+    err = accuracy(ctx, mp, A, x, y0, y, alpha, beta, Ag, Bg, C0, Cg, alpha_g, beta_g, m_)
+
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] CPU baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.cpu_threads)
+
+    out = {
+        "metric": "mp-FMA/s at 106 & 424 bits: MFMA/batched vs plain HIP, at stated MPFR error",
+        "value": value,
+        "unit": "mp-FMA/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32 residues + f64 interval",
+        "data": "synthetic",
+        "config": {"workload": "plain mp_gemv 4096x4096 + mp_gemm 512x512x512, 424-bit (32 moduli)",
+                   "moduli_set": SET, "inputs": "u*2^e, u~U[-1,1), e~U[-30,30]", "gemv_blockDim3": B3,
+                   "lanes_per_number": {"gemv": gemv_lanes(ctx, mp), "gemm": gemm_lanes(ctx, mp)}},
+        "breakdown": {"gemv_ms": gemv_op_ms, "gemm_ms": gemm_op_ms,
+                      "gemv_mpfma_per_s": fma_gemv / (gemv_op_ms / 1e3),
+                      "gemm_mpfma_per_s": fma_gemm / (gemm_op_ms / 1e3),
+                      "gemv_fold_kernel_ms": gemv_avg, "gemm_kernel_ms": gemm_avg},
+        "accuracy": err,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def ctx_moduli(ctx):
+    out = np.zeros(ctx.n, np.int32)
+    ctx.L.mpres_ctx_table(ctx.h, 0, out.ctypes.data_as(C.c_void_p), out.nbytes)
+    return out
+
+
+def gemv_lanes(ctx, mp):
+    return "default"
+
+
+def gemm_lanes(ctx, mp):
+    return "default"
+
+
+def accuracy(ctx, mp, A, x, y0, y, alpha, beta, Ag, Bg, C0, Cg, al_g, be_g, m):
+    """Relative error of 4 gemv outputs and 4 gemm entries vs the exact result of the same mp inputs."""
+    prec = ctx.precision
+    rows = [0, 1, 2047, 4095]
+    yv = ctx.array_to_host(y, M_GEMV)
+    xv = ctx.array_to_host(x, N_GEMV)
+    y0v = ctx.array_to_host(y0, M_GEMV)
+    a, b = exact_value(ctx.array_to_host(alpha, 1)[0], m), exact_value(ctx.array_to_host(beta, 1)[0], m)
+    xs = [exact_value(r, m) for r in xv]
+    worst = 0.0
+    # rows of A: element (i, j) at j*4096 + i; fetch the needed rows through a small device gather
+    Ah = fetch_rows(ctx, mp, A, rows)
+    for t, i in enumerate(rows):
+        ex = b * exact_value(y0v[i], m) + a * sum(exact_value(Ah[t][j], m) * xs[j] for j in range(N_GEMV))
+        got = exact_value(yv[i], m)
+        if ex != 0:
+            worst = max(worst, float(abs(got - ex) / abs(ex)))
+    gv = worst
+    worst = 0.0
+    Cv, C0v = ctx.array_to_host(Cg, M_GEMM * N_GEMM), ctx.array_to_host(C0, M_GEMM * N_GEMM)
+    Agv, Bgv = ctx.array_to_host(Ag, M_GEMM * K_GEMM), ctx.array_to_host(Bg, K_GEMM * N_GEMM)
+    ag, bg = exact_value(ctx.array_to_host(al_g, 1)[0], m), exact_value(ctx.array_to_host(be_g, 1)[0], m)
+    for (i, j) in [(0, 0), (1, 7), (300, 511), (511, 256)]:
+        s = sum(exact_value(Agv[l * M_GEMM + i], m) * exact_value(Bgv[j * K_GEMM + l], m) for l in range(K_GEMM))
+        ex = ag * s + bg * exact_value(C0v[j * M_GEMM + i], m)
+        got = exact_value(Cv[j * M_GEMM + i], m)
+        if ex != 0:
+            worst = max(worst, float(abs(got - ex) / abs(ex)))
+    return {"reference": "exact dyadic result of the same mp inputs (stronger than MPFR at 4x precision)",
+            "gemv_max_rel_err": gv, "gemm_max_rel_err": worst, "unit_roundoff": 2.0 ** (-prec),
+            "samples": "4 gemv outputs, 4 gemm entries"}
+
+
+def fetch_rows(ctx, mp, A, rows):
+    """Rows of the column-major 4096x4096 mp_array_t A (host copies of 4096 records each)."""
+    H = mp.hip()
+    n = ctx.n
+    out = []
+    dig = np.empty(n, np.int32)
+    sg = np.empty(1, np.int32)
+    ex = np.empty(1, np.int32)
+    ev = np.empty(1, mp.ER_DTYPE)
+    for i in rows:
+        recs = np.zeros(N_GEMV, ctx.dtype)
+        for j in range(N_GEMV):
+            e = j * M_GEMV + i
+            mp.hipchk(H.hipMemcpy(dig.ctypes.data_as(C.c_void_p), C.c_void_p(A.digits + 4 * n * e), 4 * n, 2))
+            mp.hipchk(H.hipMemcpy(sg.ctypes.data_as(C.c_void_p), C.c_void_p(A.sign + 4 * e), 4, 2))
+            mp.hipchk(H.hipMemcpy(ex.ctypes.data_as(C.c_void_p), C.c_void_p(A.exp + 4 * e), 4, 2))
+            recs[j]["digits"] = dig
+            recs[j]["sign"] = sg[0]
+            recs[j]["exp"] = ex[0]
+        out.append(recs)
+    return out
+
+
+if __name__ == "__main__":
+    main()

@@ -154,6 +154,15 @@ int mpres_spmv_csr(mpres_ctx_t *ctx, int m, const csr_t *csr, const void *x, voi
 int mpres_spmv_csrv(mpres_ctx_t *ctx, int m, const csr_t *csr, int threads_per_row, const void *x, void *y,
                     void *stream);
 
+/* ---- timing of each operation's main kernel ----
+ * With profiling on, the library brackets the main kernel of every dot (first
+ * reduction pass), gemv (strided row sums), gemm, SpMV and axpy call with HIP
+ * events on the call's stream. mpres_profile_read waits for the recorded
+ * events and returns the accumulated kernel time and launch count of one
+ * operation class (same op numbering as mpres_set_lanes). */
+int mpres_profile(mpres_ctx_t *ctx, int enable);
+int mpres_profile_read(mpres_ctx_t *ctx, int op, double *ms, long *launches, int reset);
+
 /* ---- tuning ----
  * Lanes per multiple-precision number used by the kernels of an operation
  * (0 = the built-in choice). op: 0 elementwise, 1 dot, 2 gemv, 3 gemm, 4 spmv, 5 axpy.

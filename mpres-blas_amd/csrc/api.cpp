@@ -64,6 +64,12 @@ struct mpres_ctx {
     const Ops *ops[OP_COUNT] = {};
     void *ws = nullptr;
     size_t ws_bytes = 0;
+    // optional timing of each operation's main kernel (mpres_profile)
+    bool prof = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_free;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_live;
+    double prof_ms[8] = {};
+    long prof_count[8] = {};
 };
 
 namespace {
@@ -117,6 +123,35 @@ unsigned next_pow2(unsigned v) {  // src/common.cuh:92-101
 }
 
 hipStream_t strm(void *st) { return (hipStream_t)st; }
+
+// Brackets the main kernel of operation `op` with events when profiling is on.
+struct ProfScope {
+    mpres_ctx *c;
+    int op;
+    hipStream_t st;
+    std::pair<hipEvent_t, hipEvent_t> ev{};
+    bool on = false;
+    ProfScope(mpres_ctx *c_, int op_, hipStream_t st_) : c(c_), op(op_), st(st_) {
+        if (!c->prof) return;
+        if (c->prof_free.empty()) {
+            hipEvent_t a, b;
+            if (hipEventCreate(&a) != hipSuccess) return;
+            if (hipEventCreate(&b) != hipSuccess) {
+                (void)hipEventDestroy(a);
+                return;
+            }
+            c->prof_free.push_back({a, b});
+        }
+        ev = c->prof_free.back();
+        c->prof_free.pop_back();
+        on = hipEventRecord(ev.first, st) == hipSuccess;
+    }
+    ~ProfScope() {
+        if (!on) return;
+        (void)hipEventRecord(ev.second, st);
+        c->prof_live.push_back({op, ev});
+    }
+};
 
 }  // namespace
 
@@ -181,6 +216,11 @@ int mpres_ctx_destroy(mpres_ctx_t *c) {
     (void)hipSetDevice(c->device);
     (void)hipDeviceSynchronize();
     if (c->ws) (void)hipFree(c->ws);
+    for (auto &e : c->prof_live) c->prof_free.push_back(e.second);
+    for (auto &e : c->prof_free) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
     (void)hipFree(c->dev);
     (void)hipFree(c->dpow2);
     delete c;
@@ -395,7 +435,10 @@ static int dot_common(mpres_ctx_t *c, int mode, int n, SoA xs, int incx, SoA ys,
     const Ops *o = c->ops[OP_DOT];
     const unsigned P = next_pow2((unsigned)b3);
     // pass 1: strided fold over the products, halving tree per block of b3 (src/mpreduct.cuh:38-74)
-    HIPCHK(o->fold(c->dev, mode, n, G, xs, incx, ys, incy, xa, ya, p1, st));
+    {
+        ProfScope ps(c, OP_DOT, st);
+        HIPCHK(o->fold(c->dev, mode, n, G, xs, incx, ys, incy, xa, ya, p1, st));
+    }
     HIPCHK(o->tree(c->dev, p1, b3, P, g3, p2, st));
     // pass 2: one block of b3 over the g3 block results (src/mpreduct.cuh:81-112)
     SoA none{};
@@ -440,6 +483,7 @@ int mpres_axpy(mpres_ctx_t *c, int n, const mp_array_t *alpha, const mp_array_t 
                void *st) {
     if (!c || !alpha || !x || !y) return fail(MPRES_ERR_ARG, "null argument");
     if (n <= 0) return MPRES_OK;  // src/blas/axpy.cuh:49-51
+    ProfScope ps(c, OP_AXPY, strm(st));
     HIPCHK(c->ops[OP_AXPY]->axpy(c->dev, n, soa_of(alpha), soa_of(x), incx, soa_of(y), incy, strm(st)));
     return MPRES_OK;
 }
@@ -483,7 +527,10 @@ int mpres_gemv(mpres_ctx_t *c, int trans, int m, int n, const mp_array_t *alpha,
     HIPCHK(o->scal(c->dev, lx, soa_of(alpha), soa_of(x), incx, bx, false, strm(st)));
     HIPCHK(o->scal(c->dev, ly, soa_of(beta), ys, incy, nullptr, true, strm(st)));
     // per-output strided sums of A .* buffer1 and the halving tree (:199-218, :44-76)
-    HIPCHK(o->gemv_fold(c->dev, tr, m, n, soa_of(A), lda, bx, b3, (long long)ly * b3, part, strm(st)));
+    {
+        ProfScope ps(c, OP_GEMV, strm(st));
+        HIPCHK(o->gemv_fold(c->dev, tr, m, n, soa_of(A), lda, bx, b3, (long long)ly * b3, part, strm(st)));
+    }
     HIPCHK(o->tree(c->dev, part, b3, next_pow2((unsigned)b3), ly, sums, strm(st)));
     HIPCHK(o->gemv_final(c->dev, ly, ys, incy, sums, strm(st)));
     return MPRES_OK;
@@ -511,6 +558,7 @@ int mpres_gemm(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const mp_arr
         return fail(MPRES_ERR_ARG, "invalid leading dimension");  // :82-96
     if (ta != MPRES_NO_TRANS || tb != MPRES_NO_TRANS)
         return fail(MPRES_ERR_UNSUPPORTED, "only non-transposed matrices are supported");  // :114-125
+    ProfScope ps(c, OP_GEMM, strm(st));
     HIPCHK(c->ops[OP_GEMM]->gemm(c->dev, m, n, k, soa_of(alpha), soa_of(A), lda, soa_of(B), ldb, soa_of(beta),
                                  soa_of(C), ldc, strm(st)));
     return MPRES_OK;
@@ -545,12 +593,40 @@ int mpres_spmv_csrv(mpres_ctx_t *c, int m, const csr_t *csr, int tpr, const void
         }
         if (tpr > 256 / o->L) return fail(MPRES_ERR_UNSUPPORTED, "threadsPerRow %d with %d lanes", tpr, o->L);
     }
+    ProfScope ps(c, OP_SPMV, strm(st));
     HIPCHK(o->spmv(c->dev, m, csr->irp, csr->ja, csr->as, (const char *)x, (char *)y, tpr, strm(st)));
     return MPRES_OK;
 }
 
 int mpres_spmv_csr(mpres_ctx_t *c, int m, const csr_t *csr, const void *x, void *y, void *st) {
     return mpres_spmv_csrv(c, m, csr, 1, x, y, st);
+}
+
+// ---------------------------------------------------------------- profiling
+int mpres_profile(mpres_ctx_t *c, int enable) {
+    if (!c) return fail(MPRES_ERR_ARG, "null context");
+    c->prof = enable != 0;
+    return MPRES_OK;
+}
+
+int mpres_profile_read(mpres_ctx_t *c, int op, double *ms, long *launches, int reset) {
+    if (!c || op < 0 || op >= OP_COUNT) return fail(MPRES_ERR_ARG, "invalid operation %d", op);
+    for (auto &e : c->prof_live) {
+        float t = 0;
+        HIPCHK(hipEventSynchronize(e.second.second));
+        HIPCHK(hipEventElapsedTime(&t, e.second.first, e.second.second));
+        c->prof_ms[e.first] += t;
+        c->prof_count[e.first] += 1;
+        c->prof_free.push_back(e.second);
+    }
+    c->prof_live.clear();
+    if (ms) *ms = c->prof_ms[op];
+    if (launches) *launches = c->prof_count[op];
+    if (reset) {
+        c->prof_ms[op] = 0;
+        c->prof_count[op] = 0;
+    }
+    return MPRES_OK;
 }
 
 // ---------------------------------------------------------------- tuning

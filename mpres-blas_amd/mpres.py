@@ -86,6 +86,8 @@ def lib():
         L.mpres_gemm_v2.argtypes = [vp, i32, i32, i32, i32, i32, vp, vp, i32, vp, i32, vp, vp, i32, vp]
         L.mpres_spmv_csr.argtypes = [vp, i32, vp, vp, vp, vp]
         L.mpres_spmv_csrv.argtypes = [vp, i32, vp, i32, vp, vp, vp]
+        L.mpres_profile.argtypes = [vp, i32]
+        L.mpres_profile_read.argtypes = [vp, i32, C.POINTER(C.c_double), C.POINTER(C.c_long), i32]
         L.mpres_lanes_supported.argtypes = [vp, i32]
         L.mpres_set_lanes.argtypes = [vp, i32, i32]
         _lib = L
@@ -120,6 +122,7 @@ def hip():
         H.hipFree.argtypes = [vp]
         H.hipMemcpy.argtypes = [vp, vp, sz, C.c_int]
         H.hipMemset.argtypes = [vp, C.c_int, sz]
+        H.hipMemcpyAsync.argtypes = [vp, vp, sz, C.c_int, vp]
         H.hipEventCreate.argtypes = [C.POINTER(vp)]
         H.hipEventRecord.argtypes = [vp, vp]
         H.hipEventSynchronize.argtypes = [vp]
@@ -231,6 +234,14 @@ class Context:
         except Exception:
             pass
 
+    def profile(self, enable=True):
+        check(self.L.mpres_profile(self.h, int(enable)))
+
+    def profile_read(self, op, reset=True):
+        ms, cnt = C.c_double(), C.c_long()
+        check(self.L.mpres_profile_read(self.h, op, C.byref(ms), C.byref(cnt), int(reset)))
+        return ms.value, cnt.value
+
     def set_lanes(self, op, lanes):
         check(self.L.mpres_set_lanes(self.h, op, lanes))
 
@@ -246,6 +257,15 @@ class Context:
         a = self.array(len(recs))
         check(self.L.mpres_array_host2device(self.h, C.byref(a), recs.ctypes.data_as(C.c_void_p), len(recs), None))
         return a
+
+    def copy_array(self, dst, src, count, stream=None):
+        """Device copy of the first count numbers of one mp_array_t into another of the same length."""
+        H = hip()
+        n = self.n
+        hipchk(H.hipMemcpyAsync(dst.digits, src.digits, 4 * n * count, 3, stream))
+        hipchk(H.hipMemcpyAsync(dst.sign, src.sign, 4 * count, 3, stream))
+        hipchk(H.hipMemcpyAsync(dst.exp, src.exp, 4 * count, 3, stream))
+        hipchk(H.hipMemcpyAsync(dst.eval, src.eval, 32 * count, 3, stream))
 
     def array_to_host(self, a, count):
         out = np.zeros(count, self.dtype)

@@ -99,6 +99,7 @@ int orc_ceil_log2_lib(double x) { return (int)ceil(log2(x)); }
 
 /* ---- element operations ---- */
 #define B_SET_D(s)                                                                         \
+    _Pragma("omp parallel for schedule(static)")                                           \
     for (int i = 0; i < count; i++) G(set_d, s)(c, &((G(mpf, s) *)out)[i], x[i]);
 void orc_set_d(const orc_ctx *c, int policy, int count, const double *x, void *out) { DISPATCH(c, policy, B_SET_D); }
 
