@@ -420,18 +420,20 @@ struct Grp {
             const int qi = (int)q;
             int v = (int)(rr - (long long)qi * (long long)mi[r]);
             if (minus) {
-                // (m - v) % m with C remainder semantics, |m - v| < 4m
-                int t = mi[r] - v;
+                // cuda::mod_sub(m, v, m) = ((long)m - v) % m, C remainder semantics; |m - v| < 4m
+                // needs 64 bits once moduli exceed 2^29
+                const long long mm = mi[r];
+                long long t = mm - (long long)v;
                 if (t >= 0) {
-                    t = t >= mi[r] ? t - mi[r] : t;
-                    t = t >= mi[r] ? t - mi[r] : t;
-                    t = t >= mi[r] ? t - mi[r] : t;
+                    t = t >= mm ? t - mm : t;
+                    t = t >= mm ? t - mm : t;
+                    t = t >= mm ? t - mm : t;
                 } else {
-                    t = t <= -mi[r] ? t + mi[r] : t;
-                    t = t <= -mi[r] ? t + mi[r] : t;
-                    t = t <= -mi[r] ? t + mi[r] : t;
+                    t = t <= -mm ? t + mm : t;
+                    t = t <= -mm ? t + mm : t;
+                    t = t <= -mm ? t + mm : t;
                 }
-                v = t;
+                v = (int)t;
             }
             res.d[r] = v < 0 ? v + mi[r] : v;
         }
