@@ -130,7 +130,8 @@ size_t mpres_gemv_workspace(const mpres_ctx_t *ctx, int trans, int m, int n, int
 int mpres_gemv(mpres_ctx_t *ctx, int trans, int m, int n, const mp_array_t *alpha, const mp_array_t *A, int lda,
                const mp_array_t *x, int incx, const mp_array_t *beta, mp_array_t *y, int incy, int block3, void *ws,
                size_t ws_bytes, void *stream);
-/* C = alpha*A*B + beta*C ; cuda::mp_gemm<...>, src/blas/gemm.cuh:69-167 (no-trans only, as the reference) */
+/* C = alpha*A*B + beta*C ; cuda::mp_gemm<...>, src/blas/gemm.cuh:69-167 (no-trans only, as the reference).
+ * Workspace: the m*n dot-product sums as mp_float_t (ws NULL: the context's pool). */
 size_t mpres_gemm_workspace(const mpres_ctx_t *ctx, int m, int n, int k);
 int mpres_gemm(mpres_ctx_t *ctx, int transa, int transb, int m, int n, int k, const mp_array_t *alpha,
                const mp_array_t *A, int lda, const mp_array_t *B, int ldb, const mp_array_t *beta, mp_array_t *C,
@@ -143,7 +144,7 @@ int mpres_dot_v2(mpres_ctx_t *ctx, int n, const void *x, const void *y, void *r,
                  size_t ws_bytes, void *stream);
 /* cuda::mp_axpy(n,alpha,x,y,r), src/blas/v2/axpy_v2.cuh:39-52 */
 int mpres_axpy_v2(mpres_ctx_t *ctx, int n, const void *alpha, const void *x, const void *y, void *r, void *stream);
-/* cuda::mp_gemm(transa,transb,...), src/blas/v2/gemm_v2.cuh:55-81 */
+/* cuda::mp_gemm(transa,transb,...), src/blas/v2/gemm_v2.cuh:55-81 (sums staged in the context's pool) */
 int mpres_gemm_v2(mpres_ctx_t *ctx, int transa, int transb, int m, int n, int k, const void *alpha, const void *A,
                   int lda, const void *B, int ldb, const void *beta, void *C, int ldc, void *stream);
 
@@ -155,8 +156,8 @@ int mpres_spmv_csrv(mpres_ctx_t *ctx, int m, const csr_t *csr, int threads_per_r
                     void *stream);
 
 /* ---- timing of each operation's main kernel ----
- * With profiling on, the library brackets the main kernel of every dot (first
- * reduction pass), gemv (strided row sums), gemm, SpMV and axpy call with HIP
+ * With profiling on, the library brackets the main kernels of every dot (first
+ * reduction pass), gemv (strided row sums), gemm (sums and epilogue), SpMV and axpy call with HIP
  * events on the call's stream. mpres_profile_read waits for the recorded
  * events and returns the accumulated kernel time and launch count of one
  * operation class (same op numbering as mpres_set_lanes). */

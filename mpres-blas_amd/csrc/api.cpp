@@ -14,6 +14,14 @@
 
 using namespace mph;
 
+const Ops *mph::find_variant(int N, int L) {
+#define MPH_CASE(VN, VL) \
+    if (N == VN && L == VL) return ops_n##VN##_l##VL();
+    MPH_VARIANTS(MPH_CASE)
+#undef MPH_CASE
+    return nullptr;
+}
+
 namespace {
 
 thread_local std::string g_err;
@@ -36,14 +44,7 @@ int fail(int code, const char *fmt, ...) {
 
 enum OpKind { OP_ELEM = 0, OP_DOT = 1, OP_GEMV = 2, OP_GEMM = 3, OP_SPMV = 4, OP_AXPY = 5, OP_COUNT = 6 };
 
-const Ops *find_ops(int n, int L) {
-    switch (n) {
-        case 8: return ops_n8(L);
-        case 16: return ops_n16(L);
-        case 32: return ops_n32(L);
-        default: return nullptr;
-    }
-}
+const Ops *find_ops(int n, int L) { return find_variant(n, L); }
 
 // Built-in lanes-per-number choice for each (moduli count, operation):
 // one lane per modulus everywhere unless a narrower group is compiled and
@@ -437,12 +438,13 @@ static int dot_common(mpres_ctx_t *c, int mode, int n, SoA xs, int incx, SoA ys,
     // pass 1: strided fold over the products, halving tree per block of b3 (src/mpreduct.cuh:38-74)
     {
         ProfScope ps(c, OP_DOT, st);
-        HIPCHK(o->fold(c->dev, mode, n, G, xs, incx, ys, incy, xa, ya, p1, st));
+        HIPCHK(o->fold(c->dev, mode, 0, n, G, xs, incx, ys, incy, xa, ya, p1, st));
     }
+    HIPCHK(o->fold(c->dev, mode, 1, n, G, xs, incx, ys, incy, xa, ya, p1, st));
     HIPCHK(o->tree(c->dev, p1, b3, P, g3, p2, st));
     // pass 2: one block of b3 over the g3 block results (src/mpreduct.cuh:81-112)
     SoA none{};
-    HIPCHK(o->fold(c->dev, 2, g3, b3, none, 1, none, 1, p2, nullptr, p3, st));
+    HIPCHK(o->fold(c->dev, 2, 0, g3, b3, none, 1, none, 1, p2, nullptr, p3, st));
     HIPCHK(o->tree(c->dev, p3, b3, P, 1, p4, st));
     *result = p4;
     return MPRES_OK;
@@ -529,8 +531,9 @@ int mpres_gemv(mpres_ctx_t *c, int trans, int m, int n, const mp_array_t *alpha,
     // per-output strided sums of A .* buffer1 and the halving tree (:199-218, :44-76)
     {
         ProfScope ps(c, OP_GEMV, strm(st));
-        HIPCHK(o->gemv_fold(c->dev, tr, m, n, soa_of(A), lda, bx, b3, (long long)ly * b3, part, strm(st)));
+        HIPCHK(o->gemv_fold(c->dev, tr, 0, m, n, soa_of(A), lda, bx, b3, (long long)ly * b3, part, strm(st)));
     }
+    HIPCHK(o->gemv_fold(c->dev, tr, 1, m, n, soa_of(A), lda, bx, b3, (long long)ly * b3, part, strm(st)));
     HIPCHK(o->tree(c->dev, part, b3, next_pow2((unsigned)b3), ly, sums, strm(st)));
     HIPCHK(o->gemv_final(c->dev, ly, ys, incy, sums, strm(st)));
     return MPRES_OK;
@@ -538,18 +541,14 @@ int mpres_gemv(mpres_ctx_t *c, int trans, int m, int n, const mp_array_t *alpha,
 
 // ---------------------------------------------------------------- gemm
 size_t mpres_gemm_workspace(const mpres_ctx_t *c, int m, int n, int k) {
-    (void)c;
-    (void)m;
-    (void)n;
     (void)k;
-    return 0;
+    if (!c || m <= 0 || n <= 0) return 0;
+    return mpsize(c) * (size_t)m * (size_t)n;  // the m*n dot-product sums
 }
 
 int mpres_gemm(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const mp_array_t *alpha, const mp_array_t *A,
                int lda, const mp_array_t *B, int ldb, const mp_array_t *beta, mp_array_t *C, int ldc, void *ws,
                size_t wsb, void *st) {
-    (void)ws;
-    (void)wsb;
     if (!c || !alpha || !A || !B || !beta || !C) return fail(MPRES_ERR_ARG, "null argument");
     if (m <= 0 || n <= 0 || k <= 0) return MPRES_OK;  // src/blas/gemm.cuh:75-77
     if ((ta == MPRES_NO_TRANS && lda < std::max(1, m)) || (ta != MPRES_NO_TRANS && lda < std::max(1, k)) ||
@@ -558,9 +557,17 @@ int mpres_gemm(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const mp_arr
         return fail(MPRES_ERR_ARG, "invalid leading dimension");  // :82-96
     if (ta != MPRES_NO_TRANS || tb != MPRES_NO_TRANS)
         return fail(MPRES_ERR_UNSUPPORTED, "only non-transposed matrices are supported");  // :114-125
-    ProfScope ps(c, OP_GEMM, strm(st));
-    HIPCHK(c->ops[OP_GEMM]->gemm(c->dev, m, n, k, soa_of(alpha), soa_of(A), lda, soa_of(B), ldb, soa_of(beta),
-                                 soa_of(C), ldc, strm(st)));
+    char *w;
+    int rc = get_ws(c, ws, wsb, mpres_gemm_workspace(c, m, n, k), &w);
+    if (rc) return rc;
+    const Ops *o = c->ops[OP_GEMM];
+    {
+        ProfScope ps(c, OP_GEMM, strm(st));
+        HIPCHK(o->gemm(c->dev, 0, m, n, k, soa_of(alpha), soa_of(A), lda, soa_of(B), ldb, soa_of(beta), soa_of(C),
+                       ldc, w, strm(st)));
+    }
+    HIPCHK(o->gemm(c->dev, 1, m, n, k, soa_of(alpha), soa_of(A), lda, soa_of(B), ldb, soa_of(beta), soa_of(C), ldc, w,
+                   strm(st)));
     return MPRES_OK;
 }
 
@@ -569,10 +576,13 @@ int mpres_gemm_v2(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const voi
     if (!c || !alpha || !A || !B || !beta || !C) return fail(MPRES_ERR_ARG, "null argument");
     if (m <= 0 || n <= 0) return MPRES_OK;
     if (k < 0 || lda <= 0 || ldb <= 0 || ldc < std::max(1, m)) return fail(MPRES_ERR_ARG, "invalid dimensions");
-    HIPCHK(c->ops[OP_GEMM]->gemm_v2(c->dev, ta == MPRES_TRANS || ta == MPRES_CONJ_TRANS,
-                                    tb == MPRES_TRANS || tb == MPRES_CONJ_TRANS, m, n, k, (const char *)alpha,
-                                    (const char *)A, lda, (const char *)B, ldb, (const char *)beta, (char *)C, ldc,
-                                    strm(st)));
+    char *w;
+    int rc = get_ws(c, nullptr, 0, mpres_gemm_workspace(c, m, n, k), &w);
+    if (rc) return rc;
+    const int tra = ta == MPRES_TRANS || ta == MPRES_CONJ_TRANS, trb = tb == MPRES_TRANS || tb == MPRES_CONJ_TRANS;
+    for (int pass = 0; pass < 2; pass++)
+        HIPCHK(c->ops[OP_GEMM]->gemm_v2(c->dev, pass, tra, trb, m, n, k, (const char *)alpha, (const char *)A, lda,
+                                        (const char *)B, ldb, (const char *)beta, (char *)C, ldc, w, strm(st)));
     return MPRES_OK;
 }
 
@@ -593,8 +603,11 @@ int mpres_spmv_csrv(mpres_ctx_t *c, int m, const csr_t *csr, int tpr, const void
         }
         if (tpr > 256 / o->L) return fail(MPRES_ERR_UNSUPPORTED, "threadsPerRow %d with %d lanes", tpr, o->L);
     }
-    ProfScope ps(c, OP_SPMV, strm(st));
-    HIPCHK(o->spmv(c->dev, m, csr->irp, csr->ja, csr->as, (const char *)x, (char *)y, tpr, strm(st)));
+    {
+        ProfScope ps(c, OP_SPMV, strm(st));
+        HIPCHK(o->spmv(c->dev, 0, m, csr->irp, csr->ja, csr->as, (const char *)x, (char *)y, tpr, strm(st)));
+    }
+    HIPCHK(o->spmv(c->dev, 1, m, csr->irp, csr->ja, csr->as, (const char *)x, (char *)y, tpr, strm(st)));
     return MPRES_OK;
 }
 

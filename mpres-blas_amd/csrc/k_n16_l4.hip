@@ -1,0 +1,5 @@
+// Kernel instantiation: 16 moduli, 4 lanes per number.
+#define MPH_KERNELS 1
+#include "launch.hpp"
+
+MPH_INSTANTIATE(16, 4)

@@ -1,0 +1,5 @@
+// Kernel instantiation: 8 moduli, 1 lane per number.
+#define MPH_KERNELS 1
+#include "launch.hpp"
+
+MPH_INSTANTIATE(8, 1)

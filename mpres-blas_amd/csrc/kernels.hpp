@@ -7,9 +7,23 @@
 // parameters survive only where they fix a summation order (dot: gridDim3 x
 // blockDim3, gemv: blockDim3, SpMV CSR-vector: threadsPerRow), and there the
 // same order is reproduced.
+//
+// Reductions over 16 and 32 moduli run in two kernels. The fast kernel
+// (FULL = false) carries only the common case of a multiply-accumulate step:
+// no rounding and an interval evaluation of the sum that does not straddle
+// zero. The rare cases need mixed-radix conversion, whose registers would
+// otherwise be reserved for the whole loop. A chain that meets one is
+// abandoned and marked (sign = REDO in its workspace result); the fixup
+// kernel (FULL = true, FIX = true) recomputes exactly the marked chains with
+// every case of the reference arithmetic. Both kernels run the identical
+// sequence of operations, so the result does not depend on which one
+// finished a chain. Over 8 moduli rounding is the common case and the
+// single full kernel is used.
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "group.hpp"
 #include "ops.hpp"
@@ -17,23 +31,13 @@
 namespace mph {
 
 constexpr int BLK = 256;
+constexpr int REDO = 2;  // sign of a workspace result whose chain the fixup kernel recomputes
 
-
-// mp_float_t arrays (array of structures, 4N+40 bytes per element)
-template <int N>
-struct AoSRef {
-    static constexpr int STRIDE = 4 * N + 40;
-    char *base;
-    __device__ __forceinline__ int *digits(long long i) const { return (int *)(base + i * STRIDE); }
-    __device__ __forceinline__ int *sign(long long i) const { return (int *)(base + i * STRIDE + 4 * N); }
-    __device__ __forceinline__ int *exp(long long i) const { return (int *)(base + i * STRIDE + 4 * N + 4); }
-    __device__ __forceinline__ ErMem *eval(long long i) const { return (ErMem *)(base + i * STRIDE + 4 * N + 8); }
-};
 
 template <int N, int L>
 __device__ __forceinline__ void load_soa(const Grp<N, L> &g, MP<N / L> &a, const SoA &v, long long i) {
     constexpr int R = N / L;
-    const int *p = v.d + i * N + g.lane * R;
+    const int *p = v.d + i * N + g.lane() * R;
 #pragma unroll
     for (int r = 0; r < R; r++) a.d[r] = p[r];
     a.sign = v.s[i];
@@ -45,10 +49,10 @@ __device__ __forceinline__ void load_soa(const Grp<N, L> &g, MP<N / L> &a, const
 template <int N, int L>
 __device__ __forceinline__ void store_soa(const Grp<N, L> &g, const MP<N / L> &a, const SoA &v, long long i) {
     constexpr int R = N / L;
-    int *p = v.d + i * N + g.lane * R;
+    int *p = v.d + i * N + g.lane() * R;
 #pragma unroll
     for (int r = 0; r < R; r++) p[r] = a.d[r];
-    if (g.lane == 0) {
+    if (g.lane() == 0) {
         v.s[i] = a.sign;
         v.e[i] = a.exp;
         v.ev[i] = ErMem{a.ev0.frac, (long long)a.ev0.exp};
@@ -58,7 +62,7 @@ __device__ __forceinline__ void store_soa(const Grp<N, L> &g, const MP<N / L> &a
 template <int N, int L>
 __device__ __forceinline__ void load_aos(const Grp<N, L> &g, MP<N / L> &a, const AoSRef<N> &v, long long i) {
     constexpr int R = N / L;
-    const int *p = v.digits(i) + g.lane * R;
+    const int *p = v.digits(i) + g.lane() * R;
 #pragma unroll
     for (int r = 0; r < R; r++) a.d[r] = p[r];
     a.sign = *v.sign(i);
@@ -71,10 +75,10 @@ __device__ __forceinline__ void load_aos(const Grp<N, L> &g, MP<N / L> &a, const
 template <int N, int L>
 __device__ __forceinline__ void store_aos(const Grp<N, L> &g, const MP<N / L> &a, const AoSRef<N> &v, long long i) {
     constexpr int R = N / L;
-    int *p = v.digits(i) + g.lane * R;
+    int *p = v.digits(i) + g.lane() * R;
 #pragma unroll
     for (int r = 0; r < R; r++) p[r] = a.d[r];
-    if (g.lane == 0) {
+    if (g.lane() == 0) {
         *v.sign(i) = a.sign;
         *v.exp(i) = a.exp;
         ErMem *e = v.eval(i);
@@ -145,32 +149,34 @@ __global__ __launch_bounds__(BLK) void k_elem(const DevConsts *__restrict__ C, i
 // virtual thread v accumulates terms v, v + nvt, v + 2 nvt, ... from zero.
 // MODE 0: terms are x_i * y_i with SoA x, y and BLAS strides (v1 dot)
 // MODE 1: terms are x_i * y_i with AoS x, y (v2 dot)
-// MODE 2: terms are AoS partials t_i
-template <int N, int L, int MODE>
+// MODE 2: terms are AoS partials t_i (always FULL, never FIX)
+template <int N, int L, int MODE, bool FULL, bool FIX>
 __global__ __launch_bounds__(BLK) void k_fold(const DevConsts *__restrict__ C, long long cnt, long long nvt, SoA xs,
                                               int incx, SoA ys, int incy, AoSRef<N> xa, AoSRef<N> ya, AoSRef<N> out) {
     xs = rd(xs);
     ys = rd(ys);
     const long long v = group_id<L>();
     if (v >= nvt) return;
+    if (FIX && *out.sign(v) != REDO) return;
     Grp<N, L> g(C);
-    MP<N / L> acc, a, b, p;
+    MP<N / L> acc, p;
     g.zero(acc);
+    bool ok = true;
     for (long long i = v; i < cnt; i += nvt) {
         if (MODE == 0) {
-            load_soa(g, a, xs, bidx(i, cnt, incx));
-            load_soa(g, b, ys, bidx(i, cnt, incy));
-            g.mul(p, a, b);
+            ok = g.template fma_step<FULL>(acc, xs, bidx(i, cnt, incx), ys, bidx(i, cnt, incy));
         } else if (MODE == 1) {
-            load_aos(g, a, xa, i);
-            load_aos(g, b, ya, i);
-            g.mul(p, a, b);
+            ok = g.template fma_step<FULL>(acc, xa, i, ya, i);
         } else {
             load_aos(g, p, xa, i);
+            g.add(acc, acc, p);
         }
-        g.add(acc, acc, p);
+        if (!ok) break;
     }
-    store_aos(g, acc, out, v);
+    if (ok)
+        store_aos(g, acc, out, v);
+    else if (g.lane() == 0)
+        *out.sign(v) = REDO;
 }
 
 // Halving tree of src/mpreduct.cuh:60-68 over each block of b partials:
@@ -286,25 +292,26 @@ __global__ __launch_bounds__(BLK) void k_scal(const DevConsts *__restrict__ C, i
 // virtual thread (o, t): o-th output, t-th of b3 strided partial sums of A-entries times bx
 // (matrix_row_sum_kernel / matrix_col_sum_kernel, src/blas/gemv.cuh:44-120, with the
 // products of the mat2vec scaling kernels formed on the fly)
-template <int N, int L, bool TRANS>
+template <int N, int L, bool TRANS, bool FULL, bool FIX>
 __global__ __launch_bounds__(BLK) void k_gemv_fold(const DevConsts *__restrict__ C, int m, int n, SoA A, int lda,
                                                    AoSRef<N> bx, int b3, long long nvt, AoSRef<N> part) {
     A = rd(A);
     const long long v = group_id<L>();
     if (v >= nvt) return;
+    if (FIX && *part.sign(v) != REDO) return;
     const long long o = v / b3;
     const int t = (int)(v % b3);
     Grp<N, L> g(C);
-    MP<N / L> acc, a, xb, p;
+    MP<N / L> acc;
     g.zero(acc);
     const int len = TRANS ? m : n;
-    for (int j = t; j < len; j += b3) {
-        load_soa(g, a, A, TRANS ? (o * lda + j) : ((long long)j * lda + o));
-        load_aos(g, xb, bx, j);
-        g.mul(p, a, xb);
-        g.add(acc, acc, p);
-    }
-    store_aos(g, acc, part, v);
+    bool ok = true;
+    for (int j = t; j < len && ok; j += b3)
+        ok = g.template fma_step<FULL>(acc, A, TRANS ? (o * lda + j) : ((long long)j * lda + o), bx, j);
+    if (ok)
+        store_aos(g, acc, part, v);
+    else if (g.lane() == 0)
+        *part.sign(v) = REDO;
 }
 // y[o] = y[o] (+) rowsum[o]  (src/blas/gemv.cuh:72-75 with mp_add rounding, add.cuh:223-248)
 template <int N, int L>
@@ -323,73 +330,90 @@ __global__ __launch_bounds__(BLK) void k_gemv_final(const DevConsts *__restrict_
 }
 
 // ---------------------------------------------------------------- gemm
-// v1 (src/blas/gemm.cuh:39-58 then the element-wise epilogue :127-166), fused per entry:
-// sum_k A*B in order, buffer = sum*alpha (+round), C = C*beta (+round), C = C (+) buffer (v1 add, +round)
-template <int N, int L>
-__global__ __launch_bounds__(BLK) void k_gemm(const DevConsts *__restrict__ C_, int m, int n, int k, SoA alpha, SoA A,
-                                              int lda, SoA B, int ldb, SoA beta, SoA Cm, int ldc) {
-    alpha = rd(alpha);
-    A = rd(A);
-    B = rd(B);
-    beta = rd(beta);
-    Cm = rd(Cm);
-    const long long e = group_id<L>();
-    if (e >= (long long)m * n) return;
-    const int row = (int)(e % m), col = (int)(e / m);
-    Grp<N, L> g(C_);
-    MP<N / L> sum, a, b, p;
-    g.zero(sum);
-    for (int i = 0; i < k; i++) {
-        load_soa(g, a, A, (long long)lda * i + row);
-        load_soa(g, b, B, (long long)col * ldb + i);
-        g.mul(p, a, b);
-        g.add(sum, sum, p);
-    }
-    MP<N / L> al, be, cv, s2, c2, r;
-    load_soa(g, al, alpha, 0);
-    load_soa(g, be, beta, 0);
-    const long long ic = (long long)col * ldc + row;
-    load_soa(g, cv, Cm, ic);
-    g.mul(s2, sum, al);
-    g.mul(c2, cv, be);
-    g.vadd(r, c2, s2);
-    store_soa(g, r, Cm, ic);
-}
-// v2 kernel (src/blas/v2/gemm_v2.cuh:55-81): C = (sum*alpha) (+) (C*beta)
-template <int N, int L>
-__global__ __launch_bounds__(BLK) void k_gemm_v2(const DevConsts *__restrict__ C_, int ta, int tb, int m, int n, int k,
-                                                 AoSRef<N> alpha, AoSRef<N> A, int lda, AoSRef<N> B, int ldb,
-                                                 AoSRef<N> beta, AoSRef<N> Cm, int ldc) {
-    const long long e = group_id<L>();
-    if (e >= (long long)m * n) return;
-    const int row = (int)(e % m), col = (int)(e / m);
-    Grp<N, L> g(C_);
-    MP<N / L> sum, a, b, p;
+// Entry e = (row, col) of the product, column-major: sum_i A[row, i] * B[i, col] in
+// order i = 0 .. k-1 (src/blas/gemm.cuh:39-58; v2 src/blas/v2/gemm_v2.cuh:55-75 with
+// the transposed index forms). ta/tb are ignored by the v1 layout (SoA).
+template <int N, int L, bool FULL, class VM>
+__device__ __forceinline__ bool gemm_sum(const Grp<N, L> &g, MP<N / L> &sum, int ta, int tb, int k, const VM &A,
+                                         int lda, const VM &B, int ldb, int row, int col) {
     g.zero(sum);
     for (int i = 0; i < k; i++) {
         const long long ia = ta ? (long long)i + (long long)lda * row : (long long)row + (long long)lda * i;
         const long long ib = tb ? (long long)col + (long long)ldb * i : (long long)i + (long long)ldb * col;
-        load_aos(g, a, A, ia);
-        load_aos(g, b, B, ib);
-        g.mul(p, a, b);
-        g.add(sum, sum, p);
+        if (!g.template fma_step<FULL>(sum, A, ia, B, ib)) return false;
     }
-    MP<N / L> al, be, cv, s2, c2, r;
-    load_aos(g, al, alpha, 0);
-    load_aos(g, be, beta, 0);
-    const long long ic = (long long)row + (long long)col * ldc;
-    load_aos(g, cv, Cm, ic);
-    g.mul(s2, sum, al);
-    g.mul(c2, cv, be);
-    g.add(r, s2, c2);
-    store_aos(g, r, Cm, ic);
+    return true;
+}
+// sums of all m*n entries into the AoS workspace (REDO marks an abandoned chain)
+template <int N, int L, bool FULL, class VM>
+__global__ __launch_bounds__(BLK) void k_gemm_sum(const DevConsts *__restrict__ C_, int ta, int tb, int m, int n,
+                                                  int k, VM A, int lda, VM B, int ldb, AoSRef<N> sums) {
+    if constexpr (std::is_same<VM, SoA>::value) {
+        A = rd(A);
+        B = rd(B);
+    }
+    const long long e = group_id<L>();
+    if (e >= (long long)m * n) return;
+    const int row = (int)(e % m), col = (int)(e / m);
+    Grp<N, L> g(C_);
+    MP<N / L> sum;
+    if (gemm_sum<N, L, FULL>(g, sum, ta, tb, k, A, lda, B, ldb, row, col))
+        store_aos(g, sum, sums, e);
+    else if (g.lane() == 0)
+        *sums.sign(e) = REDO;
+}
+// Epilogue per entry, with the marked sums recomputed first.
+// v1 (src/blas/gemm.cuh:127-166): buffer = sum*alpha (+round), C = C*beta (+round),
+//     C = C (+) buffer with the v1 matrix add (+round)
+// v2 (src/blas/v2/gemm_v2.cuh:76-78): C = (sum*alpha) (+) (C*beta)
+template <int N, int L, bool V2, class VM>
+__global__ __launch_bounds__(BLK) void k_gemm_fin(const DevConsts *__restrict__ C_, int ta, int tb, int m, int n,
+                                                  int k, VM alpha, VM A, int lda, VM B, int ldb, VM beta, VM Cm,
+                                                  int ldc, AoSRef<N> sums) {
+    if constexpr (!V2) {
+        alpha = rd(alpha);
+        A = rd(A);
+        B = rd(B);
+        beta = rd(beta);
+        Cm = rd(Cm);
+    }
+    const long long e = group_id<L>();
+    if (e >= (long long)m * n) return;
+    const int row = (int)(e % m), col = (int)(e / m);
+    const long long ic = (long long)col * ldc + row;
+    Grp<N, L> g(C_);
+    MP<N / L> sum, a, b, cv, s2, c2, r;
+    if (*sums.sign(e) == REDO)
+        gemm_sum<N, L, true>(g, sum, ta, tb, k, A, lda, B, ldb, row, col);
+    else
+        load_aos(g, sum, sums, e);
+    if constexpr (V2) {
+        load_aos(g, a, alpha, 0);
+        load_aos(g, b, beta, 0);
+        load_aos(g, cv, Cm, ic);
+    } else {
+        load_soa(g, a, alpha, 0);
+        load_soa(g, b, beta, 0);
+        load_soa(g, cv, Cm, ic);
+    }
+    g.mul(s2, sum, a);
+    g.mul(c2, cv, b);
+    if constexpr (V2) {
+        g.add(r, s2, c2);
+        store_aos(g, r, Cm, ic);
+    } else {
+        g.vadd(r, c2, s2);
+        store_soa(g, r, Cm, ic);
+    }
 }
 
 // ---------------------------------------------------------------- SpMV
 // Row partial sums of tpr strided lanes (src/sparse/spmv/spmv_csrv.cuh:61-65; tpr = 1
 // is the scalar kernel spmv_csr.cuh:53-62), combined by the 16/8/4/2/1 tree
 // (:66-80) through LDS. One workgroup holds BLK/L groups = (BLK/L)/tpr rows.
-template <int N, int L>
+// The fast kernel marks a row whose chain it abandoned with y[row].sign = REDO;
+// the fixup kernel recomputes those rows only.
+template <int N, int L, bool FULL, bool FIX>
 __global__ __launch_bounds__(BLK) void k_spmv(const DevConsts *__restrict__ C, int m, const int *__restrict__ irp,
                                               const int *__restrict__ ja, const double *__restrict__ as, AoSRef<N> x,
                                               AoSRef<N> y, int tpr) {
@@ -403,17 +427,15 @@ __global__ __launch_bounds__(BLK) void k_spmv(const DevConsts *__restrict__ C, i
     const long long v = group_id<L>();
     const long long row = v / tpr;
     const int lane = (int)(v % tpr);
-    const bool live = row < m;
+    bool live = row < m;
+    if (FIX && live) live = *y.sign(row) == REDO;
     Grp<N, L> g(C);
-    MP<R> acc, a, p;
+    MP<R> acc, p;
     g.zero(acc);
+    bool ok = true;
     if (live) {
         const int re = irp[row + 1];
-        for (int i = irp[row] + lane; i < re; i += tpr) {
-            load_aos(g, a, x, ja[i]);
-            g.mul_d(p, a, as[i]);
-            g.add(acc, acc, p);
-        }
+        for (int i = irp[row] + lane; i < re && ok; i += tpr) ok = g.template fma_d_step<FULL>(acc, x, ja[i], as[i]);
     }
     if (tpr > 1) {
         for (int h = tpr >> 1; h >= 1; h >>= 1) {
@@ -421,8 +443,8 @@ __global__ __launch_bounds__(BLK) void k_spmv(const DevConsts *__restrict__ C, i
             if (live && lane >= h && lane < 2 * h) {
 #pragma unroll
                 for (int r = 0; r < R; r++) sd[gl][g.ix(r)] = acc.d[r];
-                if (g.lane == 0) {
-                    ss[gl] = acc.sign;
+                if (g.lane() == 0) {
+                    ss[gl] = ok ? acc.sign : REDO;
                     se[gl] = acc.exp;
                     sf0[gl] = acc.ev0.frac;
                     sx0[gl] = acc.ev0.exp;
@@ -439,12 +461,18 @@ __global__ __launch_bounds__(BLK) void k_spmv(const DevConsts *__restrict__ C, i
                 p.exp = se[o];
                 p.ev0 = Er{sf0[o], sx0[o]};
                 p.ev1 = Er{sf1[o], sx1[o]};
-                g.add(acc, acc, p);
+                if (ok && p.sign == REDO) ok = false;
+                if (ok) ok = g.template add_try<FULL>(acc, acc, p);
             }
             __syncthreads();
         }
     }
-    if (live && lane == 0) store_aos(g, acc, y, row);
+    if (live && lane == 0) {
+        if (ok)
+            store_aos(g, acc, y, row);
+        else if (g.lane() == 0)
+            *y.sign(row) = REDO;
+    }
 }
 
 }  // namespace mph

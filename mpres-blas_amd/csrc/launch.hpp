@@ -46,17 +46,32 @@ struct Launch {
         }
         return hipGetLastError();
     }
-    static hipError_t fold(const DevConsts *c, int mode, long long cnt, long long nvt, SoA xs, int incx, SoA ys,
-                           int incy, const char *xa, const char *ya, char *out, hipStream_t st) {
+    // pass 0: the main kernel; pass 1: the fixup kernel over the chains the main
+    // kernel marked (16 and 32 moduli; nothing to do over 8 moduli)
+    static constexpr bool SPLIT = !Grp<N, L>::INL;
+    template <int MODE>
+    static void fold_pass(int pass, const dim3 &gr, const DevConsts *c, long long cnt, long long nvt, SoA xs, int incx,
+                          SoA ys, int incy, AoSRef<N> XA, AoSRef<N> YA, AoSRef<N> O, hipStream_t st) {
+        const dim3 bl(BLK);
+        if (pass == 0)
+            hipLaunchKernelGGL((k_fold<N, L, MODE, !SPLIT, false>), gr, bl, 0, st, c, cnt, nvt, xs, incx, ys, incy, XA,
+                               YA, O);
+        else if constexpr (SPLIT)
+            hipLaunchKernelGGL((k_fold<N, L, MODE, true, true>), gr, bl, 0, st, c, cnt, nvt, xs, incx, ys, incy, XA,
+                               YA, O);
+    }
+    static hipError_t fold(const DevConsts *c, int mode, int pass, long long cnt, long long nvt, SoA xs, int incx,
+                           SoA ys, int incy, const char *xa, const char *ya, char *out, hipStream_t st) {
         if (nvt <= 0) return hipSuccess;
         const dim3 gr(grid_for(nvt, L)), bl(BLK);
         AoSRef<N> XA{(char *)xa}, YA{(char *)ya}, O{out};
         if (mode == 0)
-            hipLaunchKernelGGL((k_fold<N, L, 0>), gr, bl, 0, st, c, cnt, nvt, xs, incx, ys, incy, XA, YA, O);
+            fold_pass<0>(pass, gr, c, cnt, nvt, xs, incx, ys, incy, XA, YA, O, st);
         else if (mode == 1)
-            hipLaunchKernelGGL((k_fold<N, L, 1>), gr, bl, 0, st, c, cnt, nvt, xs, incx, ys, incy, XA, YA, O);
-        else
-            hipLaunchKernelGGL((k_fold<N, L, 2>), gr, bl, 0, st, c, cnt, nvt, xs, incx, ys, incy, XA, YA, O);
+            fold_pass<1>(pass, gr, c, cnt, nvt, xs, incx, ys, incy, XA, YA, O, st);
+        else if (pass == 0)
+            hipLaunchKernelGGL((k_fold<N, L, 2, true, false>), gr, bl, 0, st, c, cnt, nvt, xs, incx, ys, incy, XA, YA,
+                               O);
         return hipGetLastError();
     }
     static hipError_t tree(const DevConsts *c, char *part, int b, unsigned p2, int nblocks, char *out, hipStream_t st) {
@@ -99,15 +114,23 @@ struct Launch {
                                AoSRef<N>{out});
         return hipGetLastError();
     }
-    static hipError_t gemv_fold(const DevConsts *c, bool trans, int m, int n, SoA A, int lda, const char *bx, int b3,
-                                long long nvt, char *part, hipStream_t st) {
+    template <bool TR>
+    static void gemv_pass(int pass, const DevConsts *c, int m, int n, SoA A, int lda, AoSRef<N> bx, int b3,
+                          long long nvt, AoSRef<N> part, hipStream_t st) {
+        const dim3 gr(grid_for(nvt, L)), bl(BLK);
+        if (pass == 0)
+            hipLaunchKernelGGL((k_gemv_fold<N, L, TR, !SPLIT, false>), gr, bl, 0, st, c, m, n, A, lda, bx, b3, nvt,
+                               part);
+        else if constexpr (SPLIT)
+            hipLaunchKernelGGL((k_gemv_fold<N, L, TR, true, true>), gr, bl, 0, st, c, m, n, A, lda, bx, b3, nvt, part);
+    }
+    static hipError_t gemv_fold(const DevConsts *c, bool trans, int pass, int m, int n, SoA A, int lda, const char *bx,
+                                int b3, long long nvt, char *part, hipStream_t st) {
         if (nvt <= 0) return hipSuccess;
         if (trans)
-            hipLaunchKernelGGL((k_gemv_fold<N, L, true>), dim3(grid_for(nvt, L)), dim3(BLK), 0, st, c, m, n, A, lda,
-                               AoSRef<N>{(char *)bx}, b3, nvt, AoSRef<N>{part});
+            gemv_pass<true>(pass, c, m, n, A, lda, AoSRef<N>{(char *)bx}, b3, nvt, AoSRef<N>{part}, st);
         else
-            hipLaunchKernelGGL((k_gemv_fold<N, L, false>), dim3(grid_for(nvt, L)), dim3(BLK), 0, st, c, m, n, A, lda,
-                               AoSRef<N>{(char *)bx}, b3, nvt, AoSRef<N>{part});
+            gemv_pass<false>(pass, c, m, n, A, lda, AoSRef<N>{(char *)bx}, b3, nvt, AoSRef<N>{part}, st);
         return hipGetLastError();
     }
     static hipError_t gemv_final(const DevConsts *c, int len, SoA y, int incy, const char *sums, hipStream_t st) {
@@ -116,28 +139,45 @@ struct Launch {
                            AoSRef<N>{(char *)sums});
         return hipGetLastError();
     }
-    static hipError_t gemm(const DevConsts *c, int m, int n, int k, SoA alpha, SoA A, int lda, SoA B, int ldb,
-                           SoA beta, SoA Cm, int ldc, hipStream_t st) {
+    // pass 0: the m*n sums into ws (m*n mp_float_t); pass 1: marked sums recomputed, epilogue
+    static hipError_t gemm(const DevConsts *c, int pass, int m, int n, int k, SoA alpha, SoA A, int lda, SoA B, int ldb,
+                           SoA beta, SoA Cm, int ldc, char *ws, hipStream_t st) {
         if ((long long)m * n <= 0) return hipSuccess;
-        hipLaunchKernelGGL((k_gemm<N, L>), dim3(grid_for((long long)m * n, L)), dim3(BLK), 0, st, c, m, n, k, alpha, A,
-                           lda, B, ldb, beta, Cm, ldc);
+        const dim3 gr(grid_for((long long)m * n, L)), bl(BLK);
+        if (pass == 0)
+            hipLaunchKernelGGL((k_gemm_sum<N, L, !SPLIT, SoA>), gr, bl, 0, st, c, 0, 0, m, n, k, A, lda, B, ldb,
+                               AoSRef<N>{ws});
+        else
+            hipLaunchKernelGGL((k_gemm_fin<N, L, false, SoA>), gr, bl, 0, st, c, 0, 0, m, n, k, alpha, A, lda, B, ldb,
+                               beta, Cm, ldc, AoSRef<N>{ws});
         return hipGetLastError();
     }
-    static hipError_t gemm_v2(const DevConsts *c, int ta, int tb, int m, int n, int k, const char *alpha,
+    static hipError_t gemm_v2(const DevConsts *c, int pass, int ta, int tb, int m, int n, int k, const char *alpha,
                               const char *A, int lda, const char *B, int ldb, const char *beta, char *Cm, int ldc,
-                              hipStream_t st) {
+                              char *ws, hipStream_t st) {
         if ((long long)m * n <= 0) return hipSuccess;
-        hipLaunchKernelGGL((k_gemm_v2<N, L>), dim3(grid_for((long long)m * n, L)), dim3(BLK), 0, st, c, ta, tb, m, n,
-                           k, AoSRef<N>{(char *)alpha}, AoSRef<N>{(char *)A}, lda, AoSRef<N>{(char *)B}, ldb,
-                           AoSRef<N>{(char *)beta}, AoSRef<N>{Cm}, ldc);
+        const dim3 gr(grid_for((long long)m * n, L)), bl(BLK);
+        const AoSRef<N> a{(char *)A}, b{(char *)B};
+        if (pass == 0)
+            hipLaunchKernelGGL((k_gemm_sum<N, L, !SPLIT, AoSRef<N>>), gr, bl, 0, st, c, ta, tb, m, n, k, a, lda, b,
+                               ldb, AoSRef<N>{ws});
+        else
+            hipLaunchKernelGGL((k_gemm_fin<N, L, true, AoSRef<N>>), gr, bl, 0, st, c, ta, tb, m, n, k,
+                               AoSRef<N>{(char *)alpha}, a, lda, b, ldb, AoSRef<N>{(char *)beta}, AoSRef<N>{Cm}, ldc,
+                               AoSRef<N>{ws});
         return hipGetLastError();
     }
-    static hipError_t spmv(const DevConsts *c, int m, const int *irp, const int *ja, const double *as, const char *x,
-                           char *y, int tpr, hipStream_t st) {
+    static hipError_t spmv(const DevConsts *c, int pass, int m, const int *irp, const int *ja, const double *as,
+                           const char *x, char *y, int tpr, hipStream_t st) {
         if (m <= 0) return hipSuccess;
         if (tpr > BLK / L) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((k_spmv<N, L>), dim3(grid_for((long long)m * tpr, L)), dim3(BLK), 0, st, c, m, irp, ja, as,
-                           AoSRef<N>{(char *)x}, AoSRef<N>{y}, tpr);
+        const dim3 gr(grid_for((long long)m * tpr, L)), bl(BLK);
+        if (pass == 0)
+            hipLaunchKernelGGL((k_spmv<N, L, !SPLIT, false>), gr, bl, 0, st, c, m, irp, ja, as, AoSRef<N>{(char *)x},
+                               AoSRef<N>{y}, tpr);
+        else if constexpr (SPLIT)
+            hipLaunchKernelGGL((k_spmv<N, L, true, true>), gr, bl, 0, st, c, m, irp, ja, as, AoSRef<N>{(char *)x},
+                               AoSRef<N>{y}, tpr);
         return hipGetLastError();
     }
     static Ops table() {
@@ -163,4 +203,12 @@ struct Launch {
 };
 
 }  // namespace mph
+
+#define MPH_INSTANTIATE(N, L)                                \
+    namespace mph {                                          \
+    const Ops *ops_n##N##_l##L() {                           \
+        static const Ops t = Launch<N, L>::table();          \
+        return &t;                                           \
+    }                                                        \
+    }
 #endif

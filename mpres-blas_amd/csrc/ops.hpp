@@ -27,7 +27,8 @@ struct Ops {
     int N, L;
     hipError_t (*set_d)(const DevConsts *, const double *, int, SoA, char *, bool, hipStream_t);
     hipError_t (*elem)(const DevConsts *, int, int, const char *, const char *, const double *, char *, hipStream_t);
-    hipError_t (*fold)(const DevConsts *, int, long long, long long, SoA, int, SoA, int, const char *,
+    // the reduction launchers take a pass: 0 = main kernel, 1 = fixup kernel (see kernels.hpp)
+    hipError_t (*fold)(const DevConsts *, int, int, long long, long long, SoA, int, SoA, int, const char *,
                        const char *, char *, hipStream_t);
     hipError_t (*tree)(const DevConsts *, char *, int, unsigned, int, char *, hipStream_t);
     hipError_t (*aos_to_soa)(const DevConsts *, const char *, SoA, int, hipStream_t);
@@ -35,20 +36,27 @@ struct Ops {
     hipError_t (*axpy)(const DevConsts *, int, SoA, SoA, int, SoA, int, hipStream_t);
     hipError_t (*axpy_v2)(const DevConsts *, int, const char *, const char *, const char *, char *, hipStream_t);
     hipError_t (*scal)(const DevConsts *, int, SoA, SoA, int, char *, bool, hipStream_t);
-    hipError_t (*gemv_fold)(const DevConsts *, bool, int, int, SoA, int, const char *, int, long long, char *,
+    hipError_t (*gemv_fold)(const DevConsts *, bool, int, int, int, SoA, int, const char *, int, long long, char *,
                             hipStream_t);
     hipError_t (*gemv_final)(const DevConsts *, int, SoA, int, const char *, hipStream_t);
-    hipError_t (*gemm)(const DevConsts *, int, int, int, SoA, SoA, int, SoA, int, SoA,
-                       SoA, int, hipStream_t);
-    hipError_t (*gemm_v2)(const DevConsts *, int, int, int, int, int, const char *, const char *, int, const char *,
-                          int, const char *, char *, int, hipStream_t);
-    hipError_t (*spmv)(const DevConsts *, int, const int *, const int *, const double *, const char *, char *, int,
+    hipError_t (*gemm)(const DevConsts *, int, int, int, int, SoA, SoA, int, SoA, int, SoA, SoA, int, char *,
                        hipStream_t);
+    hipError_t (*gemm_v2)(const DevConsts *, int, int, int, int, int, int, const char *, const char *, int,
+                          const char *, int, const char *, char *, int, char *, hipStream_t);
+    hipError_t (*spmv)(const DevConsts *, int, int, const int *, const int *, const double *, const char *, char *,
+                       int, hipStream_t);
 };
 
-// lanes-per-number values compiled for each moduli count
-const Ops *ops_n8(int L);
-const Ops *ops_n16(int L);
-const Ops *ops_n32(int L);
+// Compiled (moduli count, lanes per number) variants; each has its own
+// translation unit csrc/k_n<N>_l<L>.hip defining ops_n<N>_l<L>().
+#define MPH_VARIANTS(X) \
+    X(8, 1) X(8, 2) X(8, 8) X(16, 1) X(16, 2) X(16, 4) X(16, 16) X(32, 1) X(32, 2) X(32, 4) X(32, 8) X(32, 32)
+
+#define MPH_DECLARE(N, L) const Ops *ops_n##N##_l##L();
+MPH_VARIANTS(MPH_DECLARE)
+#undef MPH_DECLARE
+
+// table of variant (N, L), or nullptr when it is not compiled
+const Ops *find_variant(int N, int L);
 
 }  // namespace mph

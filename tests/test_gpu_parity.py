@@ -251,3 +251,117 @@ def test_spmv_powerlaw(name):
     m = 600
     irp, ja, as_ = gen.powerlaw_csr(62, m, kmax=200)
     _spmv_case(name, m, irp, ja, as_, gen.uexp(63, m, -10, 10), (1, 4, 32))
+
+
+# ---- chains that leave the common case at 16 and 32 moduli ----
+# Wide exponent ranges make aligned sums exceed the precision (rounding), and
+# exact cancellations make the sum's interval straddle zero (sign estimate by
+# mixed-radix comparison). Over 16 and 32 moduli such chains are abandoned by
+# the fast kernels and recomputed by the fixup kernels; every lanes variant
+# must still agree with the oracle bit for bit.
+WIDE = {"16_4double": (-150, 150), "32_8double": (-400, 400)}
+
+
+def _cancel_pairs(x, y, stride, frac, seed):
+    """y[i + stride] = -y[i], x[i + stride] = x[i] for a random subset of i"""
+    rng = np.random.default_rng(seed)
+    idx = np.nonzero(rng.random(len(x) - stride) < frac)[0]
+    idx = idx[((idx // stride) % 2) == 0]
+    x[idx + stride] = x[idx]
+    y[idx + stride] = -y[idx]
+    return x, y
+
+
+@pytest.mark.parametrize("name", ["16_4double", "32_8double"])
+def test_dot_rare(name):
+    c, o = ctx(name), orc(name)
+    n, g3, b3 = 4000, 16, 32
+    lo, hi = WIDE[name]
+    xs, ys = _cancel_pairs(gen.uexp(71, n, lo, hi), gen.uexp(72, n, lo, hi), g3 * b3, 0.3, 73)
+    # second half: narrow exponents with cancellations only
+    xs[n // 2:] = gen.uexp(74, n - n // 2, -10, 10)
+    ys[n // 2:] = gen.uexp(75, n - n // 2, -10, 10)
+    xs[n // 2:], ys[n // 2:] = _cancel_pairs(xs[n // 2:], ys[n // 2:], g3 * b3, 0.5, 76)
+    x, y = o.set_d(xs, D), o.set_d(ys, D)
+    ax, ay, r = c.array_from_host(x), c.array_from_host(y), c.array(1)
+    dx, dy, dr = c.aos(x), c.aos(y), c.aos(count=1)
+    want = o.dot(x, y, g3, b3, D)
+    for L in lanes_of(c):
+        c.set_lanes(mp.K_DOT, L)
+        note("dot_rare", name, L)
+        mp.check(c.L.mpres_dot(c.h, n, C.byref(ax), 1, C.byref(ay), 1, C.byref(r), g3, b3, None, 0, None))
+        same(c.array_to_host(r, 1), want, f"dot_rare {name} L={L}")
+        mp.check(c.L.mpres_dot_v2(c.h, n, dx.ptr, dy.ptr, dr.ptr, g3, b3, None, 0, None))
+        same(c.aos_to_host(dr, 1), want, f"dot_v2_rare {name} L={L}")
+
+
+@pytest.mark.parametrize("name", ["16_4double", "32_8double"])
+def test_gemv_rare(name):
+    c, o = ctx(name), orc(name)
+    m, n, b3 = 24, 96, 8
+    lo, hi = WIDE[name]
+    As = gen.uexp(81, m * n, lo, hi).reshape(n, m)  # column-major: row j of this array is column j of A
+    xs = gen.uexp(82, n, lo, hi)
+    # columns j and j + b3 equal with x[j + b3] = -x[j] for some j: exact cancellation in a strided sum
+    for j in range(0, n - b3, 2 * b3):
+        if j % 3 == 0:
+            As[j + b3] = As[j]
+            xs[j + b3] = -xs[j]
+    A = o.set_d(As.ravel(), D)
+    alpha, beta = o.set_d([1.5], D), o.set_d([-0.75], D)
+    x = o.set_d(xs, D)
+    y = o.set_d(gen.uexp(83, m, lo, hi), D)
+    want = o.gemv(0, m, n, alpha, A, m, x, beta, y, b3, D)
+    for L in lanes_of(c):
+        c.set_lanes(mp.K_GEMV, L)
+        note("gemv_rare", name, L)
+        aA, ax, ay = c.array_from_host(A), c.array_from_host(x), c.array_from_host(y)
+        aa, ab = c.array_from_host(alpha), c.array_from_host(beta)
+        mp.check(c.L.mpres_gemv(c.h, mp.NO_TRANS, m, n, C.byref(aa), C.byref(aA), m, C.byref(ax), 1, C.byref(ab),
+                                C.byref(ay), 1, b3, None, 0, None))
+        same(c.array_to_host(ay, m), want, f"gemv_rare {name} L={L}")
+
+
+@pytest.mark.parametrize("name", ["16_4double", "32_8double"])
+def test_gemm_rare(name):
+    c, o = ctx(name), orc(name)
+    m, n, k = 9, 8, 24
+    lo, hi = WIDE[name]
+    As = gen.uexp(91, m * k, lo, hi).reshape(k, m)  # column-major: As[i] is column i of A
+    Bs = gen.uexp(92, k * n, lo, hi).reshape(n, k)  # Bs[j] is column j of B
+    # A[:, i + 1] = A[:, i] and B[i + 1, j] = -B[i, j]: consecutive terms cancel for even i < 8
+    for i in range(0, 8, 2):
+        As[i + 1] = As[i]
+        Bs[:, i + 1] = -Bs[:, i]
+    A, B = o.set_d(As.ravel(), D), o.set_d(Bs.ravel(), D)
+    Cm = o.set_d(gen.uexp(93, m * n, lo, hi), D)
+    alpha, beta = o.set_d([0.5], D), o.set_d([3.0], D)
+    want = o.gemm(m, n, k, alpha, A, m, B, k, beta, Cm, m, D)
+    want2 = o.gemm_v2(0, 0, m, n, k, alpha, A, m, B, k, beta, Cm, m, D)
+    for L in lanes_of(c):
+        c.set_lanes(mp.K_GEMM, L)
+        note("gemm_rare", name, L)
+        aA, aB, aC = c.array_from_host(A), c.array_from_host(B), c.array_from_host(Cm)
+        aa, ab = c.array_from_host(alpha), c.array_from_host(beta)
+        mp.check(c.L.mpres_gemm(c.h, mp.NO_TRANS, mp.NO_TRANS, m, n, k, C.byref(aa), C.byref(aA), m, C.byref(aB), k,
+                                C.byref(ab), C.byref(aC), m, None, 0, None))
+        same(c.array_to_host(aC, m * n), want, f"gemm_rare {name} L={L}")
+        dA, dB, dC, da, db = c.aos(A), c.aos(B), c.aos(Cm), c.aos(alpha), c.aos(beta)
+        mp.check(c.L.mpres_gemm_v2(c.h, mp.NO_TRANS, mp.NO_TRANS, m, n, k, da.ptr, dA.ptr, m, dB.ptr, k, db.ptr,
+                                   dC.ptr, m, None))
+        same(c.aos_to_host(dC, m * n), want2, f"gemm_v2_rare {name} L={L}")
+
+
+@pytest.mark.parametrize("name", ["16_4double", "32_8double"])
+def test_spmv_rare(name):
+    m = 300
+    irp, ja, as_ = gen.powerlaw_csr(101, m, kmax=100)
+    lo, hi = WIDE[name]
+    xs = gen.uexp(102, m, lo, hi)
+    # rows whose first two entries hit one column with opposite values cancel exactly
+    for r in range(0, m, 3):
+        s, e = irp[r], irp[r + 1]
+        if e - s >= 2:
+            ja[s + 1] = ja[s]
+            as_[s + 1] = -as_[s]
+    _spmv_case(name, m, irp, ja, as_, xs, (1, 2, 8))
