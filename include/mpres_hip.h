@@ -144,9 +144,11 @@ int mpres_dot_v2(mpres_ctx_t *ctx, int n, const void *x, const void *y, void *r,
                  size_t ws_bytes, void *stream);
 /* cuda::mp_axpy(n,alpha,x,y,r), src/blas/v2/axpy_v2.cuh:39-52 */
 int mpres_axpy_v2(mpres_ctx_t *ctx, int n, const void *alpha, const void *x, const void *y, void *r, void *stream);
-/* cuda::mp_gemm(transa,transb,...), src/blas/v2/gemm_v2.cuh:55-81 (sums staged in the context's pool) */
+/* cuda::mp_gemm(transa,transb,...), src/blas/v2/gemm_v2.cuh:55-81.
+ * Workspace: the m*n sums, as for mpres_gemm (mpres_gemm_workspace). */
 int mpres_gemm_v2(mpres_ctx_t *ctx, int transa, int transb, int m, int n, int k, const void *alpha, const void *A,
-                  int lda, const void *B, int ldb, const void *beta, void *C, int ldc, void *stream);
+                  int lda, const void *B, int ldb, const void *beta, void *C, int ldc, void *ws, size_t ws_bytes,
+                  void *stream);
 
 /* ---- sparse: double CSR matrix times mp vector (x, y: mp_float_t arrays) ---- */
 /* cuda::mp_spmv_csr<threads>, src/sparse/spmv/spmv_csr.cuh:48-64 */
@@ -170,6 +172,7 @@ int mpres_profile_read(mpres_ctx_t *ctx, int op, double *ms, long *launches, int
  * Only values compiled into the library are accepted (mpres_lanes_supported). */
 int mpres_lanes_supported(const mpres_ctx_t *ctx, int lanes);
 int mpres_set_lanes(mpres_ctx_t *ctx, int op, int lanes);
+int mpres_get_lanes(const mpres_ctx_t *ctx, int op);  /* lanes in use, or a negative error */
 
 #ifdef __cplusplus
 }

@@ -46,12 +46,15 @@ enum OpKind { OP_ELEM = 0, OP_DOT = 1, OP_GEMV = 2, OP_GEMM = 3, OP_SPMV = 4, OP
 
 const Ops *find_ops(int n, int L) { return find_variant(n, L); }
 
-// Built-in lanes-per-number choice for each (moduli count, operation):
-// one lane per modulus everywhere unless a narrower group is compiled and
-// measured to be faster (the tuning tables override these).
+// Built-in lanes-per-number choice for each (moduli count, operation), from
+// measurements on MI355X (dot 2^16..2^24 at 8 moduli, gemv 4096^2 and gemm
+// 512^3 at 32 moduli, SpMV 10^6 rows at 16 moduli); mpres_set_lanes overrides.
 int default_lanes(int n, int op) {
-    (void)op;
-    return n;
+    switch (n) {
+        case 8: return 8;
+        case 16: return op == OP_SPMV ? 16 : 4;
+        default: return 8;
+    }
 }
 
 }  // namespace
@@ -566,21 +569,22 @@ int mpres_gemm(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const mp_arr
         HIPCHK(o->gemm(c->dev, 0, m, n, k, soa_of(alpha), soa_of(A), lda, soa_of(B), ldb, soa_of(beta), soa_of(C),
                        ldc, w, strm(st)));
     }
-    HIPCHK(o->gemm(c->dev, 1, m, n, k, soa_of(alpha), soa_of(A), lda, soa_of(B), ldb, soa_of(beta), soa_of(C), ldc, w,
-                   strm(st)));
+    for (int pass = 1; pass < 3; pass++)
+        HIPCHK(o->gemm(c->dev, pass, m, n, k, soa_of(alpha), soa_of(A), lda, soa_of(B), ldb, soa_of(beta), soa_of(C),
+                       ldc, w, strm(st)));
     return MPRES_OK;
 }
 
 int mpres_gemm_v2(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const void *alpha, const void *A, int lda,
-                  const void *B, int ldb, const void *beta, void *C, int ldc, void *st) {
+                  const void *B, int ldb, const void *beta, void *C, int ldc, void *ws, size_t wsb, void *st) {
     if (!c || !alpha || !A || !B || !beta || !C) return fail(MPRES_ERR_ARG, "null argument");
     if (m <= 0 || n <= 0) return MPRES_OK;
     if (k < 0 || lda <= 0 || ldb <= 0 || ldc < std::max(1, m)) return fail(MPRES_ERR_ARG, "invalid dimensions");
     char *w;
-    int rc = get_ws(c, nullptr, 0, mpres_gemm_workspace(c, m, n, k), &w);
+    int rc = get_ws(c, ws, wsb, mpres_gemm_workspace(c, m, n, k), &w);
     if (rc) return rc;
     const int tra = ta == MPRES_TRANS || ta == MPRES_CONJ_TRANS, trb = tb == MPRES_TRANS || tb == MPRES_CONJ_TRANS;
-    for (int pass = 0; pass < 2; pass++)
+    for (int pass = 0; pass < 3; pass++)
         HIPCHK(c->ops[OP_GEMM]->gemm_v2(c->dev, pass, tra, trb, m, n, k, (const char *)alpha, (const char *)A, lda,
                                         (const char *)B, ldb, (const char *)beta, (char *)C, ldc, w, strm(st)));
     return MPRES_OK;
@@ -653,6 +657,11 @@ int mpres_set_lanes(mpres_ctx_t *c, int op, int lanes) {
     if (!o) return fail(MPRES_ERR_UNSUPPORTED, "%d lanes per number not compiled for %d moduli", lanes, c->host.n);
     c->ops[op] = o;
     return MPRES_OK;
+}
+
+int mpres_get_lanes(const mpres_ctx_t *c, int op) {
+    if (!c || op < 0 || op >= OP_COUNT) return fail(MPRES_ERR_ARG, "invalid operation %d", op);
+    return c->ops[op]->L;
 }
 
 }  // extern "C"

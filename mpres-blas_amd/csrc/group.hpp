@@ -117,12 +117,16 @@ struct Grp {
         else return k.C->m[ix(r)];
     }
     __device__ __forceinline__ int mmul(int a, int b, int r) const { return mulmod(a, b, md(r), mv(r)); }
-    // Moduli read through a pointer the compiler must treat as new on every call:
-    // keeps a long multiply-accumulate loop from hoisting N moduli and reciprocals
-    // into registers (they are re-read from the scalar cache instead).
+    // Moduli read through a pointer the compiler must treat as new on every call
+    // (HINT): keeps the fast kernels' multiply-accumulate loops from hoisting N
+    // moduli and reciprocals into registers (they are re-read from the scalar
+    // cache instead). The full kernels do not use the hint.
+    template <bool HINT>
     __device__ __forceinline__ const DevConsts *fresh() const {
         const DevConsts *p = k.C;
-        if constexpr (!CACHE) asm volatile("" : "+s"(p));
+#if defined(__HIP_DEVICE_COMPILE__)
+        if constexpr (!CACHE && HINT) asm volatile("" : "+s"(p));
+#endif
         return p;
     }
     __device__ __forceinline__ double md(int r, const DevConsts *c) const {
@@ -368,7 +372,7 @@ struct Grp {
         const int sh = a.gamma + a.theta;
         const int *row = k.P2 + (size_t)prow(sh) * N;
         const int fs = shx ? a.fx : a.fy, fo = shx ? a.fy : a.fx;
-        const DevConsts *c = fresh();
+        const DevConsts *c = fresh<!FULL>();
 #pragma unroll
         for (int r = 0; r < R; r++)
             acc.d[r] = combine(acc.d[r], prod_digit(dx, dy, sig, dbl, r, c), row, shx, sh != 0, fs, fo, sign, r, c);
@@ -403,7 +407,7 @@ struct Grp {
             const int sh = a.gamma + a.theta;
             const int *row = k.P2 + (size_t)prow(sh) * N;
             const int fs = shx ? a.fx : a.fy, fo = shx ? a.fy : a.fx;
-            const DevConsts *c = fresh();
+            const DevConsts *c = fresh<!FULL>();
 #pragma unroll
             for (int r = 0; r < R; r++) res.d[r] = combine(x.d[r], y.d[r], row, shx, sh != 0, fs, fo, sign, r, c);
             res.sign = sign;

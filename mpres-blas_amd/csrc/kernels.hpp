@@ -344,8 +344,10 @@ __device__ __forceinline__ bool gemm_sum(const Grp<N, L> &g, MP<N / L> &sum, int
     }
     return true;
 }
-// sums of all m*n entries into the AoS workspace (REDO marks an abandoned chain)
-template <int N, int L, bool FULL, class VM>
+// sums of all m*n entries into the AoS workspace. FULL = false, FIX = false: the
+// fast kernel (REDO marks an abandoned chain); FULL = true, FIX = true: the fixup
+// kernel, recomputing the marked sums only; FULL = true, FIX = false: one full pass.
+template <int N, int L, bool FULL, bool FIX, class VM>
 __global__ __launch_bounds__(BLK) void k_gemm_sum(const DevConsts *__restrict__ C_, int ta, int tb, int m, int n,
                                                   int k, VM A, int lda, VM B, int ldb, AoSRef<N> sums) {
     if constexpr (std::is_same<VM, SoA>::value) {
@@ -354,6 +356,7 @@ __global__ __launch_bounds__(BLK) void k_gemm_sum(const DevConsts *__restrict__ 
     }
     const long long e = group_id<L>();
     if (e >= (long long)m * n) return;
+    if (FIX && *sums.sign(e) != REDO) return;
     const int row = (int)(e % m), col = (int)(e / m);
     Grp<N, L> g(C_);
     MP<N / L> sum;
@@ -362,18 +365,15 @@ __global__ __launch_bounds__(BLK) void k_gemm_sum(const DevConsts *__restrict__ 
     else if (g.lane() == 0)
         *sums.sign(e) = REDO;
 }
-// Epilogue per entry, with the marked sums recomputed first.
+// Epilogue per entry over the finished sums.
 // v1 (src/blas/gemm.cuh:127-166): buffer = sum*alpha (+round), C = C*beta (+round),
 //     C = C (+) buffer with the v1 matrix add (+round)
 // v2 (src/blas/v2/gemm_v2.cuh:76-78): C = (sum*alpha) (+) (C*beta)
 template <int N, int L, bool V2, class VM>
-__global__ __launch_bounds__(BLK) void k_gemm_fin(const DevConsts *__restrict__ C_, int ta, int tb, int m, int n,
-                                                  int k, VM alpha, VM A, int lda, VM B, int ldb, VM beta, VM Cm,
-                                                  int ldc, AoSRef<N> sums) {
+__global__ __launch_bounds__(BLK) void k_gemm_fin(const DevConsts *__restrict__ C_, int m, int n, VM alpha, VM beta,
+                                                  VM Cm, int ldc, AoSRef<N> sums) {
     if constexpr (!V2) {
         alpha = rd(alpha);
-        A = rd(A);
-        B = rd(B);
         beta = rd(beta);
         Cm = rd(Cm);
     }
@@ -383,10 +383,7 @@ __global__ __launch_bounds__(BLK) void k_gemm_fin(const DevConsts *__restrict__ 
     const long long ic = (long long)col * ldc + row;
     Grp<N, L> g(C_);
     MP<N / L> sum, a, b, cv, s2, c2, r;
-    if (*sums.sign(e) == REDO)
-        gemm_sum<N, L, true>(g, sum, ta, tb, k, A, lda, B, ldb, row, col);
-    else
-        load_aos(g, sum, sums, e);
+    load_aos(g, sum, sums, e);
     if constexpr (V2) {
         load_aos(g, a, alpha, 0);
         load_aos(g, b, beta, 0);

@@ -139,17 +139,28 @@ struct Launch {
                            AoSRef<N>{(char *)sums});
         return hipGetLastError();
     }
-    // pass 0: the m*n sums into ws (m*n mp_float_t); pass 1: marked sums recomputed, epilogue
+    // pass 0: the m*n sums into ws (m*n mp_float_t); pass 1: the marked sums
+    // recomputed (16 and 32 moduli); pass 2: the alpha/beta epilogue
+    template <class VM>
+    static void gemm_sums(int pass, const dim3 &gr, const DevConsts *c, int ta, int tb, int m, int n, int k, VM A,
+                          int lda, VM B, int ldb, char *ws, hipStream_t st) {
+        const dim3 bl(BLK);
+        if (pass == 0)
+            hipLaunchKernelGGL((k_gemm_sum<N, L, !SPLIT, false, VM>), gr, bl, 0, st, c, ta, tb, m, n, k, A, lda, B,
+                               ldb, AoSRef<N>{ws});
+        else if constexpr (SPLIT)
+            hipLaunchKernelGGL((k_gemm_sum<N, L, true, true, VM>), gr, bl, 0, st, c, ta, tb, m, n, k, A, lda, B, ldb,
+                               AoSRef<N>{ws});
+    }
     static hipError_t gemm(const DevConsts *c, int pass, int m, int n, int k, SoA alpha, SoA A, int lda, SoA B, int ldb,
                            SoA beta, SoA Cm, int ldc, char *ws, hipStream_t st) {
         if ((long long)m * n <= 0) return hipSuccess;
         const dim3 gr(grid_for((long long)m * n, L)), bl(BLK);
-        if (pass == 0)
-            hipLaunchKernelGGL((k_gemm_sum<N, L, !SPLIT, SoA>), gr, bl, 0, st, c, 0, 0, m, n, k, A, lda, B, ldb,
-                               AoSRef<N>{ws});
+        if (pass < 2)
+            gemm_sums<SoA>(pass, gr, c, 0, 0, m, n, k, A, lda, B, ldb, ws, st);
         else
-            hipLaunchKernelGGL((k_gemm_fin<N, L, false, SoA>), gr, bl, 0, st, c, 0, 0, m, n, k, alpha, A, lda, B, ldb,
-                               beta, Cm, ldc, AoSRef<N>{ws});
+            hipLaunchKernelGGL((k_gemm_fin<N, L, false, SoA>), gr, bl, 0, st, c, m, n, alpha, beta, Cm, ldc,
+                               AoSRef<N>{ws});
         return hipGetLastError();
     }
     static hipError_t gemm_v2(const DevConsts *c, int pass, int ta, int tb, int m, int n, int k, const char *alpha,
@@ -157,14 +168,12 @@ struct Launch {
                               char *ws, hipStream_t st) {
         if ((long long)m * n <= 0) return hipSuccess;
         const dim3 gr(grid_for((long long)m * n, L)), bl(BLK);
-        const AoSRef<N> a{(char *)A}, b{(char *)B};
-        if (pass == 0)
-            hipLaunchKernelGGL((k_gemm_sum<N, L, !SPLIT, AoSRef<N>>), gr, bl, 0, st, c, ta, tb, m, n, k, a, lda, b,
-                               ldb, AoSRef<N>{ws});
+        if (pass < 2)
+            gemm_sums<AoSRef<N>>(pass, gr, c, ta, tb, m, n, k, AoSRef<N>{(char *)A}, lda, AoSRef<N>{(char *)B}, ldb,
+                                 ws, st);
         else
-            hipLaunchKernelGGL((k_gemm_fin<N, L, true, AoSRef<N>>), gr, bl, 0, st, c, ta, tb, m, n, k,
-                               AoSRef<N>{(char *)alpha}, a, lda, b, ldb, AoSRef<N>{(char *)beta}, AoSRef<N>{Cm}, ldc,
-                               AoSRef<N>{ws});
+            hipLaunchKernelGGL((k_gemm_fin<N, L, true, AoSRef<N>>), gr, bl, 0, st, c, m, n, AoSRef<N>{(char *)alpha},
+                               AoSRef<N>{(char *)beta}, AoSRef<N>{Cm}, ldc, AoSRef<N>{ws});
         return hipGetLastError();
     }
     static hipError_t spmv(const DevConsts *c, int pass, int m, const int *irp, const int *ja, const double *as,

@@ -27,7 +27,8 @@ struct Ops {
     int N, L;
     hipError_t (*set_d)(const DevConsts *, const double *, int, SoA, char *, bool, hipStream_t);
     hipError_t (*elem)(const DevConsts *, int, int, const char *, const char *, const double *, char *, hipStream_t);
-    // the reduction launchers take a pass: 0 = main kernel, 1 = fixup kernel (see kernels.hpp)
+    // the reduction launchers take a pass: 0 = main kernel, 1 = fixup kernel (see kernels.hpp);
+    // gemm and gemm_v2 have a pass 2, the alpha/beta epilogue
     hipError_t (*fold)(const DevConsts *, int, int, long long, long long, SoA, int, SoA, int, const char *,
                        const char *, char *, hipStream_t);
     hipError_t (*tree)(const DevConsts *, char *, int, unsigned, int, char *, hipStream_t);

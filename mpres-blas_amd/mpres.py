@@ -83,13 +83,14 @@ def lib():
         L.mpres_gemm_workspace.restype = sz
         L.mpres_gemm_workspace.argtypes = [vp, i32, i32, i32]
         L.mpres_gemm.argtypes = [vp, i32, i32, i32, i32, i32, vp, vp, i32, vp, i32, vp, vp, i32, vp, sz, vp]
-        L.mpres_gemm_v2.argtypes = [vp, i32, i32, i32, i32, i32, vp, vp, i32, vp, i32, vp, vp, i32, vp]
+        L.mpres_gemm_v2.argtypes = [vp, i32, i32, i32, i32, i32, vp, vp, i32, vp, i32, vp, vp, i32, vp, sz, vp]
         L.mpres_spmv_csr.argtypes = [vp, i32, vp, vp, vp, vp]
         L.mpres_spmv_csrv.argtypes = [vp, i32, vp, i32, vp, vp, vp]
         L.mpres_profile.argtypes = [vp, i32]
         L.mpres_profile_read.argtypes = [vp, i32, C.POINTER(C.c_double), C.POINTER(C.c_long), i32]
         L.mpres_lanes_supported.argtypes = [vp, i32]
         L.mpres_set_lanes.argtypes = [vp, i32, i32]
+        L.mpres_get_lanes.argtypes = [vp, i32]
         _lib = L
     return _lib
 

@@ -1,0 +1,101 @@
+// CPU emulation of the library's reduction and gemm kernels for one lane per
+// number (L = 1): the kernel templates of mpres-blas_amd/csrc/kernels.hpp are
+// compiled with g++ against a host stand-in of the HIP header and every
+// (block, thread) is run in turn. This checks the kernels' own logic (fast
+// path, marking, fixup, epilogues) against the oracle on the CPU, where a
+// mistake can be found with ordinary tools. Built by tests/test_kernels_emu.py.
+#include <vector>
+
+#include "../../mpres-blas_amd/csrc/internal.hpp"
+#include "../../mpres-blas_amd/csrc/kernels.hpp"
+
+EmuIdx blockIdx, threadIdx;
+
+using namespace mph;
+
+namespace {
+
+struct Ctx {
+    DevConsts c;
+    std::vector<int> pow2;
+};
+
+bool make_ctx(int set_id, Ctx &x) {
+    const ModuliSet *s = get_set(set_id);
+    if (!s || !compute_consts(*s, x.c, x.pow2)) return false;
+    x.c.pow2 = x.pow2.data();
+    return true;
+}
+
+// run kernel body f over `groups` groups of one lane each, BLK threads per block
+template <class F>
+void run(long long groups, F &&f) {
+    const long long blocks = (groups + BLK - 1) / BLK;
+    for (long long b = 0; b < blocks; b++)
+        for (unsigned t = 0; t < (unsigned)BLK; t++) {
+            blockIdx = EmuIdx{(unsigned)b, 0, 0};
+            threadIdx = EmuIdx{t, 0, 0};
+            f();
+        }
+}
+
+template <int N>
+int gemm_v2_n(const Ctx &x, int ta, int tb, int m, int n, int k, char *alpha, char *A, int lda, char *B, int ldb,
+              char *beta, char *Cm, int ldc, char *ws, int split) {
+    const long long e = (long long)m * n;
+    const AoSRef<N> a{A}, b{B}, w{ws};
+    if (split) {
+        run(e, [&] { k_gemm_sum<N, 1, false, false, AoSRef<N>>(&x.c, ta, tb, m, n, k, a, lda, b, ldb, w); });
+        run(e, [&] { k_gemm_sum<N, 1, true, true, AoSRef<N>>(&x.c, ta, tb, m, n, k, a, lda, b, ldb, w); });
+    } else {
+        run(e, [&] { k_gemm_sum<N, 1, true, false, AoSRef<N>>(&x.c, ta, tb, m, n, k, a, lda, b, ldb, w); });
+    }
+    run(e, [&] {
+        k_gemm_fin<N, 1, true, AoSRef<N>>(&x.c, m, n, AoSRef<N>{alpha}, AoSRef<N>{beta}, AoSRef<N>{Cm}, ldc, w);
+    });
+    return 0;
+}
+
+template <int N>
+int dot_v2_pass1_n(const Ctx &x, long long cnt, long long nvt, char *X, char *Y, char *out, int split) {
+    SoA none{};
+    const AoSRef<N> xa{X}, ya{Y}, o{out};
+    if (split) {
+        run(nvt, [&] { k_fold<N, 1, 1, false, false>(&x.c, cnt, nvt, none, 1, none, 1, xa, ya, o); });
+        run(nvt, [&] { k_fold<N, 1, 1, true, true>(&x.c, cnt, nvt, none, 1, none, 1, xa, ya, o); });
+    } else {
+        run(nvt, [&] { k_fold<N, 1, 1, true, false>(&x.c, cnt, nvt, none, 1, none, 1, xa, ya, o); });
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// v2 gemm (AoS) on the CPU; ws holds m*n mp_float_t; split = 1: fast kernel + fixup epilogue
+int emu_gemm_v2(int set_id, int ta, int tb, int m, int n, int k, void *alpha, void *A, int lda, void *B, int ldb,
+                void *beta, void *Cm, int ldc, void *ws, int split) {
+    Ctx x;
+    if (!make_ctx(set_id, x)) return -1;
+    switch (x.c.n) {
+        case 8: return gemm_v2_n<8>(x, ta, tb, m, n, k, (char *)alpha, (char *)A, lda, (char *)B, ldb, (char *)beta, (char *)Cm, ldc, (char *)ws, split);
+        case 16: return gemm_v2_n<16>(x, ta, tb, m, n, k, (char *)alpha, (char *)A, lda, (char *)B, ldb, (char *)beta, (char *)Cm, ldc, (char *)ws, split);
+        case 32: return gemm_v2_n<32>(x, ta, tb, m, n, k, (char *)alpha, (char *)A, lda, (char *)B, ldb, (char *)beta, (char *)Cm, ldc, (char *)ws, split);
+    }
+    return -1;
+}
+
+// first pass of the v2 dot: the nvt strided partial sums of x_i * y_i
+int emu_dot_v2_pass1(int set_id, long long cnt, long long nvt, void *X, void *Y, void *out, int split) {
+    Ctx x;
+    if (!make_ctx(set_id, x)) return -1;
+    switch (x.c.n) {
+        case 8: return dot_v2_pass1_n<8>(x, cnt, nvt, (char *)X, (char *)Y, (char *)out, split);
+        case 16: return dot_v2_pass1_n<16>(x, cnt, nvt, (char *)X, (char *)Y, (char *)out, split);
+        case 32: return dot_v2_pass1_n<32>(x, cnt, nvt, (char *)X, (char *)Y, (char *)out, split);
+    }
+    return -1;
+}
+
+}  // extern "C"

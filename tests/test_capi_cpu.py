@@ -88,3 +88,16 @@ def test_errors_without_device(mp):
     rc = L.mpres_ctx_create(C.byref(h), 999, 0)
     assert rc == -1  # unknown moduli set -> MPRES_ERR_ARG
     assert b"unknown moduli set" in L.mpres_last_error()
+
+
+def test_compat_header_compiles_and_links(tmp_path):
+    """The header-only layer with the reference's cuda::mp_* names forwards to
+    the C ABI: a program using each name compiles and links."""
+    import subprocess
+
+    out = tmp_path / "compat_calls"
+    cmd = ["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "host", "compat_calls.cpp"), "-o", str(out),
+           "-L", os.path.join(ROOT, "mpres-blas_amd"), "-lmpres_hip"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

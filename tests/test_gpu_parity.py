@@ -220,7 +220,7 @@ def test_gemm(name, m, n, k):
             note("gemm_v2", name, ta, tb, L)
             dA, dB, dC, da, db = c.aos(A), c.aos(B), c.aos(Cm), c.aos(alpha), c.aos(beta)
             mp.check(c.L.mpres_gemm_v2(c.h, mp.TRANS if ta else mp.NO_TRANS, mp.TRANS if tb else mp.NO_TRANS, m, n, k,
-                                       da.ptr, dA.ptr, lda, dB.ptr, ldb, db.ptr, dC.ptr, m, None))
+                                       da.ptr, dA.ptr, lda, dB.ptr, ldb, db.ptr, dC.ptr, m, None, 0, None))
             same(c.aos_to_host(dC, m * n), want2, f"gemm_v2 {name} {ta}{tb} L={L}")
 
 
@@ -348,7 +348,7 @@ def test_gemm_rare(name):
         same(c.array_to_host(aC, m * n), want, f"gemm_rare {name} L={L}")
         dA, dB, dC, da, db = c.aos(A), c.aos(B), c.aos(Cm), c.aos(alpha), c.aos(beta)
         mp.check(c.L.mpres_gemm_v2(c.h, mp.NO_TRANS, mp.NO_TRANS, m, n, k, da.ptr, dA.ptr, m, dB.ptr, k, db.ptr,
-                                   dC.ptr, m, None))
+                                   dC.ptr, m, None, 0, None))
         same(c.aos_to_host(dC, m * n), want2, f"gemm_v2_rare {name} L={L}")
 
 

@@ -1,0 +1,87 @@
+"""The library's reduction kernels, compiled for the CPU with g++ against a
+host stand-in of the HIP header (tests/host/emu) and run one lane per number,
+against the oracle. This checks the kernels' own logic - the fast kernels'
+abandoning and marking of rare chains, the fixup kernels, the gemm epilogues -
+on the CPU, at 8, 16 and 32 moduli, including chains that need rounding and
+exact cancellations."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import gen
+import pyoracle as po
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+D = po.POLICY_D
+SETS = {"8_2double": (-20, 20), "16_4double": (-150, 150), "32_8double": (-400, 400)}
+
+
+@pytest.fixture(scope="module")
+def emu(tmp_path_factory):
+    out = tmp_path_factory.mktemp("emu") / "libkemu.so"
+    cmd = ["g++", "-std=c++17", "-O1", "-fPIC", "-shared", "-ffp-contract=off",
+           "-I", os.path.join(ROOT, "tests", "host", "emu"),
+           os.path.join(ROOT, "tests", "host", "kernels_emu.cpp"),
+           os.path.join(ROOT, "mpres-blas_amd", "csrc", "consts.cpp"), "-o", str(out)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return C.CDLL(str(out))
+
+
+def P(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _gemm_inputs(o, lo, hi, m, n, k):
+    As = gen.uexp(91, m * k, lo, hi).reshape(k, m)
+    Bs = gen.uexp(92, k * n, lo, hi).reshape(n, k)
+    for i in range(0, min(k - 1, 8), 2):  # exact cancellations between consecutive terms
+        As[i + 1] = As[i]
+        Bs[:, i + 1] = -Bs[:, i]
+    A, B = o.set_d(As.ravel(), D), o.set_d(Bs.ravel(), D)
+    Cm = o.set_d(gen.uexp(93, m * n, lo, hi), D)
+    return A, B, Cm
+
+
+@pytest.mark.parametrize("name", list(SETS))
+@pytest.mark.parametrize("k", [1, 2, 5, 24])
+def test_gemm_v2_fast_fixup_epilogue(emu, name, k):
+    o = po.Oracle(name)
+    sid = po.set_names().index(name)
+    lo, hi = SETS[name]
+    m, n = 9, 8
+    A, B, Cm = _gemm_inputs(o, lo, hi, m, n, k)
+    alpha, beta = o.set_d([0.5], D), o.set_d([3.0], D)
+    want = o.gemm_v2(0, 0, m, n, k, alpha, A, m, B, k, beta, Cm, m, D)
+    for split in (1, 0):
+        got, ws = Cm.copy(), o.empty(m * n)
+        assert emu.emu_gemm_v2(sid, 0, 0, m, n, k, P(alpha), P(A), m, P(B), k, P(beta), P(got), m, P(ws), split) == 0
+        assert got.tobytes() == want.tobytes(), (name, k, split)
+        assert not (ws["sign"] == 2).any()  # every marked sum was recomputed
+
+
+@pytest.mark.parametrize("name", list(SETS))
+def test_dot_pass1_fast_fixup(emu, name):
+    """First dot pass (strided partial sums) with rounding and cancellations,
+    against the oracle's dot with one block (g3 = 1), whose first-pass partials
+    are then reduced by the same tree."""
+    o = po.Oracle(name)
+    sid = po.set_names().index(name)
+    lo, hi = SETS[name]
+    cnt, nvt = 600, 32
+    xs, ys = gen.uexp(71, cnt, lo, hi), gen.uexp(72, cnt, lo, hi)
+    idx = np.arange(0, cnt - nvt, 3 * nvt)
+    xs[idx + nvt], ys[idx + nvt] = xs[idx], -ys[idx]
+    x, y = o.set_d(xs, D), o.set_d(ys, D)
+    for split in (1, 0):
+        part = o.empty(nvt)
+        assert emu.emu_dot_v2_pass1(sid, cnt, nvt, P(x), P(y), P(part), split) == 0
+        # partial v = sum over i = v, v + nvt, ... of x_i * y_i, in order, from zero
+        for v in range(nvt):
+            acc = o.set_d([0.0], D)
+            for i in range(v, cnt, nvt):
+                acc = o.add(acc, o.mul(x[i:i + 1], y[i:i + 1], D), D)
+            assert part[v].tobytes() == acc[0].tobytes(), (name, split, v)
