@@ -162,7 +162,8 @@ int mpres_spmv_csrv(mpres_ctx_t *ctx, int m, const csr_t *csr, int threads_per_r
  * reduction pass), gemv (strided row sums), gemm (sums and epilogue), SpMV and axpy call with HIP
  * events on the call's stream. mpres_profile_read waits for the recorded
  * events and returns the accumulated kernel time and launch count of one
- * operation class (same op numbering as mpres_set_lanes). */
+ * operation class (same op numbering as mpres_set_lanes; op 6: the residue
+ * products of gemm over 16 and 32 moduli, whose op 3 time is the scalar chains). */
 int mpres_profile(mpres_ctx_t *ctx, int enable);
 int mpres_profile_read(mpres_ctx_t *ctx, int op, double *ms, long *launches, int reset);
 

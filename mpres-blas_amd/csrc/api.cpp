@@ -43,6 +43,8 @@ int fail(int code, const char *fmt, ...) {
     } while (0)
 
 enum OpKind { OP_ELEM = 0, OP_DOT = 1, OP_GEMV = 2, OP_GEMM = 3, OP_SPMV = 4, OP_AXPY = 5, OP_COUNT = 6 };
+// extra timing slot: the residue products of the split-form gemm
+constexpr int PROF_GEMM_RESID = 6, PROF_SLOTS = 7;
 
 const Ops *find_ops(int n, int L) { return find_variant(n, L); }
 
@@ -65,6 +67,7 @@ struct mpres_ctx {
     DevConsts host{};
     DevConsts *dev = nullptr;
     int *dpow2 = nullptr;
+    int bits = 0;  // bit length of the largest modulus
     const Ops *ops[OP_COUNT] = {};
     void *ws = nullptr;
     size_t ws_bytes = 0;
@@ -136,7 +139,7 @@ struct ProfScope {
     std::pair<hipEvent_t, hipEvent_t> ev{};
     bool on = false;
     ProfScope(mpres_ctx *c_, int op_, hipStream_t st_) : c(c_), op(op_), st(st_) {
-        if (!c->prof) return;
+        if (!c->prof || op < 0) return;
         if (c->prof_free.empty()) {
             hipEvent_t a, b;
             if (hipEventCreate(&a) != hipSuccess) return;
@@ -195,6 +198,8 @@ int mpres_ctx_create(mpres_ctx_t **out, int set_id, int device) {
         delete c;
         return fail(MPRES_ERR_ARG, "moduli set %s: inconsistent RNS_MODULI_PRODUCT_LOG2", s->name);
     }
+    for (int i = 0; i < s->n; i++)
+        while ((1LL << c->bits) <= s->m[i]) c->bits++;
     for (int op = 0; op < OP_COUNT; op++) {
         c->ops[op] = find_ops(s->n, default_lanes(s->n, op));
         if (!c->ops[op]) {
@@ -544,9 +549,9 @@ int mpres_gemv(mpres_ctx_t *c, int trans, int m, int n, const mp_array_t *alpha,
 
 // ---------------------------------------------------------------- gemm
 size_t mpres_gemm_workspace(const mpres_ctx_t *c, int m, int n, int k) {
-    (void)k;
-    if (!c || m <= 0 || n <= 0) return 0;
-    return mpsize(c) * (size_t)m * (size_t)n;  // the m*n dot-product sums
+    if (!c || m <= 0 || n <= 0 || k < 0) return 0;
+    // the m*n dot-product sums, and over 16/32 moduli the pre-scaled residue planes
+    return c->ops[OP_GEMM]->gemm_ws(m, n, k);
 }
 
 int mpres_gemm(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const mp_array_t *alpha, const mp_array_t *A,
@@ -564,14 +569,11 @@ int mpres_gemm(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const mp_arr
     int rc = get_ws(c, ws, wsb, mpres_gemm_workspace(c, m, n, k), &w);
     if (rc) return rc;
     const Ops *o = c->ops[OP_GEMM];
-    {
-        ProfScope ps(c, OP_GEMM, strm(st));
-        HIPCHK(o->gemm(c->dev, 0, m, n, k, soa_of(alpha), soa_of(A), lda, soa_of(B), ldb, soa_of(beta), soa_of(C),
-                       ldc, w, strm(st)));
+    for (int pass = 0; pass < 5; pass++) {
+        ProfScope ps(c, pass == 1 ? OP_GEMM : pass == 2 ? PROF_GEMM_RESID : -1, strm(st));
+        HIPCHK(o->gemm(c->dev, pass, c->bits, m, n, k, soa_of(alpha), soa_of(A), lda, soa_of(B), ldb, soa_of(beta),
+                       soa_of(C), ldc, w, strm(st)));
     }
-    for (int pass = 1; pass < 3; pass++)
-        HIPCHK(o->gemm(c->dev, pass, m, n, k, soa_of(alpha), soa_of(A), lda, soa_of(B), ldb, soa_of(beta), soa_of(C),
-                       ldc, w, strm(st)));
     return MPRES_OK;
 }
 
@@ -584,8 +586,8 @@ int mpres_gemm_v2(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const voi
     int rc = get_ws(c, ws, wsb, mpres_gemm_workspace(c, m, n, k), &w);
     if (rc) return rc;
     const int tra = ta == MPRES_TRANS || ta == MPRES_CONJ_TRANS, trb = tb == MPRES_TRANS || tb == MPRES_CONJ_TRANS;
-    for (int pass = 0; pass < 3; pass++)
-        HIPCHK(c->ops[OP_GEMM]->gemm_v2(c->dev, pass, tra, trb, m, n, k, (const char *)alpha, (const char *)A, lda,
+    for (int pass = 0; pass < 5; pass++)
+        HIPCHK(c->ops[OP_GEMM]->gemm_v2(c->dev, pass, c->bits, tra, trb, m, n, k, (const char *)alpha, (const char *)A, lda,
                                         (const char *)B, ldb, (const char *)beta, (char *)C, ldc, w, strm(st)));
     return MPRES_OK;
 }
@@ -627,7 +629,7 @@ int mpres_profile(mpres_ctx_t *c, int enable) {
 }
 
 int mpres_profile_read(mpres_ctx_t *c, int op, double *ms, long *launches, int reset) {
-    if (!c || op < 0 || op >= OP_COUNT) return fail(MPRES_ERR_ARG, "invalid operation %d", op);
+    if (!c || op < 0 || op >= PROF_SLOTS) return fail(MPRES_ERR_ARG, "invalid operation %d", op);
     for (auto &e : c->prof_live) {
         float t = 0;
         HIPCHK(hipEventSynchronize(e.second.second));

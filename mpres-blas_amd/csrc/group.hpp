@@ -76,6 +76,81 @@ __device__ __forceinline__ void scalars_of(const AoSRef<N> &v, long long i, int 
     e1 = Er{up.frac, (int)up.exp};
 }
 
+// ---------------- scalar fields alone ----------------
+// Sign, exponent and interval evaluation of a number. The split form of gemm
+// runs a reduction's chain on these fields only and forms the residues
+// separately (kernels.hpp).
+struct Sc {
+    int sign;
+    int exp;
+    Er ev0, ev1;
+};
+
+// exponent alignment shared by mp_add_common (src/arith/add.cuh:126-160) and the
+// v1 vector/matrix add esi kernels (src/mpvector.cuh:275-299)
+struct Align {
+    int gamma, theta, nzx, nzy, sx, sy, fx, fy, ex, ey;
+    Er x0, x1, y0, y1;
+};
+__device__ __forceinline__ Align align_sc(int mp_j, int xs, int xe, Er xv0, Er xv1, int ys, int ye, Er yv0, Er yv1) {
+    Align a;
+    const int dexp = xe - ye;
+    a.gamma = dexp > 0 ? dexp : 0;
+    a.theta = dexp < 0 ? -dexp : 0;
+    a.nzx = (yv1.frac == 0) || (a.theta + yv1.exp < mp_j);
+    a.nzy = (xv1.frac == 0) || (a.gamma + xv1.exp < mp_j);
+    a.gamma *= a.nzy;
+    a.theta *= a.nzx;
+    a.ex = (xe - a.gamma) * a.nzx;
+    a.ey = (ye - a.theta) * a.nzy;
+    a.sx = xs * a.nzx;
+    a.sy = ys * a.nzy;
+    a.fx = (1 - 2 * a.sx) * a.nzx;
+    a.fy = (1 - 2 * a.sy) * a.nzy;
+    a.x0 = Er{xv0.frac * (double)a.fx, xv0.exp + a.gamma};
+    a.x1 = Er{xv1.frac * (double)a.fx, xv1.exp + a.gamma};
+    a.y0 = Er{yv0.frac * (double)a.fy, yv0.exp + a.theta};
+    a.y1 = Er{yv1.frac * (double)a.fy, yv1.exp + a.theta};
+    return a;
+}
+__device__ __forceinline__ bool sc_needs_round(const DevConsts *C, const Er &ev1) {
+    return ev1.frac != 0 && ev1.exp >= C->mp_h;
+}
+// scalar part of cuda::mp_mul (mp_mul_common, src/arith/mul.cuh:35-42)
+__device__ __forceinline__ void sc_mul(const DevConsts *C, Sc &r, int xs, int xe, Er x0, Er x1, int ys, int ye, Er y0,
+                                       Er y1) {
+    r.exp = xe + ye;
+    r.sign = xs ^ ys;
+    r.ev0 = er_md_rd(x0, y0, Er{C->unit_upp.frac, C->unit_upp.exp});
+    r.ev1 = er_md_ru(x1, y1, Er{C->unit_low.frac, C->unit_low.exp});
+}
+// acc = acc + x*y on the scalar fields, for chains whose every step is exact:
+// false when the product or the sum needs rounding, the sum's interval
+// straddles zero, or the alignment drops a nonzero operand. The decisions are
+// those of the lane-group step (Grp::step_tail).
+__device__ __forceinline__ bool sc_fma_step(const DevConsts *C, Sc &acc, int xs, int xe, Er x0, Er x1, int ys, int ye,
+                                            Er y0, Er y1) {
+    Sc p;
+    sc_mul(C, p, xs, xe, x0, x1, ys, ye, y0, y1);
+    if (sc_needs_round(C, p.ev1)) return false;
+    const Align a = align_sc(C->mp_j, acc.sign, acc.exp, acc.ev0, acc.ev1, p.sign, p.exp, p.ev0, p.ev1);
+    if ((!a.nzx && acc.ev1.frac != 0) || (!a.nzy && p.ev1.frac != 0)) return false;
+    const Er e0 = er_add_dir(a.sx ? a.x1 : a.x0, a.sy ? a.y1 : a.y0, false);
+    const Er e1 = er_add_dir(a.sx ? a.x0 : a.x1, a.sy ? a.y0 : a.y1, true);
+    const int sign = e0.frac < 0;
+    if (sign != (e1.frac < 0)) return false;
+    acc.sign = sign;
+    acc.exp = (a.ex == 0) ? a.ey : a.ex;
+    if (sign) {
+        acc.ev0 = Er{-e1.frac, e1.exp};
+        acc.ev1 = Er{-1.0 * e0.frac, e0.exp};
+    } else {
+        acc.ev0 = e0;
+        acc.ev1 = e1;
+    }
+    return !sc_needs_round(C, acc.ev1);
+}
+
 template <int N, int L>
 struct Grp {
     static_assert(N % L == 0, "L must divide N");
@@ -200,33 +275,8 @@ struct Grp {
         a.ev1 = Er{0.0, 0};
     }
 
-    // exponent alignment shared by mp_add_common (src/arith/add.cuh:126-160) and the
-    // v1 vector/matrix add esi kernels (src/mpvector.cuh:275-299)
-    struct Align {
-        int gamma, theta, nzx, nzy, sx, sy, fx, fy, ex, ey;
-        Er x0, x1, y0, y1;
-    };
     __device__ __forceinline__ Align align(int xs, int xe, Er xv0, Er xv1, int ys, int ye, Er yv0, Er yv1) const {
-        Align a;
-        const int dexp = xe - ye;
-        a.gamma = dexp > 0 ? dexp : 0;
-        a.theta = dexp < 0 ? -dexp : 0;
-        const int mpj = k.C->mp_j;
-        a.nzx = (yv1.frac == 0) || (a.theta + yv1.exp < mpj);
-        a.nzy = (xv1.frac == 0) || (a.gamma + xv1.exp < mpj);
-        a.gamma *= a.nzy;
-        a.theta *= a.nzx;
-        a.ex = (xe - a.gamma) * a.nzx;
-        a.ey = (ye - a.theta) * a.nzy;
-        a.sx = xs * a.nzx;
-        a.sy = ys * a.nzy;
-        a.fx = (1 - 2 * a.sx) * a.nzx;
-        a.fy = (1 - 2 * a.sy) * a.nzy;
-        a.x0 = Er{xv0.frac * (double)a.fx, xv0.exp + a.gamma};
-        a.x1 = Er{xv1.frac * (double)a.fx, xv1.exp + a.gamma};
-        a.y0 = Er{yv0.frac * (double)a.fy, yv0.exp + a.theta};
-        a.y1 = Er{yv1.frac * (double)a.fy, yv1.exp + a.theta};
-        return a;
+        return align_sc(k.C->mp_j, xs, xe, xv0, xv1, ys, ye, yv0, yv1);
     }
     __device__ __forceinline__ Align align(const MP<R> &x, const MP<R> &y) const {
         return align(x.sign, x.exp, x.ev0, x.ev1, y.sign, y.exp, y.ev0, y.ev1);
@@ -281,10 +331,12 @@ struct Grp {
     // scalar part of cuda::mp_mul (mp_mul_common, src/arith/mul.cuh:35-42)
     __device__ __forceinline__ void mul_scalars(MP<R> &res, int xs, int xe, Er x0, Er x1, int ys, int ye, Er y0,
                                                 Er y1) const {
-        res.exp = xe + ye;
-        res.sign = xs ^ ys;
-        res.ev0 = er_md_rd(x0, y0, unit_upp());
-        res.ev1 = er_md_ru(x1, y1, unit_low());
+        Sc p;
+        sc_mul(k.C, p, xs, xe, x0, x1, ys, ye, y0, y1);
+        res.sign = p.sign;
+        res.exp = p.exp;
+        res.ev0 = p.ev0;
+        res.ev1 = p.ev1;
     }
     // cuda::mp_mul, src/arith/mul.cuh:53-74
     __device__ __forceinline__ void mul(MP<R> &res, const MP<R> &x, const MP<R> &y) const {

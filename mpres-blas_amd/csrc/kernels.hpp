@@ -404,6 +404,212 @@ __global__ __launch_bounds__(BLK) void k_gemm_fin(const DevConsts *__restrict__ 
     }
 }
 
+// ---------------------------------------------------------------- gemm, split form
+// Over 16 and 32 moduli the sums of gemm are formed in three parts. In a chain
+// whose every step is exact (no rounding, no straddling interval, no dropped
+// operand) the final residues depend only on the terms, the final exponent E
+// and the final sign:
+//   digits_r = |sum_i s_i X_i 2^(e_i - E)| mod m_r,
+// so the chain runs on the scalar fields alone (k_gemm_chain), and the residues
+// come from N independent modular matrix products of pre-scaled operands
+//   A'_r[i][row] = s_A x_r 2^(e_A - EA[row]) mod m_r,  B'_r[i][col] likewise,
+//   digits_r = 2^(EA[row] + EB[col] - E) * sum_i A'_r[i][row] B'_r[i][col] mod m_r
+// (k_gemm_resid), EA / EB being the minimum exponents of each row of op(A) and
+// column of op(B). Chains that meet a rare case are marked REDO and recomputed
+// step by step by the fixup kernel.
+
+template <int N>
+__device__ __forceinline__ int sign_at(const SoA &v, long long i) { return v.s[i]; }
+template <int N>
+__device__ __forceinline__ int sign_at(const AoSRef<N> &v, long long i) { return *v.sign(i); }
+template <int N>
+__device__ __forceinline__ int exp_of(const SoA &v, long long i) { return v.e[i]; }
+template <int N>
+__device__ __forceinline__ int exp_of(const AoSRef<N> &v, long long i) { return *v.exp(i); }
+
+// index of op(A)[row, i] (SIDE 0) or op(B)[i, col] (SIDE 1) in column-major storage
+template <int SIDE>
+__device__ __forceinline__ long long op_index(int t, int j, int i, int ld) {
+    if (SIDE == 0) return t ? (long long)i + (long long)ld * j : (long long)j + (long long)ld * i;
+    return t ? (long long)j + (long long)ld * i : (long long)i + (long long)ld * j;
+}
+
+// 2^d mod m_r for any integer d (m_r odd)
+__device__ __forceinline__ int pow2mod(const DevConsts *C, int d, int r) {
+    const double md = C->md[r], mv = C->minv[r];
+    const int n = C->n, top = C->log2m;
+    int f = 1;
+    if (d >= 0) {
+        while (d > top) {
+            f = mulmod(f, C->pow2[(size_t)top * n + r], md, mv);
+            d -= top;
+        }
+        return mulmod(f, C->pow2[(size_t)d * n + r], md, mv);
+    }
+    int D = -d;
+    while (D > P2T) {
+        f = mulmod(f, C->pow2inv[P2T - 1][r], md, mv);
+        D -= P2T;
+    }
+    return mulmod(f, C->pow2inv[D - 1][r], md, mv);
+}
+
+// minimum exponent of each of the cnt rows of op(A) (SIDE 0) or columns of op(B) (SIDE 1), over k
+template <int N, int L, class VM, int SIDE>
+__global__ __launch_bounds__(BLK) void k_gemm_emin(int cnt, int k, int t, VM X, int ld, int *out) {
+    if constexpr (std::is_same<VM, SoA>::value) X = rd(X);
+    const long long j = (long long)blockIdx.x * BLK + threadIdx.x;
+    if (j >= cnt) return;
+    int e = 0x7fffffff;
+    for (int i = 0; i < k; i++) {
+        const int v = exp_of<N>(X, op_index<SIDE>(t, (int)j, i, ld));
+        e = v < e ? v : e;
+    }
+    out[j] = k > 0 ? e : 0;
+}
+
+// pre-scaled residue planes P[r][i][j] (j over the cnt rows of op(A) or columns of op(B))
+template <int N, int L, class VM, int SIDE>
+__global__ __launch_bounds__(BLK) void k_gemm_prescale(const DevConsts *__restrict__ C, int cnt, int k, int t, VM X,
+                                                       int ld, const int *__restrict__ emin, int *__restrict__ P) {
+    if constexpr (std::is_same<VM, SoA>::value) X = rd(X);
+    const long long g = (long long)blockIdx.x * BLK + threadIdx.x;
+    if (g >= (long long)cnt * k) return;
+    const int j = (int)(g % cnt), i = (int)(g / cnt);
+    const long long idx = op_index<SIDE>(t, j, i, ld);
+    const int d = exp_of<N>(X, idx) - emin[j];
+    const int s = sign_at<N>(X, idx);
+    const int *dig = digits_of<N>(X, idx);
+    for (int r = 0; r < N; r++) {
+        const int m = C->m[r];
+        int v = mulmod(dig[r], pow2mod(C, d, r), C->md[r], C->minv[r]);
+        v = (s && v) ? m - v : v;
+        P[((size_t)r * k + i) * cnt + j] = v;
+    }
+}
+
+// scalar chain of every entry (one thread per entry): the sum's sign, exponent and
+// interval evaluation into the workspace, or REDO
+template <int N, int L, class VM>
+__global__ __launch_bounds__(BLK) void k_gemm_chain(const DevConsts *__restrict__ C, int ta, int tb, int m, int n,
+                                                    int k, VM A, int lda, VM B, int ldb, AoSRef<N> sums) {
+    if constexpr (std::is_same<VM, SoA>::value) {
+        A = rd(A);
+        B = rd(B);
+    }
+    const long long e = (long long)blockIdx.x * BLK + threadIdx.x;
+    if (e >= (long long)m * n) return;
+    const int row = (int)(e % m), col = (int)(e / m);
+    Sc acc{0, 0, Er{0.0, 0}, Er{0.0, 0}};
+    bool ok = true;
+    for (int i = 0; i < k && ok; i++) {
+        const long long ia = op_index<0>(ta, row, i, lda), ib = op_index<1>(tb, col, i, ldb);
+        int xs, xe, ys, ye;
+        Er x0, x1, y0, y1;
+        scalars_of<N>(A, ia, xs, xe, x0, x1);
+        scalars_of<N>(B, ib, ys, ye, y0, y1);
+        ok = sc_fma_step(C, acc, xs, xe, x0, x1, ys, ye, y0, y1);
+    }
+    if (!ok) {
+        *sums.sign(e) = REDO;
+        return;
+    }
+    *sums.sign(e) = acc.sign;
+    *sums.exp(e) = acc.exp;
+    ErMem *ev = sums.eval(e);
+    ev[0] = ErMem{acc.ev0.frac, (long long)acc.ev0.exp};
+    ev[1] = ErMem{acc.ev1.frac, (long long)acc.ev1.exp};
+}
+
+// exact x mod m for an integer-valued 0 <= x < 2^53
+__device__ __forceinline__ double fmod_exact(double x, double md, double mv) {
+    const double q = floor(x * mv);
+    return mod_fix(fma(-q, md, x), md);
+}
+
+// digit r of entry e from the modular dot product S (0 <= S < m)
+template <int N>
+__device__ __forceinline__ void resid_digit(const DevConsts *C, int r, int S, int ea, int eb, AoSRef<N> sums,
+                                            long long e) {
+    const int sign = *sums.sign(e);
+    if (sign == REDO) return;
+    const int t = mulmod(S, pow2mod(C, ea + eb - *sums.exp(e), r), C->md[r], C->minv[r]);
+    sums.digits(e)[r] = (sign && t) ? C->m[r] - t : t;
+}
+
+// Residue products: blockIdx.z = modulus r, 64 x 64 entries per workgroup, each
+// thread 4 x 4. Operands stay exact in FP64: with B-bit moduli, a < 2^B times
+// the halves of b = bh 2^sb + bl (bl < 2^sb, bh < 2^(B-sb)) stay below
+// 2^(53 - log2 kred) per term, so kred terms accumulate exactly; the sums are
+// reduced mod m every kred terms (kred a multiple of RKC, set by the host).
+constexpr int RT = 64, RKC = 16;
+template <int N, int L>
+__global__ __launch_bounds__(BLK) void k_gemm_resid(const DevConsts *__restrict__ C, int m, int n, int k, int sb,
+                                                    int kred, const int *__restrict__ PA, const int *__restrict__ PB,
+                                                    const int *__restrict__ EA, const int *__restrict__ EB,
+                                                    AoSRef<N> sums) {
+    __shared__ double sa[RKC][RT];
+    __shared__ double sbh[RKC][RT], sbl[RKC][RT];
+    const int r = blockIdx.z;
+    const int row0 = blockIdx.y * RT, col0 = blockIdx.x * RT;
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    const double md = C->md[r], mv = C->minv[r];
+    const int *A = PA + (size_t)r * k * m, *B = PB + (size_t)r * k * n;
+    double ah[4][4], al[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) ah[a][b] = al[a][b] = 0.0;
+    for (int k0 = 0; k0 < k; k0 += RKC) {
+        for (int q = threadIdx.x; q < RKC * RT; q += BLK) {
+            const int kk = q / RT, c = q % RT, gi = k0 + kk;
+            const int gr = row0 + c, gc = col0 + c;
+            sa[kk][c] = (gi < k && gr < m) ? (double)A[(size_t)gi * m + gr] : 0.0;
+            const int bv = (gi < k && gc < n) ? B[(size_t)gi * n + gc] : 0;
+            sbh[kk][c] = (double)(bv >> sb);
+            sbl[kk][c] = (double)(bv & ((1 << sb) - 1));
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < RKC; kk++) {
+            double a[4], bh[4], bl[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                a[q] = sa[kk][ty + 16 * q];
+                bh[q] = sbh[kk][tx + 16 * q];
+                bl[q] = sbl[kk][tx + 16 * q];
+            }
+#pragma unroll
+            for (int x = 0; x < 4; x++)
+#pragma unroll
+                for (int y = 0; y < 4; y++) {
+                    ah[x][y] = fma(a[x], bh[y], ah[x][y]);
+                    al[x][y] = fma(a[x], bl[y], al[x][y]);
+                }
+        }
+        __syncthreads();
+        if ((k0 + RKC) % kred == 0) {
+#pragma unroll
+            for (int x = 0; x < 4; x++)
+#pragma unroll
+                for (int y = 0; y < 4; y++) {
+                    ah[x][y] = fmod_exact(ah[x][y], md, mv);
+                    al[x][y] = fmod_exact(al[x][y], md, mv);
+                }
+        }
+    }
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+            const int row = row0 + ty + 16 * x, col = col0 + tx + 16 * y;
+            if (row >= m || col >= n) continue;
+            const double h = fmod_exact(ah[x][y], md, mv), l = fmod_exact(al[x][y], md, mv);
+            const int S = (int)fmod_exact(h * (double)(1 << sb) + l, md, mv);
+            resid_digit<N>(C, r, S, EA[row], EB[col], sums, (long long)row + (long long)m * col);
+        }
+}
+
 // ---------------------------------------------------------------- SpMV
 // Row partial sums of tpr strided lanes (src/sparse/spmv/spmv_csrv.cuh:61-65; tpr = 1
 // is the scalar kernel spmv_csr.cuh:53-62), combined by the 16/8/4/2/1 tree

@@ -139,41 +139,73 @@ struct Launch {
                            AoSRef<N>{(char *)sums});
         return hipGetLastError();
     }
-    // pass 0: the m*n sums into ws (m*n mp_float_t); pass 1: the marked sums
-    // recomputed (16 and 32 moduli); pass 2: the alpha/beta epilogue
-    template <class VM>
-    static void gemm_sums(int pass, const dim3 &gr, const DevConsts *c, int ta, int tb, int m, int n, int k, VM A,
-                          int lda, VM B, int ldb, char *ws, hipStream_t st) {
-        const dim3 bl(BLK);
-        if (pass == 0)
-            hipLaunchKernelGGL((k_gemm_sum<N, L, !SPLIT, false, VM>), gr, bl, 0, st, c, ta, tb, m, n, k, A, lda, B,
-                               ldb, AoSRef<N>{ws});
-        else if constexpr (SPLIT)
-            hipLaunchKernelGGL((k_gemm_sum<N, L, true, true, VM>), gr, bl, 0, st, c, ta, tb, m, n, k, A, lda, B, ldb,
-                               AoSRef<N>{ws});
+    // gemm passes. Over 16 and 32 moduli (split form, see kernels.hpp):
+    //   0 exponent minima and pre-scaled residue planes, 1 scalar chains,
+    //   2 residue products and digits, 3 fixup of the marked chains, 4 epilogue.
+    // Over 8 moduli: 1 full sums, 4 epilogue. bits: bit length of the largest modulus.
+    // ws: m*n sums, then (split form) the planes of op(A) and op(B) and the minima.
+    static size_t gemm_ws(int m, int n, int k) {
+        const size_t sums = ((size_t)m * n * (4 * N + 40) + 255) / 256 * 256;
+        if (!SPLIT) return sums;
+        return sums + 4 * ((size_t)N * k * m + (size_t)N * k * n + m + n);
     }
-    static hipError_t gemm(const DevConsts *c, int pass, int m, int n, int k, SoA alpha, SoA A, int lda, SoA B, int ldb,
-                           SoA beta, SoA Cm, int ldc, char *ws, hipStream_t st) {
+    template <class VM>
+    static void gemm_pass(int pass, const DevConsts *c, int bits, int ta, int tb, int m, int n, int k, VM A, int lda,
+                          VM B, int ldb, char *ws, hipStream_t st) {
+        const long long mn = (long long)m * n;
+        const dim3 gr(grid_for(mn, L)), bl(BLK);
+        const AoSRef<N> sums{ws};
+        const size_t sb_ = ((size_t)mn * (4 * N + 40) + 255) / 256 * 256;
+        int *PA = (int *)(ws + sb_), *PB = PA + (size_t)N * k * m, *EA = PB + (size_t)N * k * n, *EB = EA + m;
+        if constexpr (SPLIT) {
+            if (pass == 0) {
+                hipLaunchKernelGGL((k_gemm_emin<N, L, VM, 0>), dim3(grid_for(m, 1)), bl, 0, st, m, k, ta, A, lda, EA);
+                hipLaunchKernelGGL((k_gemm_emin<N, L, VM, 1>), dim3(grid_for(n, 1)), bl, 0, st, n, k, tb, B, ldb, EB);
+                if (k > 0) {
+                    hipLaunchKernelGGL((k_gemm_prescale<N, L, VM, 0>), dim3(grid_for((long long)m * k, 1)), bl, 0, st,
+                                       c, m, k, ta, A, lda, EA, PA);
+                    hipLaunchKernelGGL((k_gemm_prescale<N, L, VM, 1>), dim3(grid_for((long long)n * k, 1)), bl, 0, st,
+                                       c, n, k, tb, B, ldb, EB, PB);
+                }
+            } else if (pass == 1) {
+                hipLaunchKernelGGL((k_gemm_chain<N, L, VM>), dim3(grid_for(mn, 1)), bl, 0, st, c, ta, tb, m, n, k, A,
+                                   lda, B, ldb, sums);
+            } else if (pass == 2) {
+                const int sbits = bits / 2;
+                const int hi = (bits + sbits > 2 * bits - sbits) ? bits + sbits : 2 * bits - sbits;
+                const int kred = 1 << (52 - hi);
+                const dim3 rg((n + RT - 1) / RT, (m + RT - 1) / RT, N);
+                hipLaunchKernelGGL((k_gemm_resid<N, L>), rg, bl, 0, st, c, m, n, k, sbits, kred, PA, PB, EA, EB, sums);
+            } else if (pass == 3) {
+                hipLaunchKernelGGL((k_gemm_sum<N, L, true, true, VM>), gr, bl, 0, st, c, ta, tb, m, n, k, A, lda, B,
+                                   ldb, sums);
+            }
+        } else if (pass == 1) {
+            hipLaunchKernelGGL((k_gemm_sum<N, L, true, false, VM>), gr, bl, 0, st, c, ta, tb, m, n, k, A, lda, B, ldb,
+                               sums);
+        }
+    }
+    static hipError_t gemm(const DevConsts *c, int pass, int bits, int m, int n, int k, SoA alpha, SoA A, int lda,
+                           SoA B, int ldb, SoA beta, SoA Cm, int ldc, char *ws, hipStream_t st) {
         if ((long long)m * n <= 0) return hipSuccess;
-        const dim3 gr(grid_for((long long)m * n, L)), bl(BLK);
-        if (pass < 2)
-            gemm_sums<SoA>(pass, gr, c, 0, 0, m, n, k, A, lda, B, ldb, ws, st);
+        if (pass < 4)
+            gemm_pass<SoA>(pass, c, bits, 0, 0, m, n, k, A, lda, B, ldb, ws, st);
         else
-            hipLaunchKernelGGL((k_gemm_fin<N, L, false, SoA>), gr, bl, 0, st, c, m, n, alpha, beta, Cm, ldc,
-                               AoSRef<N>{ws});
+            hipLaunchKernelGGL((k_gemm_fin<N, L, false, SoA>), dim3(grid_for((long long)m * n, L)), dim3(BLK), 0, st,
+                               c, m, n, alpha, beta, Cm, ldc, AoSRef<N>{ws});
         return hipGetLastError();
     }
-    static hipError_t gemm_v2(const DevConsts *c, int pass, int ta, int tb, int m, int n, int k, const char *alpha,
-                              const char *A, int lda, const char *B, int ldb, const char *beta, char *Cm, int ldc,
-                              char *ws, hipStream_t st) {
+    static hipError_t gemm_v2(const DevConsts *c, int pass, int bits, int ta, int tb, int m, int n, int k,
+                              const char *alpha, const char *A, int lda, const char *B, int ldb, const char *beta,
+                              char *Cm, int ldc, char *ws, hipStream_t st) {
         if ((long long)m * n <= 0) return hipSuccess;
-        const dim3 gr(grid_for((long long)m * n, L)), bl(BLK);
-        if (pass < 2)
-            gemm_sums<AoSRef<N>>(pass, gr, c, ta, tb, m, n, k, AoSRef<N>{(char *)A}, lda, AoSRef<N>{(char *)B}, ldb,
+        if (pass < 4)
+            gemm_pass<AoSRef<N>>(pass, c, bits, ta, tb, m, n, k, AoSRef<N>{(char *)A}, lda, AoSRef<N>{(char *)B}, ldb,
                                  ws, st);
         else
-            hipLaunchKernelGGL((k_gemm_fin<N, L, true, AoSRef<N>>), gr, bl, 0, st, c, m, n, AoSRef<N>{(char *)alpha},
-                               AoSRef<N>{(char *)beta}, AoSRef<N>{Cm}, ldc, AoSRef<N>{ws});
+            hipLaunchKernelGGL((k_gemm_fin<N, L, true, AoSRef<N>>), dim3(grid_for((long long)m * n, L)), dim3(BLK),
+                               0, st, c, m, n, AoSRef<N>{(char *)alpha}, AoSRef<N>{(char *)beta}, AoSRef<N>{Cm}, ldc,
+                               AoSRef<N>{ws});
         return hipGetLastError();
     }
     static hipError_t spmv(const DevConsts *c, int pass, int m, const int *irp, const int *ja, const double *as,
@@ -205,6 +237,7 @@ struct Launch {
         o.gemv_fold = &gemv_fold;
         o.gemv_final = &gemv_final;
         o.gemm = &gemm;
+        o.gemm_ws = &gemm_ws;
         o.gemm_v2 = &gemm_v2;
         o.spmv = &spmv;
         return o;

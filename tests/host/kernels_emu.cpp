@@ -56,6 +56,37 @@ int gemm_v2_n(const Ctx &x, int ta, int tb, int m, int n, int k, char *alpha, ch
     return 0;
 }
 
+// split form: minima, pre-scaled planes and chains through the kernels; the
+// residue products by exact 64-bit arithmetic, digits through resid_digit; then
+// the fixup and epilogue kernels
+template <int N>
+int gemm_v2_split_n(const Ctx &x, int ta, int tb, int m, int n, int k, char *alpha, char *A, int lda, char *B, int ldb,
+                    char *beta, char *Cm, int ldc, char *ws) {
+    const long long mn = (long long)m * n;
+    const AoSRef<N> a{A}, b{B}, w{ws};
+    std::vector<int> EA(m), EB(n), PA((size_t)N * k * m), PB((size_t)N * k * n);
+    run(m, [&] { k_gemm_emin<N, 1, AoSRef<N>, 0>(m, k, ta, a, lda, EA.data()); });
+    run(n, [&] { k_gemm_emin<N, 1, AoSRef<N>, 1>(n, k, tb, b, ldb, EB.data()); });
+    run((long long)m * k, [&] { k_gemm_prescale<N, 1, AoSRef<N>, 0>(&x.c, m, k, ta, a, lda, EA.data(), PA.data()); });
+    run((long long)n * k, [&] { k_gemm_prescale<N, 1, AoSRef<N>, 1>(&x.c, n, k, tb, b, ldb, EB.data(), PB.data()); });
+    run(mn, [&] { k_gemm_chain<N, 1, AoSRef<N>>(&x.c, ta, tb, m, n, k, a, lda, b, ldb, w); });
+    for (int r = 0; r < N; r++) {
+        const long long mod = x.c.m[r];
+        for (int col = 0; col < n; col++)
+            for (int row = 0; row < m; row++) {
+                long long S = 0;
+                for (int i = 0; i < k; i++)
+                    S = (S + (long long)PA[((size_t)r * k + i) * m + row] * PB[((size_t)r * k + i) * n + col]) % mod;
+                resid_digit<N>(&x.c, r, (int)S, EA[row], EB[col], w, (long long)row + (long long)m * col);
+            }
+    }
+    run(mn, [&] { k_gemm_sum<N, 1, true, true, AoSRef<N>>(&x.c, ta, tb, m, n, k, a, lda, b, ldb, w); });
+    run(mn, [&] {
+        k_gemm_fin<N, 1, true, AoSRef<N>>(&x.c, m, n, AoSRef<N>{alpha}, AoSRef<N>{beta}, AoSRef<N>{Cm}, ldc, w);
+    });
+    return 0;
+}
+
 template <int N>
 int dot_v2_pass1_n(const Ctx &x, long long cnt, long long nvt, char *X, char *Y, char *out, int split) {
     SoA none{};
@@ -82,6 +113,18 @@ int emu_gemm_v2(int set_id, int ta, int tb, int m, int n, int k, void *alpha, vo
         case 8: return gemm_v2_n<8>(x, ta, tb, m, n, k, (char *)alpha, (char *)A, lda, (char *)B, ldb, (char *)beta, (char *)Cm, ldc, (char *)ws, split);
         case 16: return gemm_v2_n<16>(x, ta, tb, m, n, k, (char *)alpha, (char *)A, lda, (char *)B, ldb, (char *)beta, (char *)Cm, ldc, (char *)ws, split);
         case 32: return gemm_v2_n<32>(x, ta, tb, m, n, k, (char *)alpha, (char *)A, lda, (char *)B, ldb, (char *)beta, (char *)Cm, ldc, (char *)ws, split);
+    }
+    return -1;
+}
+
+// v2 gemm in the split form (16 and 32 moduli); ws holds m*n mp_float_t
+int emu_gemm_v2_split(int set_id, int ta, int tb, int m, int n, int k, void *alpha, void *A, int lda, void *B, int ldb,
+                      void *beta, void *Cm, int ldc, void *ws) {
+    Ctx x;
+    if (!make_ctx(set_id, x)) return -1;
+    switch (x.c.n) {
+        case 16: return gemm_v2_split_n<16>(x, ta, tb, m, n, k, (char *)alpha, (char *)A, lda, (char *)B, ldb, (char *)beta, (char *)Cm, ldc, (char *)ws);
+        case 32: return gemm_v2_split_n<32>(x, ta, tb, m, n, k, (char *)alpha, (char *)A, lda, (char *)B, ldb, (char *)beta, (char *)Cm, ldc, (char *)ws);
     }
     return -1;
 }

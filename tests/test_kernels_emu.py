@@ -35,12 +35,16 @@ def P(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
-def _gemm_inputs(o, lo, hi, m, n, k):
+def _gemm_inputs(o, lo, hi, m, n, k, cancel=True):
     As = gen.uexp(91, m * k, lo, hi).reshape(k, m)
     Bs = gen.uexp(92, k * n, lo, hi).reshape(n, k)
-    for i in range(0, min(k - 1, 8), 2):  # exact cancellations between consecutive terms
-        As[i + 1] = As[i]
-        Bs[:, i + 1] = -Bs[:, i]
+    if cancel:  # exact cancellations between consecutive terms (every entry)
+        for i in range(0, min(k - 1, 8), 2):
+            As[i + 1] = As[i]
+            Bs[:, i + 1] = -Bs[:, i]
+    else:  # one column of B with a cancelling pair: only those entries take the fixup path
+        As[1] = As[0]
+        Bs[2, 1] = -Bs[2, 0]
     A, B = o.set_d(As.ravel(), D), o.set_d(Bs.ravel(), D)
     Cm = o.set_d(gen.uexp(93, m * n, lo, hi), D)
     return A, B, Cm
@@ -85,3 +89,25 @@ def test_dot_pass1_fast_fixup(emu, name):
             for i in range(v, cnt, nvt):
                 acc = o.add(acc, o.mul(x[i:i + 1], y[i:i + 1], D), D)
             assert part[v].tobytes() == acc[0].tobytes(), (name, split, v)
+
+
+@pytest.mark.parametrize("name,lo,hi", [("16_4double", -30, 30), ("16_4double", -150, 150), ("32_8double", -30, 30),
+                                        ("32_8double", -400, 400), ("16_32bit", -40, 40), ("32_32bit", -40, 40)])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("cancel", [False, True])
+def test_gemm_v2_split_form(emu, name, lo, hi, ta, tb, cancel):
+    """Split form: scalar chains + pre-scaled residue products + digit reconstruction,
+    with zeros, cancellations (marked and recomputed) and every transpose."""
+    o = po.Oracle(name)
+    sid = po.set_names().index(name)
+    m, n, k = 7, 6, 13
+    A, B, Cm = _gemm_inputs(o, lo, hi, m, n, k, cancel)
+    z = o.set_d([0.0, -0.0], D)
+    A[3], B[5] = z[0], z[1]
+    alpha, beta = o.set_d([0.5], D), o.set_d([-3.0], D)
+    lda = k if ta else m
+    ldb = n if tb else k
+    want = o.gemm_v2(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, Cm, m, D)
+    got, ws = Cm.copy(), o.empty(m * n)
+    assert emu.emu_gemm_v2_split(sid, ta, tb, m, n, k, P(alpha), P(A), lda, P(B), ldb, P(beta), P(got), m, P(ws)) == 0
+    assert got.tobytes() == want.tobytes()
