@@ -511,7 +511,8 @@ size_t mpres_gemv_workspace(const mpres_ctx_t *c, int trans, int m, int n, int b
     if (!c || m <= 0 || n <= 0 || b3 <= 0) return 0;
     const bool tr = trans != MPRES_NO_TRANS;
     const size_t lx = tr ? m : n, ly = tr ? n : m;
-    return mpsize(c) * (lx + ly * (size_t)b3 + ly);
+    // alpha*x, the ly*b3 partials, the ly row sums, then the main kernel's auxiliary data
+    return mpsize(c) * (lx + ly * (size_t)b3 + ly) + c->ops[OP_GEMV]->gemv_aux(tr, m, n);
 }
 
 int mpres_gemv(mpres_ctx_t *c, int trans, int m, int n, const mp_array_t *alpha, const mp_array_t *A, int lda,
@@ -530,18 +531,17 @@ int mpres_gemv(mpres_ctx_t *c, int trans, int m, int n, const mp_array_t *alpha,
     char *w;
     int rc = get_ws(c, ws, wsb, mpres_gemv_workspace(c, trans, m, n, b3), &w);
     if (rc) return rc;
-    char *bx = w, *part = bx + S * lx, *sums = part + S * (size_t)ly * b3;
+    char *bx = w, *part = bx + S * lx, *sums = part + S * (size_t)ly * b3, *aux = sums + S * (size_t)ly;
     const Ops *o = c->ops[OP_GEMV];
     SoA ys = soa_of(y);
     // buffer1 = alpha * x, y = beta * y (src/blas/gemv.cuh:175-190)
     HIPCHK(o->scal(c->dev, lx, soa_of(alpha), soa_of(x), incx, bx, false, strm(st)));
     HIPCHK(o->scal(c->dev, ly, soa_of(beta), ys, incy, nullptr, true, strm(st)));
     // per-output strided sums of A .* buffer1 and the halving tree (:199-218, :44-76)
-    {
-        ProfScope ps(c, OP_GEMV, strm(st));
-        HIPCHK(o->gemv_fold(c->dev, tr, 0, m, n, soa_of(A), lda, bx, b3, (long long)ly * b3, part, strm(st)));
+    for (int pass = 0; pass < 3; pass++) {
+        ProfScope ps(c, pass == 1 ? OP_GEMV : -1, strm(st));
+        HIPCHK(o->gemv_fold(c->dev, tr, pass, m, n, soa_of(A), lda, bx, b3, (long long)ly * b3, part, aux, strm(st)));
     }
-    HIPCHK(o->gemv_fold(c->dev, tr, 1, m, n, soa_of(A), lda, bx, b3, (long long)ly * b3, part, strm(st)));
     HIPCHK(o->tree(c->dev, part, b3, next_pow2((unsigned)b3), ly, sums, strm(st)));
     HIPCHK(o->gemv_final(c->dev, ly, ys, incy, sums, strm(st)));
     return MPRES_OK;

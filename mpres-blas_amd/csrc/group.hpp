@@ -382,6 +382,19 @@ struct Grp {
         mul_scalars(p, xs, xe, x0, x1, ys, ye, y0, y1);
         return step_tail<FULL>(acc, p, dx, dy, 0.0, false);
     }
+    // as fma_step, with the lane's residues of x read from dxp (e.g. staged in LDS)
+    template <bool FULL, class VX, class VY>
+    __device__ __forceinline__ bool fma_step_d(MP<R> &acc, const VX &vx, long long ixx, const int *dxp, const VY &vy,
+                                               long long iyy) const {
+        int xs, xe, ys, ye;
+        Er x0, x1, y0, y1;
+        scalars_of<N>(vx, ixx, xs, xe, x0, x1);
+        scalars_of<N>(vy, iyy, ys, ye, y0, y1);
+        const int *dy = digits_of<N>(vy, iyy) + k.lane * R;
+        MP<R> p;
+        mul_scalars(p, xs, xe, x0, x1, ys, ye, y0, y1);
+        return step_tail<FULL>(acc, p, dxp, dy, 0.0, false);
+    }
     // acc = acc + x[ix] * y (mp_mul_d then mp_add), x streamed from memory
     template <bool FULL, class VX>
     __device__ __forceinline__ bool fma_d_step(MP<R> &acc, const VX &vx, long long ixx, double y) const {

@@ -291,16 +291,21 @@ __global__ __launch_bounds__(BLK) void k_scal(const DevConsts *__restrict__ C, i
 }
 // virtual thread (o, t): o-th output, t-th of b3 strided partial sums of A-entries times bx
 // (matrix_row_sum_kernel / matrix_col_sum_kernel, src/blas/gemv.cuh:44-120, with the
-// products of the mat2vec scaling kernels formed on the fly)
+// products of the mat2vec scaling kernels formed on the fly); its partial goes to
+// part[o * b3 + t]. Groups are laid out so that a wavefront's loads are contiguous:
+// without transpose consecutive groups take consecutive rows o (same t), so each
+// step reads a run of one column of A; with transpose consecutive t of one output.
 template <int N, int L, bool TRANS, bool FULL, bool FIX>
 __global__ __launch_bounds__(BLK) void k_gemv_fold(const DevConsts *__restrict__ C, int m, int n, SoA A, int lda,
                                                    AoSRef<N> bx, int b3, long long nvt, AoSRef<N> part) {
     A = rd(A);
-    const long long v = group_id<L>();
-    if (v >= nvt) return;
+    const long long gid = group_id<L>();
+    if (gid >= nvt) return;
+    const long long ly = nvt / b3;
+    const long long o = TRANS ? gid / b3 : gid % ly;
+    const int t = (int)(TRANS ? gid % b3 : gid / ly);
+    const long long v = o * b3 + t;
     if (FIX && *part.sign(v) != REDO) return;
-    const long long o = v / b3;
-    const int t = (int)(v % b3);
     Grp<N, L> g(C);
     MP<N / L> acc;
     g.zero(acc);
@@ -313,6 +318,78 @@ __global__ __launch_bounds__(BLK) void k_gemv_fold(const DevConsts *__restrict__
     else if (g.lane() == 0)
         *part.sign(v) = REDO;
 }
+// Residues of a run of `rows` consecutive elements of an SoA array (rows * N ints,
+// 16-byte aligned) moved to LDS with element stride N + 1. When a full run
+// (rows == W) is a whole number of 16-byte loads per thread (VEC), it goes
+// through registers in two halves, so that the next step's loads can be issued
+// before the current step computes: fetch issues the loads, store writes them
+// to LDS. Other runs are copied directly by store.
+template <int N, int L>
+struct RunStage {
+    static constexpr int W = BLK / L, SP = N + 1;
+    static constexpr bool VEC = (W * N) % (4 * BLK) == 0;
+    static constexpr int V4 = VEC ? W * N / 4 / BLK : 1;  // int4 per thread for a full run
+    int4 v[V4];
+    __device__ __forceinline__ void fetch(const int *src, int rows) {
+        if (VEC && rows == W) {
+            const int4 *s4 = (const int4 *)src;
+#pragma unroll
+            for (int q = 0; q < V4; q++) v[q] = s4[q * BLK + threadIdx.x];
+        }
+    }
+    __device__ __forceinline__ void store(int *sd, const int *src, int rows) const {
+        if (VEC && rows == W) {
+#pragma unroll
+            for (int q = 0; q < V4; q++) {
+                const int i = 4 * (q * BLK + threadIdx.x), e = i / N, f = i % N;
+                int *d = sd + e * SP + f;
+                d[0] = v[q].x;
+                d[1] = v[q].y;
+                d[2] = v[q].z;
+                d[3] = v[q].w;
+            }
+        } else {
+            for (int q = threadIdx.x; q < rows * N; q += BLK) sd[(q / N) * SP + q % N] = src[q];
+        }
+    }
+};
+
+// Main pass without transpose, the residues of A staged through LDS: workgroup
+// (x, t) holds the W = BLK / L consecutive rows o of block x, all with strided
+// index t, so each step j reads one contiguous run of column j of A. The run's
+// residues are loaded cooperatively (coalesced) into LDS with a padded stride and
+// every group reads its own from there. Same partials as k_gemv_fold.
+template <int N, int L, bool FULL>
+__global__ __launch_bounds__(BLK) void k_gemv_rows(const DevConsts *__restrict__ C, int m, int n, SoA A, int lda,
+                                                   AoSRef<N> bx, int b3, AoSRef<N> part) {
+    constexpr int W = BLK / L, R = N / L, SP = N + 1;
+    __shared__ int sd[W * SP];
+    A = rd(A);
+    const int gl = threadIdx.x / L;
+    const int t = blockIdx.y;
+    const long long o0 = (long long)blockIdx.x * W, o = o0 + gl;
+    const int rows = (int)((m - o0) < W ? (m - o0) : W);
+    bool ok = o < m;
+    Grp<N, L> g(C);
+    MP<R> acc;
+    g.zero(acc);
+    RunStage<N, L> stg;
+    if (t < n) stg.fetch(A.d + (o0 + (long long)t * lda) * N, rows);
+    for (int j = t; j < n; j += b3) {
+        stg.store(sd, A.d + (o0 + (long long)j * lda) * N, rows);
+        __syncthreads();
+        if (j + b3 < n) stg.fetch(A.d + (o0 + (long long)(j + b3) * lda) * N, rows);
+        if (ok) ok = g.template fma_step_d<FULL>(acc, A, o + (long long)j * lda, sd + gl * SP + g.lane() * R, bx, j);
+        __syncthreads();
+    }
+    if (o >= m) return;
+    const long long v = o * b3 + t;
+    if (ok)
+        store_aos(g, acc, part, v);
+    else if (g.lane() == 0)
+        *part.sign(v) = REDO;
+}
+
 // y[o] = y[o] (+) rowsum[o]  (src/blas/gemv.cuh:72-75 with mp_add rounding, add.cuh:223-248)
 template <int N, int L>
 __global__ __launch_bounds__(BLK) void k_gemv_final(const DevConsts *__restrict__ C, int len, SoA y, int incy,
@@ -438,6 +515,7 @@ __device__ __forceinline__ long long op_index(int t, int j, int i, int ld) {
 __device__ __forceinline__ int pow2mod(const DevConsts *C, int d, int r) {
     const double md = C->md[r], mv = C->minv[r];
     const int n = C->n, top = C->log2m;
+    if (d >= 0 && d <= top) return C->pow2[(size_t)d * n + r];
     int f = 1;
     if (d >= 0) {
         while (d > top) {
@@ -454,18 +532,28 @@ __device__ __forceinline__ int pow2mod(const DevConsts *C, int d, int r) {
     return mulmod(f, C->pow2inv[D - 1][r], md, mv);
 }
 
-// minimum exponent of each of the cnt rows of op(A) (SIDE 0) or columns of op(B) (SIDE 1), over k
+// minimum exponent of each of the cnt rows of op(A) (SIDE 0) or columns of op(B) (SIDE 1),
+// over k: workgroup (x, y) covers EMIN_J consecutive j and the k-chunk y of EMIN_K terms,
+// combined by atomic minimum into out (filled with INT_MAX by k_fill_int first)
+constexpr int EMIN_J = 64, EMIN_K = 64;
+template <int N, int L>
+__global__ __launch_bounds__(BLK) void k_fill_int(int *p, int cnt, int v) {
+    const long long i = (long long)blockIdx.x * BLK + threadIdx.x;
+    if (i < cnt) p[i] = v;
+}
 template <int N, int L, class VM, int SIDE>
 __global__ __launch_bounds__(BLK) void k_gemm_emin(int cnt, int k, int t, VM X, int ld, int *out) {
     if constexpr (std::is_same<VM, SoA>::value) X = rd(X);
-    const long long j = (long long)blockIdx.x * BLK + threadIdx.x;
+    constexpr int S = BLK / EMIN_J;
+    const long long j = (long long)blockIdx.x * EMIN_J + threadIdx.x % EMIN_J;
+    const int i0 = blockIdx.y * EMIN_K, i1 = (i0 + EMIN_K) < k ? i0 + EMIN_K : k;
     if (j >= cnt) return;
     int e = 0x7fffffff;
-    for (int i = 0; i < k; i++) {
+    for (int i = i0 + (int)threadIdx.x / EMIN_J; i < i1; i += S) {
         const int v = exp_of<N>(X, op_index<SIDE>(t, (int)j, i, ld));
         e = v < e ? v : e;
     }
-    out[j] = k > 0 ? e : 0;
+    if (e != 0x7fffffff) atomicMin(out + j, e);
 }
 
 // pre-scaled residue planes P[r][i][j] (j over the cnt rows of op(A) or columns of op(B))
@@ -608,6 +696,136 @@ __global__ __launch_bounds__(BLK) void k_gemm_resid(const DevConsts *__restrict_
             const int S = (int)fmod_exact(h * (double)(1 << sb) + l, md, mv);
             resid_digit<N>(C, r, S, EA[row], EB[col], sums, (long long)row + (long long)m * col);
         }
+}
+
+// ---------------------------------------------------------------- gemv, split form
+// Main pass without transpose over 16 and 32 moduli. As for gemm, an exact chain's
+// residues are |sum_j s_j X_j 2^(e_j - E)| mod m, so each partial (o, t) runs the
+// interval chain on scalars (sc_fma_step) and accumulates its residues against the
+// fixed exponent E_ref = EA[o] + EB (EA[o]: minimum exponent of row o of A, EB:
+// minimum exponent of bx), then rescales once by 2^(E_ref - E). bx is pre-scaled
+// (bxp[r][j] = s x_r 2^(e_j - EB) mod m_r); per step the workgroup builds in LDS
+// the 64 multiples bxp[r][j] 2^d (d < 64), so a term costs one product mod m per
+// residue; d >= 64 takes the slow path. A's residues are staged through LDS as
+// in k_gemv_rows. Chains with a rare case are marked REDO for the fixup kernel.
+constexpr int GV_D = 64;
+constexpr int GV_REDUCE = 1 << 20;  // terms between reductions of the double accumulators
+template <int N, int L>
+__global__ __launch_bounds__(BLK) void k_gemv_split(const DevConsts *__restrict__ C, int m, int n, SoA A, int lda,
+                                                    AoSRef<N> bx, const int *__restrict__ bxp,
+                                                    const int *__restrict__ EA, const int *__restrict__ EB, int b3,
+                                                    AoSRef<N> part) {
+    constexpr int SP = N + 1;
+    __shared__ int sd[BLK * SP];
+    __shared__ double tp[GV_D * SP];
+    A = rd(A);
+    const int t = blockIdx.y;
+    const long long o0 = (long long)blockIdx.x * BLK, o = o0 + threadIdx.x;
+    const int rows = (int)((m - o0) < BLK ? (m - o0) : BLK);
+    const bool live = o < m;
+    bool ok = live;
+    const int ea = live ? EA[o] : 0;
+    // residue sums s_j x_j (bxp 2^d)_j accumulate unreduced: each term
+    // w = x f - q m (exact, |w| < 2m + 1 < 2^28) is added in FP64, which stays
+    // exact for GV_REDUCE terms between reductions
+    double acc[N];
+#pragma unroll
+    for (int r = 0; r < N; r++) acc[r] = 0.0;
+    int since = 0;
+    Sc sc{0, 0, Er{0.0, 0}, Er{0.0, 0}};
+    // table build: thread q -> residue q % N, multiples d0 .. d0 + DB - 1
+    constexpr int DB = GV_D * N / BLK;
+    const int tr = threadIdx.x % N, td0 = (threadIdx.x / N) * DB;
+    const double tmd = C->md[tr], tmv = C->minv[tr];
+    const int tm = C->m[tr];
+    const int t2 = C->pow2[(size_t)td0 * N + tr];
+    // step j's table seed and A's scalar fields are fetched during step j - b3;
+    // A's run of residues is staged at the start of step j (all loads in flight)
+    RunStage<N, 1> stg;
+    int nb = 0, nxs = 0, nxe = 0;
+    ErMem nlo{0.0, 0}, nup{0.0, 0};
+    auto fetch = [&](int j) {
+        nb = bxp[(size_t)tr * n + j];
+        if (live) {
+            const long long ia = o + (long long)j * lda;
+            nxs = A.s[ia];
+            nxe = A.e[ia];
+            nlo = A.ev[ia];
+            nup = A.ev[ia + A.len];
+        }
+    };
+    if (t < n) fetch(t);
+    for (int j = t; j < n; j += b3) {
+        const int *run = A.d + (o0 + (long long)j * lda) * N;
+        stg.fetch(run, rows);
+        stg.store(sd, run, rows);
+        int v = mulmod(nb, t2, tmd, tmv);
+        for (int d = 0; d < DB; d++) {
+            tp[(td0 + d) * SP + tr] = (double)v;
+            v += v;
+            v = v >= tm ? v - tm : v;
+        }
+        const int xs = nxs, xe = nxe;
+        const Er x0{nlo.frac, (int)nlo.exp}, x1{nup.frac, (int)nup.exp};
+        __syncthreads();
+        if (j + b3 < n) fetch(j + b3);
+        if (ok) {
+            int ys, ye;
+            Er y0, y1;
+            scalars_of<N>(bx, j, ys, ye, y0, y1);
+            ok = sc_fma_step(C, sc, xs, xe, x0, x1, ys, ye, y0, y1);
+            if (ok) {
+                int d = xe - ea;
+                int *xd = sd + threadIdx.x * SP;
+                if (d >= GV_D) {
+                    // rare: fold 2^(d - d mod 64) into this thread's own staged residues
+                    const int dh = d - (d & (GV_D - 1));
+                    for (int r = 0; r < N; r++) xd[r] = mulmod(xd[r], pow2mod(C, dh, r), C->md[r], C->minv[r]);
+                    d &= GV_D - 1;
+                }
+                const double sg = xs ? -1.0 : 1.0;
+                const double *f = tp + d * SP;
+                // groups of 8 residues, fenced so that the compiler does not hoist
+                // all LDS loads (and their registers) ahead of the arithmetic
+#pragma unroll
+                for (int r0 = 0; r0 < N; r0 += 8) {
+#pragma unroll
+                    for (int r = r0; r < r0 + 8; r++) {
+                        const double x = (double)xd[r], fr = f[r];
+                        const double p = x * fr, pl = fma(x, fr, -p);
+                        const double q = floor(p * C->minv[r]);
+                        acc[r] = fma(sg, fma(-q, C->md[r], p) + pl, acc[r]);
+                    }
+                    asm volatile("" ::: "memory");
+                }
+                if (++since == GV_REDUCE) {
+                    since = 0;
+#pragma unroll
+                    for (int r = 0; r < N; r++) acc[r] = fmod_exact(acc[r], C->md[r], C->minv[r]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!live) return;
+    const long long v = o * b3 + t;
+    if (!ok) {
+        *part.sign(v) = REDO;
+        return;
+    }
+    const int dz = ea + EB[0] - sc.exp;
+    int *dig = part.digits(v);
+#pragma unroll
+    for (int r = 0; r < N; r++) dig[r] = (int)fmod_exact(acc[r], C->md[r], C->minv[r]);
+    for (int r = 0; r < N; r++) {
+        const int w = mulmod(dig[r], pow2mod(C, dz, r), C->md[r], C->minv[r]);
+        dig[r] = (sc.sign && w) ? C->m[r] - w : w;
+    }
+    *part.sign(v) = sc.sign;
+    *part.exp(v) = sc.exp;
+    ErMem *ev = part.eval(v);
+    ev[0] = ErMem{sc.ev0.frac, (long long)sc.ev0.exp};
+    ev[1] = ErMem{sc.ev1.frac, (long long)sc.ev1.exp};
 }
 
 // ---------------------------------------------------------------- SpMV

@@ -114,23 +114,50 @@ struct Launch {
                                AoSRef<N>{out});
         return hipGetLastError();
     }
+    // gemv passes: 0 preparation (split form: row minima of A's exponents, minimum
+    // and pre-scaled residues of bx), 1 main kernel, 2 fixup of marked partials.
+    // aux: the split form's bxp (N*n ints), EA (m ints), EB (1 int).
+    static size_t gemv_aux(bool trans, int m, int n) {
+        return (SPLIT && !trans) ? 4 * ((size_t)N * n + m + 1) : 0;
+    }
     template <bool TR>
     static void gemv_pass(int pass, const DevConsts *c, int m, int n, SoA A, int lda, AoSRef<N> bx, int b3,
-                          long long nvt, AoSRef<N> part, hipStream_t st) {
+                          long long nvt, AoSRef<N> part, char *aux, hipStream_t st) {
         const dim3 gr(grid_for(nvt, L)), bl(BLK);
-        if (pass == 0)
-            hipLaunchKernelGGL((k_gemv_fold<N, L, TR, !SPLIT, false>), gr, bl, 0, st, c, m, n, A, lda, bx, b3, nvt,
-                               part);
-        else if constexpr (SPLIT)
+        const bool split = SPLIT && !TR && b3 <= 65535;
+        int *bxp = (int *)aux, *EA = bxp + (size_t)N * n, *EB = EA + m;
+        if (pass == 0) {
+            if (split) {
+                const unsigned kc = (unsigned)((n + EMIN_K - 1) / EMIN_K);
+                hipLaunchKernelGGL((k_fill_int<N, L>), dim3(grid_for((long long)m + 1, 1)), bl, 0, st, EA, m + 1,
+                                   0x7fffffff);
+                hipLaunchKernelGGL((k_gemm_emin<N, L, SoA, 0>), dim3((m + EMIN_J - 1) / EMIN_J, kc), bl, 0, st, m, n, 0,
+                                   A, lda, EA);
+                hipLaunchKernelGGL((k_gemm_emin<N, L, AoSRef<N>, 1>), dim3(1, kc), bl, 0, st, 1, n, 0, bx, 0, EB);
+                hipLaunchKernelGGL((k_gemm_prescale<N, L, AoSRef<N>, 1>), dim3(grid_for(n, 1)), bl, 0, st, c, 1, n, 0,
+                                   bx, 0, EB, bxp);
+            }
+        } else if (pass == 1) {
+            if (split)
+                hipLaunchKernelGGL((k_gemv_split<N, L>), dim3((m + BLK - 1) / BLK, b3), bl, 0, st, c, m, n, A, lda, bx,
+                                   bxp, EA, EB, b3, part);
+            else if (!TR && b3 <= 65535)  // grid y limit
+                hipLaunchKernelGGL((k_gemv_rows<N, L, !SPLIT>), dim3((m + BLK / L - 1) / (BLK / L), b3), bl, 0, st, c,
+                                   m, n, A, lda, bx, b3, part);
+            else
+                hipLaunchKernelGGL((k_gemv_fold<N, L, TR, !SPLIT, false>), gr, bl, 0, st, c, m, n, A, lda, bx, b3,
+                                   nvt, part);
+        } else if constexpr (SPLIT) {
             hipLaunchKernelGGL((k_gemv_fold<N, L, TR, true, true>), gr, bl, 0, st, c, m, n, A, lda, bx, b3, nvt, part);
+        }
     }
     static hipError_t gemv_fold(const DevConsts *c, bool trans, int pass, int m, int n, SoA A, int lda, const char *bx,
-                                int b3, long long nvt, char *part, hipStream_t st) {
+                                int b3, long long nvt, char *part, char *aux, hipStream_t st) {
         if (nvt <= 0) return hipSuccess;
         if (trans)
-            gemv_pass<true>(pass, c, m, n, A, lda, AoSRef<N>{(char *)bx}, b3, nvt, AoSRef<N>{part}, st);
+            gemv_pass<true>(pass, c, m, n, A, lda, AoSRef<N>{(char *)bx}, b3, nvt, AoSRef<N>{part}, aux, st);
         else
-            gemv_pass<false>(pass, c, m, n, A, lda, AoSRef<N>{(char *)bx}, b3, nvt, AoSRef<N>{part}, st);
+            gemv_pass<false>(pass, c, m, n, A, lda, AoSRef<N>{(char *)bx}, b3, nvt, AoSRef<N>{part}, aux, st);
         return hipGetLastError();
     }
     static hipError_t gemv_final(const DevConsts *c, int len, SoA y, int incy, const char *sums, hipStream_t st) {
@@ -159,9 +186,14 @@ struct Launch {
         int *PA = (int *)(ws + sb_), *PB = PA + (size_t)N * k * m, *EA = PB + (size_t)N * k * n, *EB = EA + m;
         if constexpr (SPLIT) {
             if (pass == 0) {
-                hipLaunchKernelGGL((k_gemm_emin<N, L, VM, 0>), dim3(grid_for(m, 1)), bl, 0, st, m, k, ta, A, lda, EA);
-                hipLaunchKernelGGL((k_gemm_emin<N, L, VM, 1>), dim3(grid_for(n, 1)), bl, 0, st, n, k, tb, B, ldb, EB);
+                hipLaunchKernelGGL((k_fill_int<N, L>), dim3(grid_for((long long)m + n, 1)), bl, 0, st, EA, m + n,
+                                   k > 0 ? 0x7fffffff : 0);
                 if (k > 0) {
+                    const unsigned kc = (unsigned)((k + EMIN_K - 1) / EMIN_K);
+                    hipLaunchKernelGGL((k_gemm_emin<N, L, VM, 0>), dim3((m + EMIN_J - 1) / EMIN_J, kc), bl, 0, st, m,
+                                       k, ta, A, lda, EA);
+                    hipLaunchKernelGGL((k_gemm_emin<N, L, VM, 1>), dim3((n + EMIN_J - 1) / EMIN_J, kc), bl, 0, st, n,
+                                       k, tb, B, ldb, EB);
                     hipLaunchKernelGGL((k_gemm_prescale<N, L, VM, 0>), dim3(grid_for((long long)m * k, 1)), bl, 0, st,
                                        c, m, k, ta, A, lda, EA, PA);
                     hipLaunchKernelGGL((k_gemm_prescale<N, L, VM, 1>), dim3(grid_for((long long)n * k, 1)), bl, 0, st,
@@ -238,6 +270,7 @@ struct Launch {
         o.gemv_final = &gemv_final;
         o.gemm = &gemm;
         o.gemm_ws = &gemm_ws;
+        o.gemv_aux = &gemv_aux;
         o.gemm_v2 = &gemm_v2;
         o.spmv = &spmv;
         return o;

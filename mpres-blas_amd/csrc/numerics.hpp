@@ -178,19 +178,27 @@ MPH_HD double mod_fix(double r, double m) {
     r = (r >= m) ? r - m : r;
     return r;
 }
-// (a * b) mod m for 0 <= a, b < 2^31
+// x in [-m, 2m) -> [0, m)
+MPH_HD double mod_fix1(double r, double m) {
+    r = (r < 0) ? r + m : r;
+    return (r >= m) ? r - m : r;
+}
+// (a * b) mod m for 0 <= a, b < m < 2^31. With p = RN(a b) and pl = a b - p
+// (exact, |pl| <= m^2 2^-53), RN(p RN(1/m)) differs from a b / m by less than
+// m 2^-51 < 2^-20, so q = floor of it is within one of the true quotient and
+// r = a b - q m (formed exactly: |p - q m| < 2^32) lies in [-m, 2m).
 MPH_HD int mulmod(int a, int b, double m, double minv) {
     const double da = (double)a, db = (double)b;
     const double p = da * db;
     const double pl = fma(da, db, -p);
     const double q = floor(p * minv);
     double r = fma(-q, m, p) + pl;
-    return (int)mod_fix(r, m);
+    return (int)mod_fix1(r, m);
 }
-// v mod m for an integer-valued |v| < 2^53
+// v mod m for an integer-valued 0 <= v < 2^53 (q is again within one of v / m)
 MPH_HD int modd(double v, double m, double minv) {
     const double q = floor(v * minv);
-    return (int)mod_fix(fma(-q, m, v), m);
+    return (int)mod_fix1(fma(-q, m, v), m);
 }
 
 }  // namespace mph

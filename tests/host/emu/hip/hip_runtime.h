@@ -21,6 +21,10 @@ typedef int hipError_t;
 typedef void *hipStream_t;
 enum { hipSuccess = 0, hipErrorInvalidValue = 1 };
 
+struct int4 {
+    int x, y, z, w;
+};
+
 struct EmuIdx {
     unsigned x, y, z;
 };
@@ -31,5 +35,10 @@ template <class T>
 inline T __shfl_xor(T v, int, int = 64) { return v; }
 template <class T>
 inline T __shfl(T v, int, int = 64) { return v; }
+inline int atomicMin(int *p, int v) {
+    const int o = *p;
+    if (v < o) *p = v;
+    return o;
+}
 inline void __syncthreads() {}
 inline void __threadfence_block() {}

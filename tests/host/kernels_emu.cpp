@@ -4,6 +4,7 @@
 // (block, thread) is run in turn. This checks the kernels' own logic (fast
 // path, marking, fixup, epilogues) against the oracle on the CPU, where a
 // mistake can be found with ordinary tools. Built by tests/test_kernels_emu.py.
+#include <algorithm>
 #include <vector>
 
 #include "../../mpres-blas_amd/csrc/internal.hpp"
@@ -65,8 +66,17 @@ int gemm_v2_split_n(const Ctx &x, int ta, int tb, int m, int n, int k, char *alp
     const long long mn = (long long)m * n;
     const AoSRef<N> a{A}, b{B}, w{ws};
     std::vector<int> EA(m), EB(n), PA((size_t)N * k * m), PB((size_t)N * k * n);
-    run(m, [&] { k_gemm_emin<N, 1, AoSRef<N>, 0>(m, k, ta, a, lda, EA.data()); });
-    run(n, [&] { k_gemm_emin<N, 1, AoSRef<N>, 1>(n, k, tb, b, ldb, EB.data()); });
+    // the exponent minima kernels reduce through LDS across a workgroup: restated here
+    for (int row = 0; row < m; row++) {
+        int e = 0x7fffffff;
+        for (int i = 0; i < k; i++) e = std::min(e, *a.exp(op_index<0>(ta, row, i, lda)));
+        EA[row] = k > 0 ? e : 0;
+    }
+    for (int col = 0; col < n; col++) {
+        int e = 0x7fffffff;
+        for (int i = 0; i < k; i++) e = std::min(e, *b.exp(op_index<1>(tb, col, i, ldb)));
+        EB[col] = k > 0 ? e : 0;
+    }
     run((long long)m * k, [&] { k_gemm_prescale<N, 1, AoSRef<N>, 0>(&x.c, m, k, ta, a, lda, EA.data(), PA.data()); });
     run((long long)n * k, [&] { k_gemm_prescale<N, 1, AoSRef<N>, 1>(&x.c, n, k, tb, b, ldb, EB.data(), PB.data()); });
     run(mn, [&] { k_gemm_chain<N, 1, AoSRef<N>>(&x.c, ta, tb, m, n, k, a, lda, b, ldb, w); });
