@@ -32,6 +32,10 @@ import gen  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
 PROF_GEMM_RESID = 6        # library timing slot of the gemm residue products
+# HBM bytes per launch of the step's main kernels on this workload, from rocprofv3
+# --pmc FETCH_SIZE and --pmc WRITE_SIZE (profiles/bench_424bit_counters.md), each
+# scaled by its reading for a 256 MiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0)
+TRAFFIC_BYTES = {"chain": 128.4e6, "resid": 649.9e6, "gemv": 2899.6e6}
 FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 (vector and matrix)
 
 SET = "32_8double"
@@ -64,39 +68,46 @@ def exact_value(rec, m):
     return -v if int(rec["sign"]) else v
 
 
-def cpu_baseline(threads):
+def cpu_baseline(threads, min_seconds=10.0, max_reps=20):
     """The reference's CPU mp_float_t algorithm (the oracle's policy-E restatement),
-    composed in the GPU's order, on a bounded sample: 2048 full rows of the gemv
-    and a 256 x 256 block of the gemm (full K)."""
+    composed in the GPU's order, on the same workload as the GPU step (gemv 4096 x
+    4096 + gemm 512^3, same input distribution), repeated until about min_seconds
+    of CPU time have been measured."""
     sys.path[:0] = [os.path.join(ROOT, "oracle")]
     import pyoracle as po
 
     po.set_threads(threads)
     o = po.Oracle(SET)
-    rows = 2048
-    bm = bn = 256
-    A = o.set_d(gen.uexp(301, rows * N_GEMV, ELO, EHI), po.POLICY_E)
-    x = o.set_d(gen.uexp(302, N_GEMV, ELO, EHI), po.POLICY_E)
-    y = o.set_d(gen.uexp(303, rows, ELO, EHI), po.POLICY_E)
-    al = o.set_d(gen.uexp(304, 1, ELO, EHI), po.POLICY_E)
-    be = o.set_d(gen.uexp(305, 1, ELO, EHI), po.POLICY_E)
-    Ag = o.set_d(gen.uexp(311, bm * K_GEMM, ELO, EHI), po.POLICY_E)
-    Bg = o.set_d(gen.uexp(312, K_GEMM * bn, ELO, EHI), po.POLICY_E)
-    Cg = o.set_d(gen.uexp(313, bm * bn, ELO, EHI), po.POLICY_E)
-    t0 = time.perf_counter()
-    o.gemv(0, rows, N_GEMV, al, A, rows, x, be, y, B3, policy=po.POLICY_E)
-    t1 = time.perf_counter()
-    o.gemm(bm, bn, K_GEMM, al, Ag, bm, Bg, K_GEMM, be, Cg, bm, policy=po.POLICY_E)
-    t2 = time.perf_counter()
-    fmas = rows * N_GEMV + bm * bn * K_GEMM
+    E = po.POLICY_E
+    A = o.set_d(gen.uexp(301, M_GEMV * N_GEMV, ELO, EHI), E)
+    x = o.set_d(gen.uexp(302, N_GEMV, ELO, EHI), E)
+    y = o.set_d(gen.uexp(303, M_GEMV, ELO, EHI), E)
+    al = o.set_d(gen.uexp(304, 1, ELO, EHI), E)
+    be = o.set_d(gen.uexp(305, 1, ELO, EHI), E)
+    Ag = o.set_d(gen.uexp(311, M_GEMM * K_GEMM, ELO, EHI), E)
+    Bg = o.set_d(gen.uexp(312, K_GEMM * N_GEMM, ELO, EHI), E)
+    Cg = o.set_d(gen.uexp(313, M_GEMM * N_GEMM, ELO, EHI), E)
+    t_gemv = t_gemm = 0.0
+    reps = 0
+    while reps < max_reps and t_gemv + t_gemm < min_seconds:
+        t0 = time.perf_counter()
+        o.gemv(0, M_GEMV, N_GEMV, al, A, M_GEMV, x, be, y, B3, policy=E)
+        t1 = time.perf_counter()
+        o.gemm(M_GEMM, N_GEMM, K_GEMM, al, Ag, M_GEMM, Bg, K_GEMM, be, Cg, M_GEMM, policy=E)
+        t2 = time.perf_counter()
+        t_gemv += t1 - t0
+        t_gemm += t2 - t1
+        reps += 1
+        print(f"[bench] CPU step {reps}: {t2 - t0:.2f} s", file=sys.stderr, flush=True)
+    fmas = M_GEMV * N_GEMV + M_GEMM * N_GEMM * K_GEMM
     return {
-        "value": fmas / (t2 - t0),
+        "value": reps * fmas / (t_gemv + t_gemm),
         "unit": "mp-FMA/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{rows} rows of the 4096x4096 gemv ({rows * N_GEMV} mp-FMA, {t1 - t0:.2f} s) + a "
-                  f"{bm}x{bn} block of the 512^3 gemm ({bm * bn * K_GEMM} mp-FMA, {t2 - t1:.2f} s), "
-                  f"CPU mp_float_t algorithm (emulated directed rounding), OpenMP over rows/entries",
+        "sample": f"{reps} repetitions of the full step (gemv 4096x4096: {t_gemv / reps:.2f} s, gemm 512^3: "
+                  f"{t_gemm / reps:.2f} s each), CPU mp_float_t algorithm (emulated directed rounding), "
+                  f"OpenMP over rows/entries with {threads} threads",
     }
 
 
@@ -107,6 +118,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--counters", action="store_true",
+                    help="for hardware-counter runs: one warm-up and one measured call of the step's gemv and gemm, "
+                         "then two 256 MiB device copies (the calibration of the byte counters); no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); run with --gpus 1", file=sys.stderr)
@@ -137,6 +151,9 @@ def main():
     print("[bench] inputs converted", file=sys.stderr, flush=True)
 
     L = ctx.L
+    if args.counters:
+        counter_run(ctx, mp, A, x, y, alpha, beta, Ag, Bg, Cg, alpha_g, beta_g, ws, ws_bytes)
+        return
     ev = [mp.Timer() for _ in range(2)]
     t_gemv = [0.0]
     t_gemm = [0.0]
@@ -201,19 +218,21 @@ def main():
         # eval unit) and one sum, i.e. 6 FP64 operations
         ach = 6.0 * fma_gemm / (gemm_avg / 1e3) / 1e12
         roof = {"kernel": "k_gemm_chain (scalar chains of the gemm entries)", "bound": "mfma", "achieved": ach,
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
+                "traffic": TRAFFIC_BYTES["chain"],
                 "avg_ms": gemm_avg, "work": "M*N*K interval steps, 6 FP64 operations each"}
     elif dom == "resid":
         # residue multiply-accumulates M*N*K*n, 2 FP64 FMAs (4 flops) each
         ach = 4.0 * fma_gemm * nmod / (resid_avg / 1e3) / 1e12
         roof = {"kernel": "k_gemm_resid (modular residue products)", "bound": "mfma", "achieved": ach,
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
+                "traffic": TRAFFIC_BYTES["resid"],
                 "avg_ms": resid_avg, "work": "M*N*K*n residue multiply-accumulates, 2 FP64 FMAs each"}
     else:
         bytes_ = (M_GEMV * N_GEMV + N_GEMV + 2 * M_GEMV) * S
         ach = bytes_ / (gemv_avg / 1e3) / 1e9
-        roof = {"kernel": "k_gemv_fold", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": ach / HBM_PEAK_GBS, "traffic": None, "avg_ms": gemv_avg,
+        roof = {"kernel": "k_gemv_split (gemv partial sums)", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES["gemv"], "avg_ms": gemv_avg,
                 "work": "(m*n + n + 2m) * (4n+40) bytes"}
 
     # ---- accuracy: sampled outputs against the exact dyadic result ----
@@ -252,6 +271,22 @@ def main():
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def counter_run(ctx, mp, A, x, y, alpha, beta, Ag, Bg, Cg, alpha_g, beta_g, ws, ws_bytes):
+    L = ctx.L
+    for _ in range(2):
+        mp.check(L.mpres_gemv(ctx.h, mp.NO_TRANS, M_GEMV, N_GEMV, C.byref(alpha), C.byref(A), M_GEMV, C.byref(x), 1,
+                              C.byref(beta), C.byref(y), 1, B3, ws.ptr, ws_bytes, None))
+        mp.check(L.mpres_gemm(ctx.h, mp.NO_TRANS, mp.NO_TRANS, M_GEMM, N_GEMM, K_GEMM, C.byref(alpha_g), C.byref(Ag),
+                              M_GEMM, C.byref(Bg), K_GEMM, C.byref(beta_g), C.byref(Cg), M_GEMM, None, 0, None))
+    src, dst = mp.DevBuf(nbytes=256 << 20), mp.DevBuf(nbytes=256 << 20)
+    H = mp.hip()
+    for _ in range(2):
+        if H.hipMemcpy(dst.ptr, src.ptr, C.c_size_t(256 << 20), 3) != 0:
+            raise RuntimeError("device copy failed")
+    mp.synchronize()
+    print("[bench] counter run done", file=sys.stderr, flush=True)
 
 
 def ctx_moduli(ctx):
