@@ -137,6 +137,21 @@ int mpres_gemm(mpres_ctx_t *ctx, int transa, int transb, int m, int n, int k, co
                const mp_array_t *A, int lda, const mp_array_t *B, int ldb, const mp_array_t *beta, mp_array_t *C,
                int ldc, void *ws, size_t ws_bytes, void *stream);
 
+/* C = alpha*op(A)*op(B) + beta*C in the precision form on INT8 MFMA (no counterpart in the
+ * reference; the reference's gemm is cuda::mp_gemm, src/blas/gemm.cuh:69-167). Every row of
+ * op(A) and column of op(B) shares one power-of-two exponent; the aligned residues are split
+ * into 7-bit limbs and multiplied exactly on INT8 MFMA with INT32 accumulation, so the product
+ * is one exact sum per entry, rounded once to the working precision (the reference rounds every
+ * product and partial sum). Rows or columns whose aligned magnitudes exceed the bit budget that
+ * keeps |sum| < M/4 are truncated by the excess bits (tile alignment rounding). The epilogue
+ * is (sum*alpha) + (C*beta) with mp_mul / mp_add. Results are not bit-identical to mpres_gemm.
+ * Requires 0 <= k <= MPRES_MFMA_MAX_K; transposes allowed. */
+#define MPRES_MFMA_MAX_K 16384
+size_t mpres_gemm_mfma_workspace(const mpres_ctx_t *ctx, int m, int n, int k);
+int mpres_gemm_mfma(mpres_ctx_t *ctx, int transa, int transb, int m, int n, int k, const mp_array_t *alpha,
+                    const mp_array_t *A, int lda, const mp_array_t *B, int ldb, const mp_array_t *beta, mp_array_t *C,
+                    int ldc, void *ws, size_t ws_bytes, void *stream);
+
 /* ---- BLAS, v2 interface over mp_float_t arrays (src/blas/v2) ---- */
 /* cuda::mp_dot<blocks,threads>(n,x,y,r), src/blas/v2/dot_v2.cuh:121-129 */
 size_t mpres_dot_v2_workspace(const mpres_ctx_t *ctx, int n, int blocks, int threads);
@@ -163,7 +178,8 @@ int mpres_spmv_csrv(mpres_ctx_t *ctx, int m, const csr_t *csr, int threads_per_r
  * events on the call's stream. mpres_profile_read waits for the recorded
  * events and returns the accumulated kernel time and launch count of one
  * operation class (same op numbering as mpres_set_lanes; op 6: the residue
- * products of gemm over 16 and 32 moduli, whose op 3 time is the scalar chains). */
+ * products of gemm over 16 and 32 moduli, whose op 3 time is the scalar chains;
+ * op 7: mpres_gemm_mfma, all its kernels). */
 int mpres_profile(mpres_ctx_t *ctx, int enable);
 int mpres_profile_read(mpres_ctx_t *ctx, int op, double *ms, long *launches, int reset);
 

@@ -44,7 +44,7 @@ int fail(int code, const char *fmt, ...) {
 
 enum OpKind { OP_ELEM = 0, OP_DOT = 1, OP_GEMV = 2, OP_GEMM = 3, OP_SPMV = 4, OP_AXPY = 5, OP_COUNT = 6 };
 // extra timing slot: the residue products of the split-form gemm
-constexpr int PROF_GEMM_RESID = 6, PROF_SLOTS = 7;
+constexpr int PROF_GEMM_RESID = 6, PROF_GEMM_MFMA = 7, PROF_SLOTS = 8;
 
 const Ops *find_ops(int n, int L) { return find_variant(n, L); }
 
@@ -574,6 +574,29 @@ int mpres_gemm(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const mp_arr
         HIPCHK(o->gemm(c->dev, pass, c->bits, m, n, k, soa_of(alpha), soa_of(A), lda, soa_of(B), ldb, soa_of(beta),
                        soa_of(C), ldc, w, strm(st)));
     }
+    return MPRES_OK;
+}
+
+size_t mpres_gemm_mfma_workspace(const mpres_ctx_t *c, int m, int n, int k) {
+    if (!c || m <= 0 || n <= 0 || k < 0) return 0;
+    return c->ops[OP_GEMM]->gemm_mfma_ws(c->bits, m, n, k);
+}
+
+int mpres_gemm_mfma(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const mp_array_t *alpha,
+                    const mp_array_t *A, int lda, const mp_array_t *B, int ldb, const mp_array_t *beta, mp_array_t *C,
+                    int ldc, void *ws, size_t wsb, void *st) {
+    if (!c || !alpha || !A || !B || !beta || !C) return fail(MPRES_ERR_ARG, "null argument");
+    if (m <= 0 || n <= 0) return MPRES_OK;
+    const bool tra = ta == MPRES_TRANS || ta == MPRES_CONJ_TRANS, trb = tb == MPRES_TRANS || tb == MPRES_CONJ_TRANS;
+    if (k < 0 || lda < std::max(1, tra ? k : m) || ldb < std::max(1, trb ? n : k) || ldc < std::max(1, m))
+        return fail(MPRES_ERR_ARG, "invalid dimensions");
+    if (k > MPRES_MFMA_MAX_K) return fail(MPRES_ERR_UNSUPPORTED, "k = %d exceeds %d", k, MPRES_MFMA_MAX_K);
+    char *w;
+    int rc = get_ws(c, ws, wsb, mpres_gemm_mfma_workspace(c, m, n, k), &w);
+    if (rc) return rc;
+    ProfScope ps(c, PROF_GEMM_MFMA, strm(st));
+    HIPCHK(c->ops[OP_GEMM]->gemm_mfma(c->dev, c->bits, c->host.log2m, tra, trb, m, n, k, soa_of(alpha), soa_of(A), lda,
+                                      soa_of(B), ldb, soa_of(beta), soa_of(C), ldc, w, strm(st)));
     return MPRES_OK;
 }
 

@@ -247,6 +247,20 @@ struct Grp {
             }
         }
     }
+    // full interval evaluation of a number's residues (rns_eval_compute, CUDA)
+    __device__ __forceinline__ void eval_full(const int *d, Er &lo, Er &up) const {
+        if constexpr (INL) {
+            rns_eval<N, L, true, true>(k, d, lo, up, false);
+        } else {
+            int t[R];
+            Er a, b;
+#pragma unroll
+            for (int r = 0; r < R; r++) t[r] = d[r];
+            ool_eval_full<N, L>(k, t, &a, &b, false);
+            lo = a;
+            up = b;
+        }
+    }
     // sign_estimate, src/arith/arith_utils.cuh:199-207
     __device__ __forceinline__ int sign_estimate(const int *dx, const int *dy, int sx, int sy, int gamma, int theta,
                                                  int nzx, int nzy) const {

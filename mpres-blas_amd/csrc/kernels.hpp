@@ -541,7 +541,8 @@ __global__ __launch_bounds__(BLK) void k_fill_int(int *p, int cnt, int v) {
     const long long i = (long long)blockIdx.x * BLK + threadIdx.x;
     if (i < cnt) p[i] = v;
 }
-template <int N, int L, class VM, int SIDE>
+// SKIPZ: ignore zero elements (interval upper bound 0)
+template <int N, int L, class VM, int SIDE, bool SKIPZ = false>
 __global__ __launch_bounds__(BLK) void k_gemm_emin(int cnt, int k, int t, VM X, int ld, int *out) {
     if constexpr (std::is_same<VM, SoA>::value) X = rd(X);
     constexpr int S = BLK / EMIN_J;
@@ -550,7 +551,14 @@ __global__ __launch_bounds__(BLK) void k_gemm_emin(int cnt, int k, int t, VM X, 
     if (j >= cnt) return;
     int e = 0x7fffffff;
     for (int i = i0 + (int)threadIdx.x / EMIN_J; i < i1; i += S) {
-        const int v = exp_of<N>(X, op_index<SIDE>(t, (int)j, i, ld));
+        const long long idx = op_index<SIDE>(t, (int)j, i, ld);
+        if constexpr (SKIPZ) {
+            int s_, e_;
+            Er lo_, up_;
+            scalars_of<N>(X, idx, s_, e_, lo_, up_);
+            if (up_.frac == 0) continue;
+        }
+        const int v = exp_of<N>(X, idx);
         e = v < e ? v : e;
     }
     if (e != 0x7fffffff) atomicMin(out + j, e);

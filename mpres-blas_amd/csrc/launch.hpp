@@ -10,6 +10,7 @@
 
 #if defined(__HIPCC__) && defined(MPH_KERNELS)
 #include "kernels.hpp"
+#include "kernels_mfma.hpp"
 
 namespace mph {
 
@@ -240,6 +241,55 @@ struct Launch {
                                AoSRef<N>{ws});
         return hipGetLastError();
     }
+    // ---- precision form on INT8 MFMA (kernels_mfma.hpp) ----
+    static constexpr long long rup(long long v, long long q) { return (v + q - 1) / q * q; }
+    static int mf_limbs(int bits) { return (bits + MF_LB - 1) / MF_LB; }
+    // ws: S (N*m*n ints), EA EB (m + n ints), bits / effective exponents (m + n ints), limb planes
+    static size_t gemm_mfma_ws(int bits, int m, int n, int k) {
+        const int nl = mf_limbs(bits);
+        const size_t MPd = rup(m, 64), NPd = rup(n, 64), KPd = rup(k, MF_T);
+        return 4 * ((size_t)N * m * n + 2 * ((size_t)m + n)) + 256 + (size_t)N * nl * (MPd + NPd) * KPd;
+    }
+    static hipError_t gemm_mfma(const DevConsts *c, int bits, int log2m, int ta, int tb, int m, int n, int k,
+                                SoA alpha, SoA A, int lda, SoA B, int ldb, SoA beta, SoA Cm, int ldc, char *ws,
+                                hipStream_t st) {
+        if ((long long)m * n <= 0) return hipSuccess;
+        const int nl = mf_limbs(bits);
+        if (nl != 4 && nl != 5) return hipErrorInvalidValue;
+        const int MP = (int)rup(m, 64), NP = (int)rup(n, 64), KP = (int)rup(k > 0 ? k : 1, MF_T);
+        int *S = (int *)ws, *EA = S + (size_t)N * m * n, *EB = EA + m, *BA = EB + n, *BB = BA + m;
+        signed char *PA = (signed char *)ws + ((4 * ((size_t)N * m * n + 2 * ((size_t)m + n)) + 255) / 256 * 256);
+        signed char *PB = PA + (size_t)N * nl * MP * KP;
+        const dim3 bl(BLK);
+        hipLaunchKernelGGL((k_fill_int<N, L>), dim3(grid_for((long long)m + n, 1)), bl, 0, st, EA, m + n, 0x7fffffff);
+        hipLaunchKernelGGL((k_fill_int<N, L>), dim3(grid_for((long long)m + n, 1)), bl, 0, st, BA, m + n,
+                           -0x40000000);
+        if (k > 0) {
+            const unsigned kc = (unsigned)((k + EMIN_K - 1) / EMIN_K);
+            const dim3 ga((m + EMIN_J - 1) / EMIN_J, kc), gb((n + EMIN_J - 1) / EMIN_J, kc);
+            hipLaunchKernelGGL((k_gemm_emin<N, L, SoA, 0, true>), ga, bl, 0, st, m, k, ta, A, lda, EA);
+            hipLaunchKernelGGL((k_gemm_emin<N, L, SoA, 1, true>), gb, bl, 0, st, n, k, tb, B, ldb, EB);
+            hipLaunchKernelGGL((k_mf_bits<N, L, SoA, 0>), ga, bl, 0, st, c, m, k, ta, A, lda, EA, BA);
+            hipLaunchKernelGGL((k_mf_bits<N, L, SoA, 1>), gb, bl, 0, st, c, n, k, tb, B, ldb, EB, BB);
+        }
+        int lk = 0;
+        while ((1LL << lk) < (long long)(k > 0 ? k : 1)) lk++;
+        const int half = (log2m - 2 - lk) / 2;
+        hipLaunchKernelGGL((k_mf_shift<N, L>), dim3(grid_for(m, 1)), bl, 0, st, m, half, EA, BA);
+        hipLaunchKernelGGL((k_mf_shift<N, L>), dim3(grid_for(n, 1)), bl, 0, st, n, half, EB, BB);
+        hipLaunchKernelGGL((k_mf_prescale<N, L, SoA, 0>), dim3(grid_for((long long)MP * KP, 1)), bl, 0, st, c, m, MP,
+                           k, KP, nl, ta, A, lda, EA, PA);
+        hipLaunchKernelGGL((k_mf_prescale<N, L, SoA, 1>), dim3(grid_for((long long)NP * KP, 1)), bl, 0, st, c, n, NP,
+                           k, KP, nl, tb, B, ldb, EB, PB);
+        const dim3 gg(NP / 64, MP / 64, N);
+        if (nl == 4)
+            hipLaunchKernelGGL((k_mf_gemm<N, L, 4>), gg, bl, 0, st, c, m, n, MP, NP, KP, PA, PB, S);
+        else
+            hipLaunchKernelGGL((k_mf_gemm<N, L, 5>), gg, bl, 0, st, c, m, n, MP, NP, KP, PA, PB, S);
+        hipLaunchKernelGGL((k_mf_finish<N, L>), dim3(grid_for((long long)m * n, L)), bl, 0, st, c, m, n, lk, S, EA, EB,
+                           BA, BB, alpha, beta, Cm, ldc);
+        return hipGetLastError();
+    }
     static hipError_t spmv(const DevConsts *c, int pass, int m, const int *irp, const int *ja, const double *as,
                            const char *x, char *y, int tpr, hipStream_t st) {
         if (m <= 0) return hipSuccess;
@@ -271,6 +321,8 @@ struct Launch {
         o.gemm = &gemm;
         o.gemm_ws = &gemm_ws;
         o.gemv_aux = &gemv_aux;
+        o.gemm_mfma = &gemm_mfma;
+        o.gemm_mfma_ws = &gemm_mfma_ws;
         o.gemm_v2 = &gemm_v2;
         o.spmv = &spmv;
         return o;

@@ -46,6 +46,9 @@ struct Ops {
     hipError_t (*gemm_v2)(const DevConsts *, int, int, int, int, int, int, int, const char *, const char *, int,
                           const char *, int, const char *, char *, int, char *, hipStream_t);
     size_t (*gemm_ws)(int, int, int);
+    hipError_t (*gemm_mfma)(const DevConsts *, int, int, int, int, int, int, int, SoA, SoA, int, SoA, int, SoA, SoA,
+                            int, char *, hipStream_t);
+    size_t (*gemm_mfma_ws)(int, int, int, int);
     hipError_t (*spmv)(const DevConsts *, int, int, const int *, const int *, const double *, const char *, char *,
                        int, hipStream_t);
 };

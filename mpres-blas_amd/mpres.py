@@ -83,6 +83,9 @@ def lib():
         L.mpres_gemm_workspace.restype = sz
         L.mpres_gemm_workspace.argtypes = [vp, i32, i32, i32]
         L.mpres_gemm.argtypes = [vp, i32, i32, i32, i32, i32, vp, vp, i32, vp, i32, vp, vp, i32, vp, sz, vp]
+        L.mpres_gemm_mfma_workspace.restype = sz
+        L.mpres_gemm_mfma_workspace.argtypes = [vp, i32, i32, i32]
+        L.mpres_gemm_mfma.argtypes = [vp, i32, i32, i32, i32, i32, vp, vp, i32, vp, i32, vp, vp, i32, vp, sz, vp]
         L.mpres_gemm_v2.argtypes = [vp, i32, i32, i32, i32, i32, vp, vp, i32, vp, i32, vp, vp, i32, vp, sz, vp]
         L.mpres_spmv_csr.argtypes = [vp, i32, vp, vp, vp, vp]
         L.mpres_spmv_csrv.argtypes = [vp, i32, vp, i32, vp, vp, vp]

@@ -365,3 +365,116 @@ def test_spmv_rare(name):
             ja[s + 1] = ja[s]
             as_[s + 1] = -as_[s]
     _spmv_case(name, m, irp, ja, as_, xs, (1, 2, 8))
+
+
+# ---- precision form on INT8 MFMA (mpres_gemm_mfma) ----
+def _log2m(c):
+    v = [C.c_int() for _ in range(5)]
+    mp.check(c.L.mpres_ctx_info(c.h, *[C.byref(x) for x in v]))
+    return v[4].value
+
+
+def mfma_reference(o, log2m, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc):
+    """The precision form restated with exact integers and the oracle's primitives:
+    per row of op(A) / column of op(B) the common exponent (minimum over nonzero
+    elements, raised by the excess over the bit budget), the aligned and truncated
+    operands, the exact signed sum, then the oracle's evaluation, rounding and
+    epilogue (mp_mul, mp_add)."""
+    mm = [int(v) for v in o.moduli()]
+    lk = (max(k, 1) - 1).bit_length()
+    half = (log2m - 2 - lk) // 2
+
+    def opa(row, i):
+        return A[i + lda * row] if ta else A[row + lda * i]
+
+    def opb(i, col):
+        return B[col + ldb * i] if tb else B[i + ldb * col]
+
+    def common(elems):
+        """(effective exponent, aligned bit bound) of a row of op(A) or column of op(B)"""
+        nz = [r for r in elems if float(r["eval"][1]["frac"]) != 0]
+        if not nz:
+            return 0, -(1 << 30)
+        emin = min(int(r["exp"]) for r in nz)
+        bits = max(int(r["eval"][1]["exp"]) + 2 + log2m + int(r["exp"]) - emin for r in nz)
+        return emin + max(0, bits - half), min(bits, half)
+
+    def aligned(r, eff):
+        X = exact.crt(r["digits"], mm)
+        d = int(r["exp"]) - eff
+        v = X << d if d >= 0 else X >> (-d)
+        return -v if int(r["sign"]) else v
+
+    ea, ba = zip(*[common([opa(row, i) for i in range(k)]) for row in range(m)])
+    eb, bb = zip(*[common([opb(i, col) for i in range(k)]) for col in range(n)])
+    av = [[aligned(opa(row, i), ea[row]) for i in range(k)] for row in range(m)]
+    bv = [[aligned(opb(i, col), eb[col]) for i in range(k)] for col in range(n)]
+    sums = o.empty(m * n)
+    scaled = np.zeros((m * n, len(mm)), np.int32)
+    k0s = np.zeros(m * n, np.int64)
+    for col in range(n):
+        for row in range(m):
+            S = sum(av[row][i] * bv[col][i] for i in range(k))
+            e = row + m * col
+            rec = sums[e]
+            rec["digits"] = [abs(S) % q for q in mm]
+            rec["sign"] = int(S < 0)
+            rec["exp"] = ea[row] + eb[col]
+            # evaluation of |S| 2^K0 (|S| 2^K0 < M/2), exponents lowered by K0
+            k0 = min(max(log2m - 1 - (ba[row] + bb[col] + lk), 0), log2m - 1)
+            scaled[e] = [(abs(S) << k0) % q for q in mm]
+            k0s[e] = k0
+    lo, up = o.eval(scaled, fast=False)
+    lo["exp"] = np.where(lo["frac"] != 0, lo["exp"] - k0s, lo["exp"])
+    up["exp"] = np.where(up["frac"] != 0, up["exp"] - k0s, up["exp"])
+    sums["eval"][:, 0] = lo
+    sums["eval"][:, 1] = up
+    sums = o.round(sums)
+    want = np.array(Cm, copy=True)
+    for col in range(n):
+        for row in range(m):
+            e = row + m * col
+            s2 = o.mul(sums[e:e + 1], alpha, D)
+            c2 = o.mul(Cm[col * ldc + row:col * ldc + row + 1], beta, D)
+            want[col * ldc + row] = o.add(s2, c2, D)[0]
+    return want, sums
+
+
+@pytest.mark.parametrize("name,lo,hi", [("8_2double", -8, 8), ("8_2double", -60, 60), ("16_4double", -20, 20),
+                                        ("32_8double", -60, 60), ("16_32bit", -30, 30)])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 1)])
+def test_gemm_mfma(name, lo, hi, ta, tb):
+    c, o = ctx(name), orc(name)
+    m, n, k = 37, 29, 45
+    lda = k if ta else m
+    ldb = n if tb else k
+    As = gen.uexp(121, m * k, lo, hi)
+    Bs = gen.uexp(122, k * n, lo, hi)
+    As[5] = 0.0
+    Bs[7] = -0.0
+    A, B = o.set_d(As, D), o.set_d(Bs, D)
+    Cm = o.set_d(gen.uexp(123, m * n, lo, hi), D)
+    alpha, beta = o.set_d([1.25], D), o.set_d([-0.5], D)
+    want, sums = mfma_reference(o, _log2m(c), ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, Cm, m)
+    for L in lanes_of(c):
+        c.set_lanes(mp.K_GEMM, L)
+        note("gemm_mfma", name, lo, hi, ta, tb, L)
+        aA, aB, aC = c.array_from_host(A), c.array_from_host(B), c.array_from_host(Cm)
+        aa, ab = c.array_from_host(alpha), c.array_from_host(beta)
+        mp.check(c.L.mpres_gemm_mfma(c.h, mp.TRANS if ta else mp.NO_TRANS, mp.TRANS if tb else mp.NO_TRANS, m, n, k,
+                                     C.byref(aa), C.byref(aA), lda, C.byref(aB), ldb, C.byref(ab), C.byref(aC), m,
+                                     None, 0, None))
+        same(c.array_to_host(aC, m * n), want, f"gemm_mfma {name} [{lo},{hi}] {ta}{tb} L={L}")
+    # accuracy of the sums against the exact product (truncation only when the budget is exceeded)
+    mm = [int(v) for v in o.moduli()]
+    prec = o.mp_consts()[0]
+    worst = 0.0
+    for col in range(0, n, 7):
+        for row in range(0, m, 5):
+            ex = sum(exact.mp_value(A[(i + lda * row) if ta else (row + lda * i)], mm) *
+                     exact.mp_value(B[(col + ldb * i) if tb else (i + ldb * col)], mm) for i in range(k))
+            got = exact.mp_value(sums[row + m * col], mm)
+            if ex != 0:
+                worst = max(worst, float(abs(got - ex) / abs(ex)))
+    if name != "8_2double" or hi <= 8:
+        assert worst <= 2.0 ** (-prec + 2), worst
