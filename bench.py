@@ -118,6 +118,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-mfma", action="store_true", help="skip the precision-form (INT8 MFMA) gemm lines")
     ap.add_argument("--counters", action="store_true",
                     help="for hardware-counter runs: one warm-up and one measured call of the step's gemv and gemm, "
                          "then two 256 MiB device copies (the calibration of the byte counters); no JSON line")
@@ -239,6 +240,10 @@ def main():
     m_ = [int(v) for v in ctx_moduli(ctx)]
     err = accuracy(ctx, mp, A, x, y0, y, alpha, beta, Ag, Bg, C0, Cg, alpha_g, beta_g, m_)
 
+    mfma = None
+    if not args.no_mfma:
+        mfma = mfma_lines(mp)
+
     cpu = None
     if not args.no_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
@@ -267,10 +272,106 @@ def main():
                       "gemm_resid_kernel_ms": resid_avg},
         "accuracy": err,
         "roofline": roof,
+        "mfma_form": mfma,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def mfma_lines(mp, size=2048, samples=((0, 0), (1000, 7), (2047, 2047))):
+    """Precision form of gemm on INT8 MFMA (mpres_gemm_mfma) beside the plain path on the
+    same inputs: M = N = K = size at 8 and 32 moduli, exponents in [-8, 8] and [-60, 60].
+    Errors: relative error of sampled entries of alpha*A*B + beta*C against the exact result."""
+    out = []
+    for name in ("8_2double", "32_8double"):
+        ctx = mp.Context(name)
+        m_ = [int(v) for v in ctx_moduli(ctx)]
+        for lo, hi in ((-8, 8), (-60, 60)):
+            print(f"[bench] mfma form {name} e[{lo},{hi}]", file=sys.stderr, flush=True)
+            A = ctx.array_set_d(gen.uexp(401, size * size, lo, hi))
+            B = ctx.array_set_d(gen.uexp(402, size * size, lo, hi))
+            C0 = ctx.array_set_d(gen.uexp(403, size * size, lo, hi))
+            Cm = ctx.array_set_d(gen.uexp(403, size * size, lo, hi))
+            al, be = ctx.array_set_d(gen.uexp(404, 1, lo, hi)), ctx.array_set_d(gen.uexp(405, 1, lo, hi))
+            L = ctx.L
+
+            def run(fn):
+                ctx.copy_array(Cm, C0, size * size)
+                mp.check(fn(ctx.h, mp.NO_TRANS, mp.NO_TRANS, size, size, size, C.byref(al), C.byref(A), size,
+                            C.byref(B), size, C.byref(be), C.byref(Cm), size, None, 0, None))
+
+            tm = mp.Timer()
+            run(L.mpres_gemm_mfma)
+            reps = 3
+            t_mf = 0.0
+            for _ in range(reps):
+                ctx.copy_array(Cm, C0, size * size)
+                tm.start()
+                mp.check(L.mpres_gemm_mfma(ctx.h, mp.NO_TRANS, mp.NO_TRANS, size, size, size, C.byref(al),
+                                           C.byref(A), size, C.byref(B), size, C.byref(be), C.byref(Cm), size,
+                                           None, 0, None))
+                tm.stop()
+                t_mf += tm.ms()
+            t_mf /= reps
+            err_mf = sample_error(ctx, mp, A, B, C0, Cm, al, be, size, samples, m_)
+            ctx.copy_array(Cm, C0, size * size)
+            tm.start()
+            mp.check(L.mpres_gemm(ctx.h, mp.NO_TRANS, mp.NO_TRANS, size, size, size, C.byref(al), C.byref(A), size,
+                                  C.byref(B), size, C.byref(be), C.byref(Cm), size, None, 0, None))
+            tm.stop()
+            t_pl = tm.ms()
+            err_pl = sample_error(ctx, mp, A, B, C0, Cm, al, be, size, samples, m_)
+            fmas = float(size) ** 3
+            out.append({"moduli_set": name, "bits": ctx.precision, "exponents": [lo, hi], "size": size,
+                        "mfma_ms": t_mf, "mfma_mpfma_per_s": fmas / (t_mf / 1e3),
+                        "plain_ms": t_pl, "plain_mpfma_per_s": fmas / (t_pl / 1e3), "speedup": t_pl / t_mf,
+                        "mfma_max_rel_err": err_mf, "plain_max_rel_err": err_pl,
+                        "unit_roundoff": 2.0 ** (-ctx.precision)})
+            del A, B, C0, Cm
+        ctx.close()
+    return out
+
+
+def sample_error(ctx, mp, A, B, C0, Cm, al, be, size, samples, m_):
+    """max relative error of sampled entries of Cm against alpha*A*B + beta*C0 (exact)"""
+    rows = sorted({i for i, _ in samples})
+    Ah = dict(zip(rows, fetch_rows(ctx, mp, A, rows, size)))
+    a = exact_value(ctx.array_to_host(al, 1)[0], m_)
+    b = exact_value(ctx.array_to_host(be, 1)[0], m_)
+    worst = 0.0
+    for (i, j) in samples:
+        colB = fetch_range(ctx, mp, B, j * size, size)
+        ex = a * sum(exact_value(Ah[i][t], m_) * exact_value(colB[t], m_) for t in range(size))
+        ex += b * exact_value(fetch_range(ctx, mp, C0, j * size + i, 1)[0], m_)
+        got = exact_value(fetch_range(ctx, mp, Cm, j * size + i, 1)[0], m_)
+        if ex != 0:
+            worst = max(worst, float(abs(got - ex) / abs(ex)))
+    return worst
+
+
+def fetch_range(ctx, mp, X, start, count):
+    """records start .. start+count-1 of an mp_array_t"""
+    H = mp.hip()
+    n = ctx.n
+    dig = np.empty(count * n, np.int32)
+    sg = np.empty(count, np.int32)
+    ex = np.empty(count, np.int32)
+    lo = np.empty(count, mp.ER_DTYPE)
+    up = np.empty(count, mp.ER_DTYPE)
+    length = array_len(mp, X)
+    for dst, src, size in ((dig, X.digits + 4 * n * start, 4 * n * count), (sg, X.sign + 4 * start, 4 * count),
+                           (ex, X.exp + 4 * start, 4 * count), (lo, X.eval + 16 * start, 16 * count),
+                           (up, X.eval + 16 * (length + start), 16 * count)):
+        if H.hipMemcpy(dst.ctypes.data_as(C.c_void_p), C.c_void_p(src), C.c_size_t(size), 2) != 0:
+            raise RuntimeError("copy failed")
+    recs = np.zeros(count, ctx.dtype)
+    recs["digits"] = dig.reshape(count, n)
+    recs["sign"] = sg
+    recs["exp"] = ex
+    recs["eval"][:, 0] = lo
+    recs["eval"][:, 1] = up
+    return recs
 
 
 def counter_run(ctx, mp, A, x, y, alpha, beta, Ag, Bg, Cg, alpha_g, beta_g, ws, ws_bytes):
@@ -314,7 +415,7 @@ def accuracy(ctx, mp, A, x, y0, y, alpha, beta, Ag, Bg, C0, Cg, al_g, be_g, m):
     xs = [exact_value(r, m) for r in xv]
     worst = 0.0
     # rows of A: element (i, j) at j*4096 + i; fetch the needed rows through a small device gather
-    Ah = fetch_rows(ctx, mp, A, rows)
+    Ah = fetch_rows(ctx, mp, A, rows, M_GEMV)
     for t, i in enumerate(rows):
         ex = b * exact_value(y0v[i], m) + a * sum(exact_value(Ah[t][j], m) * xs[j] for j in range(N_GEMV))
         got = exact_value(yv[i], m)
@@ -336,27 +437,17 @@ def accuracy(ctx, mp, A, x, y0, y, alpha, beta, Ag, Bg, C0, Cg, al_g, be_g, m):
             "samples": "4 gemv outputs, 4 gemm entries"}
 
 
-def fetch_rows(ctx, mp, A, rows):
-    """Rows of the column-major 4096x4096 mp_array_t A (host copies of 4096 records each)."""
-    H = mp.hip()
-    n = ctx.n
-    out = []
-    dig = np.empty(n, np.int32)
-    sg = np.empty(1, np.int32)
-    ex = np.empty(1, np.int32)
-    ev = np.empty(1, mp.ER_DTYPE)
-    for i in rows:
-        recs = np.zeros(N_GEMV, ctx.dtype)
-        for j in range(N_GEMV):
-            e = j * M_GEMV + i
-            mp.hipchk(H.hipMemcpy(dig.ctypes.data_as(C.c_void_p), C.c_void_p(A.digits + 4 * n * e), 4 * n, 2))
-            mp.hipchk(H.hipMemcpy(sg.ctypes.data_as(C.c_void_p), C.c_void_p(A.sign + 4 * e), 4, 2))
-            mp.hipchk(H.hipMemcpy(ex.ctypes.data_as(C.c_void_p), C.c_void_p(A.exp + 4 * e), 4, 2))
-            recs[j]["digits"] = dig
-            recs[j]["sign"] = sg[0]
-            recs[j]["exp"] = ex[0]
-        out.append(recs)
-    return out
+def array_len(mp, X):
+    """the device-side length of an mp_array_t"""
+    v = np.zeros(1, np.int32)
+    if mp.hip().hipMemcpy(v.ctypes.data_as(C.c_void_p), C.c_void_p(X.len), C.c_size_t(4), 2) != 0:
+        raise RuntimeError("copy failed")
+    return int(v[0])
+
+
+def fetch_rows(ctx, mp, A, rows, ld):
+    """rows of a column-major mp_array_t with leading dimension and column count ld (host records)"""
+    return [np.concatenate([fetch_range(ctx, mp, A, j * ld + i, 1) for j in range(ld)]) for i in rows]
 
 
 if __name__ == "__main__":
