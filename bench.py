@@ -119,6 +119,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-mfma", action="store_true", help="skip the precision-form (INT8 MFMA) gemm lines")
+    ap.add_argument("--no-other-ops", action="store_true",
+                    help="skip the dot / axpy (106-bit) and SpMV (16 moduli) lines")
     ap.add_argument("--counters", action="store_true",
                     help="for hardware-counter runs: one warm-up and one measured call of the step's gemv and gemm, "
                          "then two 256 MiB device copies (the calibration of the byte counters); no JSON line")
@@ -243,6 +245,9 @@ def main():
     mfma = None
     if not args.no_mfma:
         mfma = mfma_lines(mp)
+    others = None
+    if not args.no_other_ops:
+        others = plain_lines(mp)
 
     cpu = None
     if not args.no_cpu:
@@ -273,6 +278,7 @@ def main():
         "accuracy": err,
         "roofline": roof,
         "mfma_form": mfma,
+        "other_ops": others,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -330,6 +336,106 @@ def mfma_lines(mp, size=2048, samples=((0, 0), (1000, 7), (2047, 2047))):
                         "unit_roundoff": 2.0 ** (-ctx.precision)})
             del A, B, C0, Cm
         ctx.close()
+    return out
+
+
+def time_call(mp, fn, reps):
+    """average milliseconds of fn() over reps calls (one untimed call first), HIP events on the null stream"""
+    fn()
+    mp.synchronize()
+    tm = mp.Timer()
+    tm.start()
+    for _ in range(reps):
+        fn()
+    tm.stop()
+    return tm.ms() / reps
+
+
+def exact_dot(a, b):
+    """exact sum of a_i * b_i over float64 arrays, as a Fraction"""
+    ma, ea = np.frexp(np.asarray(a, np.float64))
+    mb, eb = np.frexp(np.asarray(b, np.float64))
+    ia, ib = np.ldexp(ma, 53).astype(np.int64), np.ldexp(mb, 53).astype(np.int64)
+    e = ea.astype(np.int64) + eb.astype(np.int64) - 106
+    nz = (ia != 0) & (ib != 0)
+    if not nz.any():
+        return Fraction(0)
+    ia, ib, e = ia[nz], ib[nz], e[nz]
+    e0 = int(e.min())
+    s = 0
+    for p, q, k in zip(ia.tolist(), ib.tolist(), (e - e0).tolist()):
+        s += (p * q) << k
+    return Fraction(s) * Fraction(2) ** e0
+
+
+def rel_err(v, exact):
+    if exact == 0:
+        return 0.0 if v == 0 else float("inf")
+    return float(abs(v - exact) / abs(exact))
+
+
+def plain_lines(mp):
+    """The plain path's HBM-side operations: mp_dot and mp_axpy at 106 bits (8 moduli), the
+    CSR SpMV at 16 moduli. Bytes are algorithmic (every operand read once, every result
+    written once); errors are relative errors against the exact result of the fp64 inputs."""
+    out = []
+    ctx = mp.Context("8_2double")
+    S = ctx.size
+    m_ = [int(v) for v in ctx_moduli(ctx)]
+    L = ctx.L
+    for n, reps in ((65536, 20), (1 << 24, 5)):
+        print(f"[bench] dot 106-bit n={n}", file=sys.stderr, flush=True)
+        hx, hy = gen.uexp(1, n, -20, 20), gen.uexp(2, n, -20, 20)
+        x, y, r = ctx.array_set_d(hx), ctx.array_set_d(hy), ctx.array(1)
+        ms = time_call(mp, lambda: mp.check(L.mpres_dot(ctx.h, n, C.byref(x), 1, C.byref(y), 1, C.byref(r), 256, 64,
+                                                        None, 0, None)), reps)
+        err = rel_err(exact_value(ctx.array_to_host(r, 1)[0], m_), exact_dot(hx, hy)) if n <= 65536 else None
+        b = 2.0 * n * S
+        out.append({"op": "mp_dot", "moduli_set": "8_2double", "bits": ctx.precision, "n": n,
+                    "order": "gridDim3=256, blockDim3=64", "ms": ms, "mpfma_per_s": n / (ms / 1e3),
+                    "bytes": b, "gbs": b / (ms / 1e3) / 1e9, "hbm_frac": b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                    "max_rel_err": err})
+        del x, y, r
+    n = 1 << 24
+    print(f"[bench] axpy 106-bit n={n}", file=sys.stderr, flush=True)
+    x, y, al = ctx.array_set_d(gen.uexp(3, n, -20, 20)), ctx.array_set_d(gen.uexp(4, n, -20, 20)), ctx.array_set_d(
+        gen.uexp(5, 1, -20, 20))
+    ms = time_call(mp, lambda: mp.check(L.mpres_axpy(ctx.h, n, C.byref(al), C.byref(x), 1, C.byref(y), 1, None)), 5)
+    b = 3.0 * n * S
+    out.append({"op": "mp_axpy", "moduli_set": "8_2double", "bits": ctx.precision, "n": n, "ms": ms,
+                "mpfma_per_s": n / (ms / 1e3), "bytes": b, "gbs": b / (ms / 1e3) / 1e9,
+                "hbm_frac": b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS})
+    del x, y, al
+    ctx.close()
+
+    m = 1000000
+    print(f"[bench] spmv 16 moduli, {m} rows (power-law)", file=sys.stderr, flush=True)
+    irp, ja, as_ = gen.powerlaw_csr(7, m)
+    nnz = int(irp[-1])
+    ctx = mp.Context("16_4double")
+    S = ctx.size
+    m_ = [int(v) for v in ctx_moduli(ctx)]
+    L = ctx.L
+    hx = gen.uexp(8, m, -10, 10)
+    xs, ys = ctx.set_d(hx), ctx.aos(count=m)
+    dirp, dja, das = mp.DevBuf(host=irp), mp.DevBuf(host=ja), mp.DevBuf(host=as_)
+    csr = mp.Csr(das.ptr.value, dja.ptr.value, dirp.ptr.value)
+    rows = (0, 1, 12345, m - 1)
+    want = {i: exact_dot(as_[irp[i]:irp[i + 1]], hx[ja[irp[i]:irp[i + 1]]]) for i in rows}
+    b = nnz * 12.0 + (m + 1) * 4.0 + 2.0 * m * S
+    for tpr in (None, 4, 16, 32):
+        if tpr is None:
+            fn = lambda: mp.check(L.mpres_spmv_csr(ctx.h, m, C.byref(csr), xs.ptr, ys.ptr, None))  # noqa: E731
+        else:
+            fn = lambda: mp.check(L.mpres_spmv_csrv(ctx.h, m, C.byref(csr), tpr, xs.ptr, ys.ptr, None))  # noqa: E731
+        ms = time_call(mp, fn, 5)
+        yh = ctx.aos_to_host(ys, m)
+        err = max(rel_err(exact_value(yh[i], m_), want[i]) for i in rows)
+        out.append({"op": "mp_spmv_csr" if tpr is None else "mp_spmv_csrv", "threads_per_row": tpr,
+                    "moduli_set": "16_4double", "bits": ctx.precision, "rows": m, "nnz": nnz, "ms": ms,
+                    "mpfma_per_s": nnz / (ms / 1e3), "bytes": b, "gbs": b / (ms / 1e3) / 1e9,
+                    "hbm_frac": b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, "max_rel_err_sampled_rows": err})
+    ctx.close()
     return out
 
 
