@@ -119,6 +119,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-mfma", action="store_true", help="skip the precision-form (INT8 MFMA) gemm lines")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-form line (16 moduli, 4096 problems)")
     ap.add_argument("--no-other-ops", action="store_true",
                     help="skip the dot / axpy (106-bit) and SpMV (16 moduli) lines")
     ap.add_argument("--counters", action="store_true",
@@ -248,6 +249,9 @@ def main():
     others = None
     if not args.no_other_ops:
         others = plain_lines(mp)
+    batched = None
+    if not args.no_batch:
+        batched = batch_lines(mp)
 
     cpu = None
     if not args.no_cpu:
@@ -279,6 +283,7 @@ def main():
         "roofline": roof,
         "mfma_form": mfma,
         "other_ops": others,
+        "batched_form": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -435,6 +440,91 @@ def plain_lines(mp):
                     "moduli_set": "16_4double", "bits": ctx.precision, "rows": m, "nnz": nnz, "ms": ms,
                     "mpfma_per_s": nnz / (ms / 1e3), "bytes": b, "gbs": b / (ms / 1e3) / 1e9,
                     "hbm_frac": b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, "max_rel_err_sampled_rows": err})
+    ctx.close()
+    return out
+
+
+def batch_lines(mp, count=4096):
+    """Batched form (mpres_batch_run) on the batching workload at 16 moduli: the geometry
+    autotuned live (every candidate timed), the tuned plan's graph timed, and the same
+    problems run as one library call per problem. Parity: sampled problems of the batch
+    against their single calls, bit for bit; accuracy: sampled dots against the exact
+    result of the fp64 inputs."""
+    sys.path.insert(0, os.path.join(ROOT, "mpres-blas_amd"))
+    import batch_tune
+
+    ctx = mp.Context("16_4double")
+    m_ = [int(v) for v in ctx_moduli(ctx)]
+    L = ctx.L
+    print(f"[bench] batch workload: {count} problems", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    wl = gen.batch_workload(ctx, mp, count)
+    print(f"[bench] batch workload built in {time.perf_counter() - t0:.1f} s; tuning", file=sys.stderr, flush=True)
+    probs = wl["problems"]
+    best, table = batch_tune.tune(mp, ctx, probs, reps=2, log=lambda s: print(s, file=sys.stderr, flush=True))
+    default_ms = batch_tune.time_plan(mp, ctx, probs, None, reps=2)
+    plan = mp.Batch(ctx, probs, best)
+    info = plan.info()
+    ms = time_call(mp, plan.run, 5)
+
+    def single(kind, a):
+        if kind == "dot":
+            mp.check(L.mpres_dot(ctx.h, a[0], C.byref(a[1]), a[2], C.byref(a[3]), a[4], C.byref(a[5]), a[6], a[7],
+                                 None, 0, None))
+        else:
+            mp.check(L.mpres_gemv(ctx.h, a[0], a[1], a[2], C.byref(a[3]), C.byref(a[4]), a[5], C.byref(a[6]), a[7],
+                                  C.byref(a[8]), C.byref(a[9]), a[10], a[11], None, 0, None))
+
+    def singles():
+        for kind, a in wl["singles"]:
+            single(kind, a)
+
+    print("[bench] batch: one call per problem", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    ms_single = time_call(mp, singles, 2)
+    wall_single = (time.perf_counter() - t0) / 3 * 1e3
+    # parity on sampled problems: the batch's result against the single call from the same state
+    sample = list(range(0, count, max(1, count // 16)))
+    sample += [i + 1 for i in sample if i + 1 < count]
+    saved = {}
+    for i in sample:
+        kind, a = wl["singles"][i]
+        if kind == "gemv":
+            saved[i] = ctx.array_to_host(a[9], a[1])
+    plan.run()
+    mp.synchronize()
+    got = {}
+    for i in sample:
+        kind, a = wl["singles"][i]
+        got[i] = ctx.array_to_host(a[5], 1) if kind == "dot" else ctx.array_to_host(a[9], a[1])
+    mismatches = 0
+    for i in sample:
+        kind, a = wl["singles"][i]
+        if kind == "gemv":
+            mp.check(L.mpres_array_host2device(ctx.h, C.byref(a[9]), saved[i].ctypes.data_as(C.c_void_p), a[1], None))
+        single(kind, a)
+        mp.synchronize()
+        ref = ctx.array_to_host(a[5], 1) if kind == "dot" else ctx.array_to_host(a[9], a[1])
+        mismatches += int(ref.tobytes() != got[i].tobytes())
+    errs = []
+    for i in sample[:8]:
+        kind, a = wl["singles"][i]
+        if kind == "dot":
+            s = 500 + 10 + 3 * i
+            n = a[0]
+            errs.append(rel_err(exact_value(got[i][0], m_), exact_dot(gen.uexp(s, n, -20, 20),
+                                                                        gen.uexp(s + 1, n, -20, 20))))
+    plan.close()
+    fm, b = wl["fmas"], wl["bytes"]
+    out = {"workload": wl["name"], "moduli_set": "16_4double", "bits": ctx.precision, "problems": count,
+           "items": info["items"], "mp_fmas": fm, "batch_ms": ms, "batch_mpfma_per_s": fm / (ms / 1e3),
+           "per_problem_ms": ms_single, "per_problem_wall_ms": wall_single,
+           "per_problem_mpfma_per_s": fm / (ms_single / 1e3), "speedup": ms_single / ms,
+           "bytes": b, "gbs": b / (ms / 1e3) / 1e9, "hbm_frac": b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+           "tuning": best, "default_tuning_ms": default_ms, "candidates_timed": len(table),
+           "sampled_problems_bit_identical_to_single_calls": f"{len(sample) - mismatches}/{len(sample)}",
+           "sampled_dot_max_rel_err": max(errs) if errs else None}
+    del wl
     ctx.close()
     return out
 

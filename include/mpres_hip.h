@@ -172,6 +172,49 @@ int mpres_spmv_csr(mpres_ctx_t *ctx, int m, const csr_t *csr, const void *x, voi
 int mpres_spmv_csrv(mpres_ctx_t *ctx, int m, const csr_t *csr, int threads_per_row, const void *x, void *y,
                     void *stream);
 
+/* ---- batched form: many independent dot / gemv problems in one launch ----
+ * No counterpart in the reference: a batch stands for a loop of the single calls
+ * mpres_dot (cuda::mp_dot, src/blas/dot.cuh:84-107) and mpres_gemv (cuda::mp_gemv,
+ * src/blas/gemv.cuh:150-268), and every problem's result is bit-identical to its
+ * single call with the same order arguments. A plan holds a device table of work
+ * items (whole dots and runs of gemv output rows, largest first) and a HIP graph of
+ * one persistent kernel whose workgroups take items from a device counter;
+ * mpres_batch_run launches that graph. The plan keeps the operands' device
+ * pointers: the arrays must stay allocated while the plan is used, and their
+ * contents may change between runs. No output (dot r, gemv y) may overlap any
+ * operand of another problem or of its own problem: problems run in no fixed
+ * order. */
+#define MPRES_BATCH_DOT 0
+#define MPRES_BATCH_GEMV 1
+typedef struct {
+    int op;               /* MPRES_BATCH_DOT or MPRES_BATCH_GEMV */
+    int trans;            /* gemv: MPRES_NO_TRANS, MPRES_TRANS or MPRES_CONJ_TRANS */
+    int m, n;             /* dot: length n (m unused); gemv: A is m x n (column-major) */
+    int lda, incx, incy;  /* gemv: lda, incx, incy; dot: incx, incy */
+    int grid3, block3;    /* dot: the reduction shape of mpres_dot; gemv: block3 of mpres_gemv */
+    const mp_array_t *alpha, *A, *x, *beta;  /* gemv: alpha, A, x, beta; dot: x */
+    mp_array_t *y;        /* dot: second vector (read); gemv: y (updated) */
+    mp_array_t *r;        /* dot: result (one number); gemv: unused */
+} mpres_batch_problem_t;
+/* Geometry of the persistent kernel. It never changes results; 0 = built-in choice. */
+typedef struct {
+    int lanes;            /* lanes per number (a value mpres_lanes_supported accepts) */
+    int wg_per_cu;        /* persistent workgroups per compute unit */
+    int gemv_rows;        /* output rows of a gemv per work item */
+} mpres_batch_tuning_t;
+typedef struct mpres_batch mpres_batch_t;
+/* scratch bytes of a plan of these problems (0 for an empty batch) */
+size_t mpres_batch_workspace(const mpres_ctx_t *ctx, const mpres_batch_problem_t *probs, int count,
+                             const mpres_batch_tuning_t *tuning);
+/* Checks every problem as its single call does (MPRES_ERR_ARG names the problem), builds the
+ * item table and the graph. ws == NULL: the plan allocates its own scratch. tuning may be NULL. */
+int mpres_batch_plan(mpres_ctx_t *ctx, const mpres_batch_problem_t *probs, int count,
+                     const mpres_batch_tuning_t *tuning, void *ws, size_t ws_bytes, mpres_batch_t **plan);
+int mpres_batch_run(mpres_batch_t *plan, void *stream);
+/* multiply-accumulates of the batch (dot: n, gemv: m * n, summed) and the plan's item count */
+int mpres_batch_info(const mpres_batch_t *plan, double *mp_fmas, int *items, mpres_batch_tuning_t *used);
+int mpres_batch_destroy(mpres_batch_t *plan);
+
 /* ---- timing of each operation's main kernel ----
  * With profiling on, the library brackets the main kernels of every dot (first
  * reduction pass), gemv (strided row sums), gemm (sums and epilogue), SpMV and axpy call with HIP
@@ -179,7 +222,7 @@ int mpres_spmv_csrv(mpres_ctx_t *ctx, int m, const csr_t *csr, int threads_per_r
  * events and returns the accumulated kernel time and launch count of one
  * operation class (same op numbering as mpres_set_lanes; op 6: the residue
  * products of gemm over 16 and 32 moduli, whose op 3 time is the scalar chains;
- * op 7: mpres_gemm_mfma, all its kernels). */
+ * op 7: mpres_gemm_mfma, all its kernels; op 8: mpres_batch_run, the whole graph). */
 int mpres_profile(mpres_ctx_t *ctx, int enable);
 int mpres_profile_read(mpres_ctx_t *ctx, int op, double *ms, long *launches, int reset);
 

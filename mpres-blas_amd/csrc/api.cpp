@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/mpres_hip.h"
+#include "batch_plan.hpp"
 #include "internal.hpp"
 #include "ops.hpp"
 
@@ -44,7 +45,7 @@ int fail(int code, const char *fmt, ...) {
 
 enum OpKind { OP_ELEM = 0, OP_DOT = 1, OP_GEMV = 2, OP_GEMM = 3, OP_SPMV = 4, OP_AXPY = 5, OP_COUNT = 6 };
 // extra timing slot: the residue products of the split-form gemm
-constexpr int PROF_GEMM_RESID = 6, PROF_GEMM_MFMA = 7, PROF_SLOTS = 8;
+constexpr int PROF_GEMM_RESID = 6, PROF_GEMM_MFMA = 7, PROF_BATCH = 8, PROF_SLOTS = 9;
 
 const Ops *find_ops(int n, int L) { return find_variant(n, L); }
 
@@ -75,24 +76,15 @@ struct mpres_ctx {
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_free;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_live;
-    double prof_ms[8] = {};
-    long prof_count[8] = {};
+    double prof_ms[PROF_SLOTS] = {};
+    long prof_count[PROF_SLOTS] = {};
 };
 
 namespace {
 
 size_t mpsize(const mpres_ctx *c) { return (size_t)(4 * c->host.n + 40); }
 
-SoA soa_of(const mp_array_t *a) {
-    SoA s;
-    s.d = a->digits;
-    s.s = a->sign;
-    s.e = a->exp;
-    s.ev = (ErMem *)a->eval;
-    s.lenp = a->len;
-    s.len = 0;
-    return s;
-}
+SoA soa_of(const mp_array_t *a) { return soa_view(a); }
 
 // workspace: caller's buffer or the context pool
 int get_ws(mpres_ctx *c, void *ws, size_t ws_bytes, size_t need, char **out) {
@@ -668,6 +660,141 @@ int mpres_profile_read(mpres_ctx_t *c, int op, double *ms, long *launches, int r
         c->prof_ms[op] = 0;
         c->prof_count[op] = 0;
     }
+    return MPRES_OK;
+}
+
+// ---------------------------------------------------------------- batched form
+}  // extern "C"
+
+struct mpres_batch {
+    mpres_ctx *ctx = nullptr;
+    const Ops *ops = nullptr;
+    char *dmem = nullptr;  // item table, then the queue counter
+    int count = 0;
+    char *ws = nullptr;
+    bool own_ws = false;
+    double fmas = 0;
+    mpres_batch_tuning_t used{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int batch_resolve(const mpres_ctx *c, const mpres_batch_tuning_t *t, mpres_batch_tuning_t &u, const Ops **ops) {
+    u.lanes = (t && t->lanes) ? t->lanes : c->ops[OP_GEMV]->L;
+    u.wg_per_cu = (t && t->wg_per_cu) ? t->wg_per_cu : 2;
+    u.gemv_rows = (t && t->gemv_rows) ? t->gemv_rows : 64;
+    if (u.wg_per_cu < 0 || u.wg_per_cu > 32 || u.gemv_rows < 0)
+        return fail(MPRES_ERR_ARG, "invalid batch tuning (%d workgroups per CU, %d rows)", u.wg_per_cu, u.gemv_rows);
+    *ops = find_ops(c->host.n, u.lanes);
+    if (!*ops) return fail(MPRES_ERR_UNSUPPORTED, "%d lanes per number not compiled for %d moduli", u.lanes, c->host.n);
+    return MPRES_OK;
+}
+
+void batch_free(mpres_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->dmem) (void)hipFree(b->dmem);
+    if (b->own_ws && b->ws) (void)hipFree(b->ws);
+    delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t mpres_batch_workspace(const mpres_ctx_t *c, const mpres_batch_problem_t *probs, int count,
+                             const mpres_batch_tuning_t *tuning) {
+    if (!c) return 0;
+    mpres_batch_tuning_t u;
+    const Ops *o;
+    if (batch_resolve(c, tuning, u, &o)) return 0;
+    BatchLayout lay;
+    if (batch_layout(probs, count, c->host.n, u.gemv_rows, lay)) return 0;
+    return lay.ws_bytes;
+}
+
+int mpres_batch_plan(mpres_ctx_t *c, const mpres_batch_problem_t *probs, int count,
+                     const mpres_batch_tuning_t *tuning, void *ws, size_t wsb, mpres_batch_t **plan) {
+    if (!c || !plan) return fail(MPRES_ERR_ARG, "null argument");
+    *plan = nullptr;
+    mpres_batch_tuning_t u;
+    const Ops *o;
+    int rc = batch_resolve(c, tuning, u, &o);
+    if (rc) return rc;
+    BatchLayout lay;
+    rc = batch_layout(probs, count, c->host.n, u.gemv_rows, lay);
+    if (rc) return fail(rc, "%s", lay.err.c_str());
+    if (ws && wsb < lay.ws_bytes)
+        return fail(MPRES_ERR_WORKSPACE, "workspace of %zu bytes, %zu needed", wsb, lay.ws_bytes);
+    auto *b = new mpres_batch();
+    b->ctx = c;
+    b->ops = o;
+    b->count = (int)lay.items.size();
+    b->fmas = lay.fmas;
+    b->used = u;
+    *plan = b;
+    if (b->count == 0) return MPRES_OK;
+    auto bail = [&](const char *what, hipError_t e) {
+        batch_free(b);
+        *plan = nullptr;
+        return fail(MPRES_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+    };
+    hipError_t e;
+    if (ws) {
+        b->ws = (char *)ws;
+    } else if (lay.ws_bytes) {
+        if ((e = hipMalloc(&b->ws, lay.ws_bytes)) != hipSuccess) return bail("hipMalloc(batch scratch)", e);
+        b->own_ws = true;
+    }
+    const size_t tbl = lay.items.size() * sizeof(BatchItem);
+    const size_t qoff = (tbl + 255) / 256 * 256;
+    if ((e = hipMalloc(&b->dmem, qoff + 256)) != hipSuccess) return bail("hipMalloc(batch items)", e);
+    if ((e = hipMemcpy(b->dmem, lay.items.data(), tbl, hipMemcpyHostToDevice)) != hipSuccess)
+        return bail("hipMemcpy(batch items)", e);
+    int cus = 0;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device)) != hipSuccess)
+        return bail("hipDeviceGetAttribute", e);
+    const int grid = std::max(1, std::min(b->count, cus * std::max(1, u.wg_per_cu)));
+    // capture the queue reset and the persistent kernel once
+    hipStream_t cs;
+    if ((e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking)) != hipSuccess) return bail("hipStreamCreate", e);
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        const hipError_t le = o->batch(c->dev, (const BatchItem *)b->dmem, b->count, (int *)(b->dmem + qoff), b->ws,
+                                       grid, cs);
+        e = hipStreamEndCapture(cs, &b->graph);
+        if (le != hipSuccess) e = le;
+    }
+    (void)hipStreamDestroy(cs);
+    if (e != hipSuccess) return bail("capturing the batch graph", e);
+    if ((e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0)) != hipSuccess)
+        return bail("hipGraphInstantiate", e);
+    return MPRES_OK;
+}
+
+int mpres_batch_run(mpres_batch_t *b, void *st) {
+    if (!b) return fail(MPRES_ERR_ARG, "null plan");
+    if (b->count == 0) return MPRES_OK;
+    ProfScope ps(b->ctx, PROF_BATCH, strm(st));
+    HIPCHK(hipGraphLaunch(b->exec, strm(st)));
+    return MPRES_OK;
+}
+
+int mpres_batch_info(const mpres_batch_t *b, double *fmas, int *items, mpres_batch_tuning_t *used) {
+    if (!b) return fail(MPRES_ERR_ARG, "null plan");
+    if (fmas) *fmas = b->fmas;
+    if (items) *items = b->count;
+    if (used) *used = b->used;
+    return MPRES_OK;
+}
+
+int mpres_batch_destroy(mpres_batch_t *b) {
+    if (!b) return MPRES_OK;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipDeviceSynchronize();
+    batch_free(b);
     return MPRES_OK;
 }
 

@@ -11,6 +11,7 @@
 #if defined(__HIPCC__) && defined(MPH_KERNELS)
 #include "kernels.hpp"
 #include "kernels_mfma.hpp"
+#include "kernels_batch.hpp"
 
 namespace mph {
 
@@ -303,6 +304,14 @@ struct Launch {
                                AoSRef<N>{y}, tpr);
         return hipGetLastError();
     }
+    // ---- batched form (kernels_batch.hpp): clear the queue counter, then the persistent kernel ----
+    static hipError_t batch(const DevConsts *c, const BatchItem *items, int count, int *queue, char *ws, int grid,
+                            hipStream_t st) {
+        if (count <= 0) return hipSuccess;
+        hipLaunchKernelGGL((k_fill_int<N, L>), dim3(1), dim3(BLK), 0, st, queue, 1, 0);
+        hipLaunchKernelGGL((k_batch<N, L>), dim3(grid), dim3(BLK), 0, st, c, items, count, queue, ws);
+        return hipGetLastError();
+    }
     static Ops table() {
         Ops o;
         o.N = N;
@@ -325,6 +334,7 @@ struct Launch {
         o.gemm_mfma_ws = &gemm_mfma_ws;
         o.gemm_v2 = &gemm_v2;
         o.spmv = &spmv;
+        o.batch = &batch;
         return o;
     }
 };

@@ -23,6 +23,19 @@ struct SoA {
     int len;          // filled from *lenp inside the kernels: eval upper bounds start at ev + len
 };
 
+// One unit of work of the batched form (mpres_batch_run), built on the host and
+// read by the persistent kernel: a whole dot, or a run of output rows of a gemv.
+enum BatchKind { BATCH_DOT = 0, BATCH_GEMV_N = 1, BATCH_GEMV_T = 2 };
+struct BatchItem {
+    int kind;
+    int r0, r1;          // gemv: output rows [r0, r1) of this item
+    int m, n;            // dot: n; gemv: A is m x n
+    int lda, incx, incy;
+    int g3, b3;          // dot: grid3 x block3; gemv: block3 (the summation order)
+    long long ws;        // byte offset of the item's scratch in the batch workspace
+    SoA alpha, A, x, beta, y, r;  // dot reads x, y and writes r; gemv reads alpha, A, x, beta and updates y
+};
+
 struct Ops {
     int N, L;
     hipError_t (*set_d)(const DevConsts *, const double *, int, SoA, char *, bool, hipStream_t);
@@ -51,6 +64,8 @@ struct Ops {
     size_t (*gemm_mfma_ws)(int, int, int, int);
     hipError_t (*spmv)(const DevConsts *, int, int, const int *, const int *, const double *, const char *, char *,
                        int, hipStream_t);
+    // batched form: reset the work queue, then the persistent kernel over `count` items on `grid` workgroups
+    hipError_t (*batch)(const DevConsts *, const BatchItem *, int, int *, char *, int, hipStream_t);
 };
 
 // Compiled (moduli count, lanes per number) variants; each has its own

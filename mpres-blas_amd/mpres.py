@@ -36,6 +36,33 @@ class Csr(C.Structure):  # csr_t, src/types.cuh:109-113
     _fields_ = [("as_", C.c_void_p), ("ja", C.c_void_p), ("irp", C.c_void_p)]
 
 
+BATCH_DOT, BATCH_GEMV = 0, 1
+
+
+class BatchProblem(C.Structure):  # mpres_batch_problem_t
+    _fields_ = [("op", C.c_int), ("trans", C.c_int), ("m", C.c_int), ("n", C.c_int), ("lda", C.c_int),
+                ("incx", C.c_int), ("incy", C.c_int), ("grid3", C.c_int), ("block3", C.c_int),
+                ("alpha", C.c_void_p), ("A", C.c_void_p), ("x", C.c_void_p), ("beta", C.c_void_p),
+                ("y", C.c_void_p), ("r", C.c_void_p)]
+
+
+class BatchTuning(C.Structure):  # mpres_batch_tuning_t
+    _fields_ = [("lanes", C.c_int), ("wg_per_cu", C.c_int), ("gemv_rows", C.c_int)]
+
+
+def dot_problem(n, x, y, r, grid3, block3, incx=1, incy=1):
+    """batch entry standing for mpres_dot(n, x, incx, y, incy, r, grid3, block3); x, y, r: MpArray"""
+    return BatchProblem(op=BATCH_DOT, n=n, incx=incx, incy=incy, grid3=grid3, block3=block3,
+                        x=C.addressof(x), y=C.addressof(y), r=C.addressof(r))
+
+
+def gemv_problem(trans, m, n, alpha, A, lda, x, beta, y, block3, incx=1, incy=1):
+    """batch entry standing for mpres_gemv(trans, m, n, alpha, A, lda, x, incx, beta, y, incy, block3)"""
+    return BatchProblem(op=BATCH_GEMV, trans=trans, m=m, n=n, lda=lda, incx=incx, incy=incy, block3=block3,
+                        alpha=C.addressof(alpha), A=C.addressof(A), x=C.addressof(x), beta=C.addressof(beta),
+                        y=C.addressof(y))
+
+
 class MpresError(RuntimeError):
     pass
 
@@ -89,6 +116,12 @@ def lib():
         L.mpres_gemm_v2.argtypes = [vp, i32, i32, i32, i32, i32, vp, vp, i32, vp, i32, vp, vp, i32, vp, sz, vp]
         L.mpres_spmv_csr.argtypes = [vp, i32, vp, vp, vp, vp]
         L.mpres_spmv_csrv.argtypes = [vp, i32, vp, i32, vp, vp, vp]
+        L.mpres_batch_workspace.restype = sz
+        L.mpres_batch_workspace.argtypes = [vp, vp, i32, vp]
+        L.mpres_batch_plan.argtypes = [vp, vp, i32, vp, vp, sz, C.POINTER(vp)]
+        L.mpres_batch_run.argtypes = [vp, vp]
+        L.mpres_batch_info.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(i32), vp]
+        L.mpres_batch_destroy.argtypes = [vp]
         L.mpres_profile.argtypes = [vp, i32]
         L.mpres_profile_read.argtypes = [vp, i32, C.POINTER(C.c_double), C.POINTER(C.c_long), i32]
         L.mpres_lanes_supported.argtypes = [vp, i32]
@@ -202,6 +235,39 @@ class Timer:
         t = C.c_float()
         hipchk(hip().hipEventElapsedTime(C.byref(t), self.a, self.b))
         return t.value
+
+
+class Batch:
+    """A batch plan (mpres_batch_plan) of BatchProblem entries; run() launches its graph.
+    tuning: dict with keys of BatchTuning (lanes, wg_per_cu, gemv_rows), or None."""
+
+    def __init__(self, ctx, problems, tuning=None):
+        self.L = ctx.L
+        arr = (BatchProblem * max(1, len(problems)))(*problems)
+        t = C.byref(BatchTuning(**tuning)) if tuning else None
+        h = C.c_void_p()
+        check(self.L.mpres_batch_plan(ctx.h, arr, len(problems), t, None, 0, C.byref(h)))
+        self.h = h
+
+    def run(self, stream=None):
+        check(self.L.mpres_batch_run(self.h, stream))
+
+    def info(self):
+        f, n, u = C.c_double(), C.c_int(), BatchTuning()
+        check(self.L.mpres_batch_info(self.h, C.byref(f), C.byref(n), C.byref(u)))
+        return {"mp_fmas": f.value, "items": n.value,
+                "tuning": {"lanes": u.lanes, "wg_per_cu": u.wg_per_cu, "gemv_rows": u.gemv_rows}}
+
+    def close(self):
+        if self.h:
+            self.L.mpres_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Context:

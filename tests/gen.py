@@ -68,3 +68,43 @@ def read_mtx(path):
     ja = np.array([j for _, j, _ in ents], np.int32)
     as_ = np.array([v for _, _, v in ents], np.float64)
     return m, n, irp, ja, as_
+
+
+def batch_workload(ctx, mp, count=4096, seed=500, elo=-20, ehi=20, dot_order=(1, 64), gemv_block3=8):
+    """The batching configuration on the device: problem i even is a dot of integer
+    length log-uniform in [64, 8192], i odd a square gemv y = alpha*A*x + beta*y of
+    size log-uniform in [32, 1024]; values u*2^e with e uniform in [elo, ehi]; one
+    alpha and one beta shared by all gemvs. Order arguments: dots grid3 x block3 =
+    dot_order, gemvs block3 = gemv_block3. Returns the batch entries (mpres
+    BatchProblem), the arrays they use, the single calls they stand for, the
+    multiply-accumulate count and the algorithmic bytes (every operand read once,
+    every result written once)."""
+    rng = np.random.default_rng(seed)
+    S = ctx.size
+    keep, problems, singles = [], [], []
+    fmas = nbytes = 0.0
+    alpha = ctx.array_set_d(uexp(seed + 1, 1, elo, ehi))
+    beta = ctx.array_set_d(uexp(seed + 2, 1, elo, ehi))
+    keep += [alpha, beta]
+    for i in range(count):
+        s = int(seed + 10 + 3 * i)
+        if i % 2 == 0:
+            n = int(np.rint(np.exp(rng.uniform(np.log(64), np.log(8192)))))
+            x, y, r = ctx.array_set_d(uexp(s, n, elo, ehi)), ctx.array_set_d(uexp(s + 1, n, elo, ehi)), ctx.array(1)
+            keep += [x, y, r]
+            problems.append(mp.dot_problem(n, x, y, r, *dot_order))
+            singles.append(("dot", (n, x, 1, y, 1, r, dot_order[0], dot_order[1])))
+            fmas += n
+            nbytes += (2 * n + 1) * S
+        else:
+            m = int(np.rint(np.exp(rng.uniform(np.log(32), np.log(1024)))))
+            A = ctx.array_set_d(uexp(s, m * m, elo, ehi))
+            x, y = ctx.array_set_d(uexp(s + 1, m, elo, ehi)), ctx.array_set_d(uexp(s + 2, m, elo, ehi))
+            keep += [A, x, y]
+            problems.append(mp.gemv_problem(mp.NO_TRANS, m, m, alpha, A, m, x, beta, y, gemv_block3))
+            singles.append(("gemv", (mp.NO_TRANS, m, m, alpha, A, m, x, 1, beta, y, 1, gemv_block3)))
+            fmas += m * m
+            nbytes += (m * m + 3 * m) * S
+    name = (f"{count} problems ({ctx.set_name}): dots n~logU[64,8192] order {dot_order[0]}x{dot_order[1]}, "
+            f"square gemvs m~logU[32,1024] block3 {gemv_block3}, e~U[{elo},{ehi}]")
+    return {"problems": problems, "keep": keep, "singles": singles, "fmas": fmas, "bytes": nbytes, "name": name}

@@ -1,8 +1,11 @@
-// Host stand-in for <hip/hip_runtime.h> used ONLY by tests/host/kernels_emu.cpp,
-// which compiles this library's own kernel templates with g++ and runs them
-// sequentially on the CPU, one lane per number (L = 1). With L = 1 a group is
-// a single lane, so every cross-lane exchange is the identity; kernels that
-// synchronise a workgroup (LDS trees) are not emulated.
+// Host stand-in for <hip/hip_runtime.h> used ONLY by the host tests
+// (tests/host/kernels_emu.cpp, tests/host/batch_emu.cpp), which compile this
+// library's own kernel templates with g++ and run them on the CPU, one lane per
+// number (L = 1). With L = 1 a group is a single lane, so every cross-lane
+// exchange is the identity. kernels_emu.cpp runs the threads of a workgroup one
+// after another (kernels that synchronise a workgroup are not run there);
+// batch_emu.cpp runs the threads of one workgroup as concurrent host threads and
+// sets emu_barrier to a barrier wait, so __syncthreads synchronises them.
 #pragma once
 
 #include <cmath>
@@ -28,7 +31,8 @@ struct int4 {
 struct EmuIdx {
     unsigned x, y, z;
 };
-extern EmuIdx blockIdx, threadIdx;
+inline thread_local EmuIdx blockIdx, threadIdx;
+inline void (*emu_barrier)() = nullptr;
 
 inline unsigned __lane_id() { return threadIdx.x & 63u; }
 template <class T>
@@ -40,5 +44,9 @@ inline int atomicMin(int *p, int v) {
     if (v < o) *p = v;
     return o;
 }
-inline void __syncthreads() {}
-inline void __threadfence_block() {}
+inline int atomicAdd(int *p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline void __syncthreads() {
+    if (emu_barrier) emu_barrier();
+}
+inline void __threadfence_block() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
+#define __builtin_amdgcn_readfirstlane(v) (v)

@@ -10,8 +10,6 @@
 #include "../../mpres-blas_amd/csrc/internal.hpp"
 #include "../../mpres-blas_amd/csrc/kernels.hpp"
 
-EmuIdx blockIdx, threadIdx;
-
 using namespace mph;
 
 namespace {

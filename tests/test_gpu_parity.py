@@ -478,3 +478,84 @@ def test_gemm_mfma(name, lo, hi, ta, tb):
                 worst = max(worst, float(abs(got - ex) / abs(ex)))
     if name != "8_2double" or hi <= 8:
         assert worst <= 2.0 ** (-prec + 2), worst
+
+
+def _batch_case(c, o, name, seed0):
+    """a mixed batch on the device: dots and gemvs of several shapes, strides and orders,
+    plus a skipped dot; returns (problems, device arrays, [(kind, array, count, expected, reset)])"""
+    lo, hi = (-20, 20) if name.startswith("8_") else (-60, 60)
+    seed = [seed0]
+    keep = []
+
+    def rec(cnt):
+        seed[0] += 1
+        return o.set_d(gen.uexp(seed[0], cnt, lo, hi), D)
+
+    def dev(recs):
+        a = c.array_from_host(recs)
+        keep.append(a)
+        return a
+
+    probs, checks = [], []
+    for n, g3, b3, incx, incy in [(1, 1, 1, 1, 1), (777, 2, 3, 1, 1), (3000, 4, 64, 1, 1), (257, 3, 5, -1, 2),
+                                  (4100, 1, 32, 2, -1), (64, 8, 8, 1, 1), (8192, 256, 64, 1, 1)]:
+        xr, yr = rec(1 + (n - 1) * abs(incx)), rec(1 + (n - 1) * abs(incy))
+        x, y, r = dev(xr), dev(yr), c.array(1)
+        probs.append(mp.dot_problem(n, x, y, r, g3, b3, incx, incy))
+        checks.append(("dot", r, 1, o.dot(xr, yr, g3, b3, D, incx=incx, incy=incy, n=n), None))
+    for m, n, trans, b3, incx, incy in [(5, 7, 0, 1, 1, 1), (33, 20, 0, 3, 1, 1), (33, 20, 1, 8, 1, 1),
+                                        (130, 90, 0, 2, -1, 2), (40, 100, 1, 1, 2, -1), (1, 1, 0, 4, 1, 1),
+                                        (300, 257, 0, 32, 1, 1)]:
+        lda = m + 3
+        lx, ly = (m, n) if trans else (n, m)
+        Ar, xr, yr, ar, br = rec(lda * n), rec(1 + (lx - 1) * abs(incx)), rec(1 + (ly - 1) * abs(incy)), rec(1), rec(1)
+        A, x, y, al, be = dev(Ar), dev(xr), dev(yr), dev(ar), dev(br)
+        probs.append(mp.gemv_problem(mp.TRANS if trans else mp.NO_TRANS, m, n, al, A, lda, x, be, y, b3, incx, incy))
+        checks.append(("gemv", y, len(yr), o.gemv(trans, m, n, ar, Ar, lda, xr, br, yr, b3, D, incx=incx, incy=incy),
+                       yr))
+    zr = rec(1)
+    z, zout = dev(rec(3)), dev(zr)
+    probs.append(mp.dot_problem(0, z, z, zout, 2, 2))
+    checks.append(("dot", zout, 1, zr, None))
+    return probs, keep, checks
+
+
+@pytest.mark.parametrize("name", ["16_4double", "8_2double", "32_8double"])
+def test_batch(name):
+    """mpres_batch_run: every problem bit-identical to its single call (oracle, policy D), for
+    every lanes variant and several geometries; the graph replayed three times, the updated
+    gemv outputs restored between replays, every replay checked"""
+    c, o = ctx(name), orc(name)
+    probs, keep, checks = _batch_case(c, o, name, 600)
+    geoms = [dict(lanes=L, wg_per_cu=2, gemv_rows=64) for L in lanes_of(c)]
+    geoms += [dict(lanes=lanes_of(c)[-1], wg_per_cu=1, gemv_rows=16), dict(lanes=0, wg_per_cu=4, gemv_rows=1000)]
+    for g in geoms:
+        note("batch", name, g)
+        b = mp.Batch(c, probs, g)
+        info = b.info()
+        assert info["mp_fmas"] == sum(p.n if p.op == mp.BATCH_DOT else p.m * p.n for p in probs)
+        for rep in range(3):
+            for kind, arr, cnt, want, reset in checks:
+                if reset is not None:
+                    mp.check(c.L.mpres_array_host2device(c.h, C.byref(arr), reset.ctypes.data_as(C.c_void_p), cnt,
+                                                         None))
+            mp.synchronize()
+            b.run()
+            mp.synchronize()
+            for i, (kind, arr, cnt, want, reset) in enumerate(checks):
+                same(c.array_to_host(arr, cnt), want, f"batch {name} {g} replay {rep} problem {i} ({kind})")
+        b.close()
+
+
+def test_batch_errors():
+    c = ctx("16_4double")
+    x = c.array(4)
+    bad = mp.dot_problem(4, x, x, x, 0, 4)
+    with pytest.raises(mp.MpresError, match="problem 0"):
+        mp.Batch(c, [bad])
+    with pytest.raises(mp.MpresError):
+        mp.Batch(c, [mp.dot_problem(4, x, x, x, 1, 4)], dict(lanes=3, wg_per_cu=1, gemv_rows=8))
+    empty = mp.Batch(c, [])
+    assert empty.info()["items"] == 0
+    empty.run()
+    empty.close()
