@@ -125,12 +125,18 @@ def main():
     ap.add_argument("--counters", action="store_true",
                     help="for hardware-counter runs: one warm-up and one measured call of the step's gemv and gemm, "
                          "then two 256 MiB device copies (the calibration of the byte counters); no JSON line")
+    ap.add_argument("--counters-ops", action="store_true",
+                    help="for hardware-counter runs: dot, axpy, SpMV, precision-form gemm and batched form, one "
+                         "warm-up and one measured call each, then the calibration copies; no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     mp = load_mpres()
+    if args.counters_ops:
+        counter_ops_run(mp)
+        return
     ctx = mp.Context(SET)
     nmod = ctx.n
     S = ctx.size
@@ -335,6 +341,7 @@ def mfma_lines(mp, size=2048, samples=((0, 0), (1000, 7), (2047, 2047))):
             err_pl = sample_error(ctx, mp, A, B, C0, Cm, al, be, size, samples, m_)
             fmas = float(size) ** 3
             out.append({"moduli_set": name, "bits": ctx.precision, "exponents": [lo, hi], "size": size,
+                        "lanes_per_number": ctx.L.mpres_get_lanes(ctx.h, mp.K_MFMA),
                         "mfma_ms": t_mf, "mfma_mpfma_per_s": fmas / (t_mf / 1e3),
                         "plain_ms": t_pl, "plain_mpfma_per_s": fmas / (t_pl / 1e3), "speedup": t_pl / t_mf,
                         "mfma_max_rel_err": err_mf, "plain_max_rel_err": err_pl,
@@ -397,7 +404,8 @@ def plain_lines(mp):
         err = rel_err(exact_value(ctx.array_to_host(r, 1)[0], m_), exact_dot(hx, hy)) if n <= 65536 else None
         b = 2.0 * n * S
         out.append({"op": "mp_dot", "moduli_set": "8_2double", "bits": ctx.precision, "n": n,
-                    "order": "gridDim3=256, blockDim3=64", "ms": ms, "mpfma_per_s": n / (ms / 1e3),
+                    "order": "gridDim3=256, blockDim3=64", "lanes_per_number": L.mpres_get_lanes(ctx.h, mp.K_DOT),
+                    "ms": ms, "mpfma_per_s": n / (ms / 1e3),
                     "bytes": b, "gbs": b / (ms / 1e3) / 1e9, "hbm_frac": b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
                     "max_rel_err": err})
         del x, y, r
@@ -408,6 +416,7 @@ def plain_lines(mp):
     ms = time_call(mp, lambda: mp.check(L.mpres_axpy(ctx.h, n, C.byref(al), C.byref(x), 1, C.byref(y), 1, None)), 5)
     b = 3.0 * n * S
     out.append({"op": "mp_axpy", "moduli_set": "8_2double", "bits": ctx.precision, "n": n, "ms": ms,
+                "lanes_per_number": L.mpres_get_lanes(ctx.h, mp.K_AXPY),
                 "mpfma_per_s": n / (ms / 1e3), "bytes": b, "gbs": b / (ms / 1e3) / 1e9,
                 "hbm_frac": b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS})
     del x, y, al
@@ -438,6 +447,7 @@ def plain_lines(mp):
         err = max(rel_err(exact_value(yh[i], m_), want[i]) for i in rows)
         out.append({"op": "mp_spmv_csr" if tpr is None else "mp_spmv_csrv", "threads_per_row": tpr,
                     "moduli_set": "16_4double", "bits": ctx.precision, "rows": m, "nnz": nnz, "ms": ms,
+                    "lanes_per_number": L.mpres_get_lanes(ctx.h, mp.K_SPMV),
                     "mpfma_per_s": nnz / (ms / 1e3), "bytes": b, "gbs": b / (ms / 1e3) / 1e9,
                     "hbm_frac": b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, "max_rel_err_sampled_rows": err})
     ctx.close()
@@ -584,6 +594,59 @@ def counter_run(ctx, mp, A, x, y, alpha, beta, Ag, Bg, Cg, alpha_g, beta_g, ws, 
             raise RuntimeError("device copy failed")
     mp.synchronize()
     print("[bench] counter run done", file=sys.stderr, flush=True)
+
+
+def counter_ops_run(mp):
+    """for hardware-counter runs of the other operations: one warm-up and one measured call of
+    mp_dot and mp_axpy (n = 2^24, 8 moduli), the CSR and CSR-vector SpMV (10^6 rows, 16 moduli,
+    16 threads per row), the precision-form gemm (2048^3, 8 moduli, e in [-8, 8]) and the
+    batched form (1024 problems of the batching workload), then the 256 MiB calibration copies"""
+    ctx = mp.Context("8_2double")
+    L = ctx.L
+    n = 1 << 24
+    x, y, r = ctx.array_set_d(gen.uexp(1, n, -20, 20)), ctx.array_set_d(gen.uexp(2, n, -20, 20)), ctx.array(1)
+    al = ctx.array_set_d(gen.uexp(5, 1, -20, 20))
+    for _ in range(2):
+        mp.check(L.mpres_dot(ctx.h, n, C.byref(x), 1, C.byref(y), 1, C.byref(r), 256, 64, None, 0, None))
+    for _ in range(2):
+        mp.check(L.mpres_axpy(ctx.h, n, C.byref(al), C.byref(x), 1, C.byref(y), 1, None))
+    mp.synchronize()
+    del x, y
+    size = 2048
+    A, B = ctx.array_set_d(gen.uexp(401, size * size, -8, 8)), ctx.array_set_d(gen.uexp(402, size * size, -8, 8))
+    Cm = ctx.array_set_d(gen.uexp(403, size * size, -8, 8))
+    be = ctx.array_set_d(gen.uexp(405, 1, -8, 8))
+    for _ in range(2):
+        mp.check(L.mpres_gemm_mfma(ctx.h, mp.NO_TRANS, mp.NO_TRANS, size, size, size, C.byref(al), C.byref(A), size,
+                                   C.byref(B), size, C.byref(be), C.byref(Cm), size, None, 0, None))
+    mp.synchronize()
+    ctx.close()
+    m = 1000000
+    irp, ja, as_ = gen.powerlaw_csr(7, m)
+    ctx = mp.Context("16_4double")
+    L = ctx.L
+    xs, ys = ctx.set_d(gen.uexp(8, m, -10, 10)), ctx.aos(count=m)
+    dirp, dja, das = mp.DevBuf(host=irp), mp.DevBuf(host=ja), mp.DevBuf(host=as_)
+    csr = mp.Csr(das.ptr.value, dja.ptr.value, dirp.ptr.value)
+    for _ in range(2):
+        mp.check(L.mpres_spmv_csr(ctx.h, m, C.byref(csr), xs.ptr, ys.ptr, None))
+    for _ in range(2):
+        mp.check(L.mpres_spmv_csrv(ctx.h, m, C.byref(csr), 16, xs.ptr, ys.ptr, None))
+    mp.synchronize()
+    wl = gen.batch_workload(ctx, mp, 1024)
+    b = mp.Batch(ctx, wl["problems"], dict(lanes=4, wg_per_cu=4, gemv_rows=128))
+    for _ in range(2):
+        b.run()
+    mp.synchronize()
+    b.close()
+    src, dst = mp.DevBuf(nbytes=256 << 20), mp.DevBuf(nbytes=256 << 20)
+    H = mp.hip()
+    for _ in range(2):
+        if H.hipMemcpy(dst.ptr, src.ptr, C.c_size_t(256 << 20), 3) != 0:
+            raise RuntimeError("device copy failed")
+    mp.synchronize()
+    ctx.close()
+    print("[bench] counter run (other operations) done", file=sys.stderr, flush=True)
 
 
 def ctx_moduli(ctx):

@@ -228,7 +228,8 @@ int mpres_profile_read(mpres_ctx_t *ctx, int op, double *ms, long *launches, int
 
 /* ---- tuning ----
  * Lanes per multiple-precision number used by the kernels of an operation
- * (0 = the built-in choice). op: 0 elementwise, 1 dot, 2 gemv, 3 gemm, 4 spmv, 5 axpy.
+ * (0 = the built-in choice). op: 0 elementwise, 1 dot, 2 gemv, 3 gemm, 4 spmv, 5 axpy,
+ * 7 precision-form gemm (mpres_gemm_mfma).
  * Only values compiled into the library are accepted (mpres_lanes_supported). */
 int mpres_lanes_supported(const mpres_ctx_t *ctx, int lanes);
 int mpres_set_lanes(mpres_ctx_t *ctx, int op, int lanes);

@@ -43,18 +43,23 @@ int fail(int code, const char *fmt, ...) {
         if (e_ != hipSuccess) return fail(MPRES_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
 
-enum OpKind { OP_ELEM = 0, OP_DOT = 1, OP_GEMV = 2, OP_GEMM = 3, OP_SPMV = 4, OP_AXPY = 5, OP_COUNT = 6 };
+// operation classes of mpres_set_lanes; 6 is a timing slot only (gemm residue products)
+enum OpKind { OP_ELEM = 0, OP_DOT = 1, OP_GEMV = 2, OP_GEMM = 3, OP_SPMV = 4, OP_AXPY = 5, OP_MFMA = 7, OP_COUNT = 8 };
+bool lanes_op(int op) { return op >= 0 && op < OP_COUNT && op != 6; }
 // extra timing slot: the residue products of the split-form gemm
 constexpr int PROF_GEMM_RESID = 6, PROF_GEMM_MFMA = 7, PROF_BATCH = 8, PROF_SLOTS = 9;
 
 const Ops *find_ops(int n, int L) { return find_variant(n, L); }
 
 // Built-in lanes-per-number choice for each (moduli count, operation), from
-// measurements on MI355X (dot 2^16..2^24 at 8 moduli, gemv 4096^2 and gemm
-// 512^3 at 32 moduli, SpMV 10^6 rows at 16 moduli); mpres_set_lanes overrides.
+// measurements on MI355X (dot and axpy 2^24 at 8 moduli, gemv 4096^2 and gemm
+// 512^3 at 32 moduli, SpMV 10^6 rows at 16 moduli, precision-form gemm 2048^3 at
+// 8 and 32 moduli); mpres_set_lanes overrides. Element-parallel work with
+// enough numbers prefers one lane per number (no lane repeats a number's scalar
+// logic); reductions with few independent chains prefer more lanes.
 int default_lanes(int n, int op) {
     switch (n) {
-        case 8: return 8;
+        case 8: return (op == OP_AXPY || op == OP_MFMA) ? 1 : 8;
         case 16: return op == OP_SPMV ? 16 : 4;
         default: return 8;
     }
@@ -571,7 +576,7 @@ int mpres_gemm(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const mp_arr
 
 size_t mpres_gemm_mfma_workspace(const mpres_ctx_t *c, int m, int n, int k) {
     if (!c || m <= 0 || n <= 0 || k < 0) return 0;
-    return c->ops[OP_GEMM]->gemm_mfma_ws(c->bits, m, n, k);
+    return c->ops[OP_MFMA]->gemm_mfma_ws(c->bits, m, n, k);
 }
 
 int mpres_gemm_mfma(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const mp_array_t *alpha,
@@ -587,7 +592,7 @@ int mpres_gemm_mfma(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const m
     int rc = get_ws(c, ws, wsb, mpres_gemm_mfma_workspace(c, m, n, k), &w);
     if (rc) return rc;
     ProfScope ps(c, PROF_GEMM_MFMA, strm(st));
-    HIPCHK(c->ops[OP_GEMM]->gemm_mfma(c->dev, c->bits, c->host.log2m, tra, trb, m, n, k, soa_of(alpha), soa_of(A), lda,
+    HIPCHK(c->ops[OP_MFMA]->gemm_mfma(c->dev, c->bits, c->host.log2m, tra, trb, m, n, k, soa_of(alpha), soa_of(A), lda,
                                       soa_of(B), ldb, soa_of(beta), soa_of(C), ldc, w, strm(st)));
     return MPRES_OK;
 }
@@ -804,7 +809,7 @@ int mpres_lanes_supported(const mpres_ctx_t *c, int lanes) {
 }
 
 int mpres_set_lanes(mpres_ctx_t *c, int op, int lanes) {
-    if (!c || op < 0 || op >= OP_COUNT) return fail(MPRES_ERR_ARG, "invalid operation %d", op);
+    if (!c || !lanes_op(op)) return fail(MPRES_ERR_ARG, "invalid operation %d", op);
     const Ops *o = find_ops(c->host.n, lanes == 0 ? default_lanes(c->host.n, op) : lanes);
     if (!o) return fail(MPRES_ERR_UNSUPPORTED, "%d lanes per number not compiled for %d moduli", lanes, c->host.n);
     c->ops[op] = o;
@@ -812,7 +817,7 @@ int mpres_set_lanes(mpres_ctx_t *c, int op, int lanes) {
 }
 
 int mpres_get_lanes(const mpres_ctx_t *c, int op) {
-    if (!c || op < 0 || op >= OP_COUNT) return fail(MPRES_ERR_ARG, "invalid operation %d", op);
+    if (!c || !lanes_op(op)) return fail(MPRES_ERR_ARG, "invalid operation %d", op);
     return c->ops[op]->L;
 }
 

@@ -17,7 +17,7 @@ MPRES_OK = 0
 NO_TRANS, TRANS, CONJ_TRANS = 111, 112, 113
 OP_ADD, OP_SUB, OP_MUL, OP_MUL_D, OP_VADD = 0, 1, 2, 3, 4
 # operation classes of mpres_set_lanes
-K_ELEM, K_DOT, K_GEMV, K_GEMM, K_SPMV, K_AXPY = 0, 1, 2, 3, 4, 5
+K_ELEM, K_DOT, K_GEMV, K_GEMM, K_SPMV, K_AXPY, K_MFMA = 0, 1, 2, 3, 4, 5, 7
 
 ER_DTYPE = np.dtype([("frac", "<f8"), ("exp", "<i8")])
 

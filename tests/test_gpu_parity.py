@@ -457,7 +457,7 @@ def test_gemm_mfma(name, lo, hi, ta, tb):
     alpha, beta = o.set_d([1.25], D), o.set_d([-0.5], D)
     want, sums = mfma_reference(o, _log2m(c), ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, Cm, m)
     for L in lanes_of(c):
-        c.set_lanes(mp.K_GEMM, L)
+        c.set_lanes(mp.K_MFMA, L)
         note("gemm_mfma", name, lo, hi, ta, tb, L)
         aA, aB, aC = c.array_from_host(A), c.array_from_host(B), c.array_from_host(Cm)
         aa, ab = c.array_from_host(alpha), c.array_from_host(beta)
