@@ -28,6 +28,7 @@ typedef int mf_v16i __attribute__((ext_vector_type(16)));
 
 constexpr int MF_T = 32;     // MFMA tile (rows, columns and k per instruction)
 constexpr int MF_LB = 7;     // bits per limb (limbs are 0..127, valid int8)
+constexpr int MF_NONE = -0x40000000;  // no element (bit bounds of empty rows, columns, bands)
 
 // bit length bound of an element from its interval upper bound: X < 2^(exp+1) M < 2^(exp+2+log2m)
 template <int N, class VM>
@@ -35,72 +36,103 @@ __device__ __forceinline__ int mf_bits(const VM &X, long long i, int log2m) {
     int s, e;
     Er lo, up;
     scalars_of<N>(X, i, s, e, lo, up);
-    return up.frac != 0 ? up.exp + 2 + log2m : -0x40000000;
+    return up.frac != 0 ? up.exp + 2 + log2m : MF_NONE;
 }
 
-// maximum aligned bit length of each row of op(A) (SIDE 0) or column of op(B) (SIDE 1):
-// max over i of bits(x) + e(x) - emin[j], combined by atomic maximum into out
-template <int N, int L, class VM, int SIDE>
-__global__ __launch_bounds__(BLK) void k_mf_bits(const DevConsts *__restrict__ C, int cnt, int k, int t, VM X, int ld,
-                                                 const int *__restrict__ emin, int *out) {
+// Highest bit position (bits + exponent) of each row of op(A) (SIDE 0) or column of op(B)
+// (SIDE 1), over its nonzero elements (LOW: only those with exponent below eh[j]),
+// combined by atomic maximum into out (filled with MF_NONE first).
+template <int N, int L, class VM, int SIDE, bool LOW>
+__global__ __launch_bounds__(BLK) void k_mf_top(const DevConsts *__restrict__ C, int cnt, int k, int t, VM X, int ld,
+                                                const int *__restrict__ eh, int *out) {
     if constexpr (std::is_same<VM, SoA>::value) X = rd(X);
     constexpr int S = BLK / EMIN_J;
     const long long j = (long long)blockIdx.x * EMIN_J + threadIdx.x % EMIN_J;
     const int i0 = blockIdx.y * EMIN_K, i1 = (i0 + EMIN_K) < k ? i0 + EMIN_K : k;
     if (j >= cnt) return;
-    int b = -0x40000000;
+    int b = MF_NONE;
     for (int i = i0 + (int)threadIdx.x / EMIN_J; i < i1; i += S) {
         const long long idx = op_index<SIDE>(t, (int)j, i, ld);
-        const int v = mf_bits<N>(X, idx, C->log2m) + exp_of<N>(X, idx) - emin[j];
+        const int bits = mf_bits<N>(X, idx, C->log2m);
+        if (bits == MF_NONE) continue;
+        const int e = exp_of<N>(X, idx);
+        if (LOW && e >= eh[j]) continue;
+        const int v = bits + e;
         b = v > b ? v : b;
     }
-    if (b != -0x40000000) atomicMax(out + j, b);
+    if (b != MF_NONE) atomicMax(out + j, b);
 }
 
-// shift of each row / column: s = max(0, bits - half); emin becomes the effective
-// exponent E + s (0 for a row or column of zeros), bits the aligned bound min(bits, half)
+// Exponent bands of each row / column. With half the bit budget of one side:
+//   high band: exponent eh = max(emin, top - half), aligned bits bh = top - eh <= half; it
+//              holds the elements with exponent >= eh (exactly: no truncation);
+//   low band:  the elements with exponent < eh; exponent el = emin + max(0, topl - emin - half),
+//              aligned bits bl = min(topl - emin, half); elements below el are truncated.
+// Both are e = emin + max(0, t - emin - half), b = min(t - emin, half) for t = top (high band)
+// or t = topl (low band). No element (t = MF_NONE, or a row of zeros): e = 0, b = MF_NONE.
 template <int N, int L>
-__global__ __launch_bounds__(BLK) void k_mf_shift(int cnt, int half, int *emin_eff, int *bits) {
+__global__ __launch_bounds__(BLK) void k_mf_band(int cnt, int half, const int *__restrict__ emin,
+                                                 const int *__restrict__ top, int *e_out, int *b_out) {
     const long long j = (long long)blockIdx.x * BLK + threadIdx.x;
     if (j >= cnt) return;
-    const int b = bits[j];
-    const int s = b > half ? b - half : 0;
-    emin_eff[j] = emin_eff[j] == 0x7fffffff ? 0 : emin_eff[j] + s;
-    bits[j] = b < half ? b : half;
+    const int lo = emin[j], tp = top[j];
+    if (lo == 0x7fffffff || tp == MF_NONE) {
+        e_out[j] = 0;
+        b_out[j] = MF_NONE;
+        return;
+    }
+    const int span = tp - lo;
+    e_out[j] = lo + (span > half ? span - half : 0);
+    b_out[j] = span < half ? span : half;
 }
 
 // aligned, signed residues of op(A) rows (SIDE 0) or op(B) columns (SIDE 1) as int8 limb
-// planes P[r][limb][j][kp] (kp = 0 .. KP-1, zero padded; j = 0 .. JP-1, zero padded)
+// planes P[band][r][limb][j][kp] (band 1 high, 0 low; kp = 0 .. KP-1 and j = 0 .. JP-1 zero
+// padded); an element goes to one band, the other band's planes hold 0 there
 template <int N, int L, class VM, int SIDE>
 __global__ __launch_bounds__(BLK) void k_mf_prescale(const DevConsts *__restrict__ C, int cnt, int JP, int k, int KP,
-                                                     int nl, int t, VM X, int ld, const int *__restrict__ eff,
-                                                     signed char *__restrict__ P) {
+                                                     int nl, int t, VM X, int ld, const int *__restrict__ eh,
+                                                     const int *__restrict__ el, signed char *__restrict__ P) {
     if constexpr (std::is_same<VM, SoA>::value) X = rd(X);
     const long long g = (long long)blockIdx.x * BLK + threadIdx.x;
     if (g >= (long long)JP * KP) return;
     const int kp = (int)(g % KP), j = (int)(g / KP);
-    const size_t plane = (size_t)JP * KP;
+    const size_t plane = (size_t)JP * KP, band = (size_t)N * nl * plane;
+    signed char *PL = P, *PH = P + band;
     if (j >= cnt || kp >= k) {
         for (int r = 0; r < N; r++)
-            for (int a = 0; a < nl; a++) P[((size_t)r * nl + a) * plane + g] = 0;
+            for (int a = 0; a < nl; a++) {
+                PL[((size_t)r * nl + a) * plane + g] = 0;
+                PH[((size_t)r * nl + a) * plane + g] = 0;
+            }
         return;
     }
     const long long idx = op_index<SIDE>(t, j, kp, ld);
-    const int d = exp_of<N>(X, idx) - eff[j];
-    const int s = sign_at<N>(X, idx);
+    int s, e;
+    Er lo, up;
+    scalars_of<N>(X, idx, s, e, lo, up);
+    const bool high = e >= eh[j];
+    signed char *Pw = high ? PH : PL, *Pz = high ? PL : PH;
+    const int d = e - (high ? eh[j] : el[j]);
     const int *dig = digits_of<N>(X, idx);
     int x[N];
     for (int r = 0; r < N; r++) x[r] = dig[r];
-    if (d < 0) {
-        // truncation of the row's alignment: floor(X / 2^-d)
+    if (up.frac == 0 || -d > C->log2m + 1) {
+        // zero, or truncated to zero (X < M < 2^(log2m + 1))
+        for (int r = 0; r < N; r++) x[r] = 0;
+    } else if (d < 0) {
+        // truncation of the low band's alignment: floor(X / 2^-d)
         Core<N, 1> core{C, C->pow2, 0};
         rns_scale2pow<N, 1, false>(core, x, (unsigned)(-d));
     }
     for (int r = 0; r < N; r++) {
         const int m = C->m[r];
-        int v = d > 0 ? mulmod(x[r], pow2mod(C, d, r), C->md[r], C->minv[r]) : x[r];
+        int v = (d > 0 && x[r]) ? mulmod(x[r], pow2mod(C, d, r), C->md[r], C->minv[r]) : x[r];
         v = (s && v) ? m - v : v;
-        for (int a = 0; a < nl; a++) P[((size_t)r * nl + a) * plane + g] = (signed char)((v >> (MF_LB * a)) & 127);
+        for (int a = 0; a < nl; a++) {
+            Pw[((size_t)r * nl + a) * plane + g] = (signed char)((v >> (MF_LB * a)) & 127);
+            Pz[((size_t)r * nl + a) * plane + g] = 0;
+        }
     }
 }
 
@@ -108,11 +140,27 @@ __global__ __launch_bounds__(BLK) void k_mf_prescale(const DevConsts *__restrict
 // (each wave a 32 x 32 tile); blockIdx.z = modulus. Operands are swapped
 // (B^T as the first, A^T as the second) so that a lane holds one row of C.
 // Output: S[r][e], e = row + m * col, the residue of the exact sum mod m_r.
+// fa / fb (low-band products only): the bit bounds of the rows / columns' low bands; a
+// workgroup whose 64 rows (or columns) all lack a low band has nothing to add, and the
+// finish does not read its entries, so it returns at once.
 template <int N, int L, int NL>
 __global__ __launch_bounds__(BLK) void k_mf_gemm(const DevConsts *__restrict__ C, int m, int n, int MP, int NP, int KP,
                                                  const signed char *__restrict__ PA, const signed char *__restrict__ PB,
+                                                 const int *__restrict__ fa, const int *__restrict__ fb,
                                                  int *__restrict__ S) {
     constexpr int NG = 2 * NL - 1;  // limb shift groups
+    if (fa || fb) {
+        __shared__ int live[2];
+        if (threadIdx.x < 2) live[threadIdx.x] = 0;
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            const int row = blockIdx.y * 64 + threadIdx.x, col = blockIdx.x * 64 + threadIdx.x;
+            if (!fa || (row < m && fa[row] != MF_NONE)) live[0] = 1;
+            if (!fb || (col < n && fb[col] != MF_NONE)) live[1] = 1;
+        }
+        __syncthreads();
+        if (!(live[0] && live[1])) return;
+    }
     const int r = blockIdx.z;
     const int w = threadIdx.x / 64, l = threadIdx.x % 64;
     const int row0 = blockIdx.y * 64 + 32 * (w & 1), col0 = blockIdx.x * 64 + 32 * (w >> 1);
@@ -149,54 +197,86 @@ __global__ __launch_bounds__(BLK) void k_mf_gemm(const DevConsts *__restrict__ C
     }
 }
 
-// per entry: the exact sum from its residues, rounded once, then C = (sum alpha) + (C beta)
-// The interval evaluation of the sum is the reference's full evaluation of the
-// residues of |sum| 2^K0, exponents lowered by K0: with the bit bounds of the
-// entry's row and column, |sum| 2^K0 < M/2, so the evaluation starts close to
-// the top of the range instead of refining up from |sum| / M.
+// Band pair p = 2 bA + bB (band 1 high, 0 low) of an entry: the exact sum S_p of the
+// products of the row's band bA with the column's band bB, as an mp number rounded once.
+// The interval evaluation is the reference's full evaluation of the residues of
+// |S_p| 2^K0, exponents lowered by K0: with the bit bounds ba, bb of the two bands,
+// |S_p| 2^K0 < M/2, so the evaluation starts close to the top of the range instead of
+// refining up from |S_p| / M. Values above M/2 are negative sums (|S_p| < M/4).
+template <int N, int L>
+__device__ __forceinline__ void mf_pair_value(const Grp<N, L> &g, MP<N / L> &v, const int *__restrict__ Sp,
+                                              long long e, long long mn, int ea, int eb, int ba, int bb, int lk) {
+    constexpr int R = N / L;
+    const DevConsts *C = g.k.C;
+    const int log2m = C->log2m;
+    ba = ba == MF_NONE ? 0 : ba;
+    bb = bb == MF_NONE ? 0 : bb;
+    int k0 = log2m - 1 - (ba + bb + lk);
+    k0 = k0 < 0 ? 0 : (k0 > log2m - 1 ? log2m - 1 : k0);
+    const int *p2 = C->pow2 + (size_t)k0 * N;
+    int sc[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        v.d[q] = Sp[(size_t)g.ix(q) * mn + e];
+        sc[q] = mulmod(v.d[q], p2[g.ix(q)], g.md(q), g.mv(q));
+    }
+    v.exp = ea + eb;
+    Er lo, up;
+    g.eval_full(sc, lo, up);
+    v.sign = 0;
+    if (up.frac != 0 && up.exp >= -1) {
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            v.d[q] = v.d[q] ? g.mi(q) - v.d[q] : 0;
+            sc[q] = sc[q] ? g.mi(q) - sc[q] : 0;
+        }
+        v.sign = 1;
+        g.eval_full(sc, lo, up);
+    }
+    if (lo.frac != 0) lo.exp -= k0;
+    if (up.frac != 0) up.exp -= k0;
+    v.ev0 = lo;
+    v.ev1 = up;
+    g.round_if_needed(v);
+}
+
+// Per entry: the band pairs present (both bands non-empty; the high pair always) in the
+// order low x low, low x high, high x low, high x high, each rounded once and added
+// with mp_add; then C = (sum alpha) + (C beta) with mp_mul / mp_add. With one band per
+// row and column (no low bands) this is the single exact sum rounded once.
+// Band arrays: E* exponents, B* bit bounds, *H high and *L low band, *A rows, *B columns.
 template <int N, int L>
 __global__ __launch_bounds__(BLK) void k_mf_finish(const DevConsts *__restrict__ C_, int m, int n, int lk,
-                                                   const int *__restrict__ S, const int *__restrict__ EA,
-                                                   const int *__restrict__ EB, const int *__restrict__ BA,
-                                                   const int *__restrict__ BB, SoA alpha, SoA beta, SoA Cm, int ldc) {
+                                                   const int *__restrict__ S, const int *__restrict__ EHA,
+                                                   const int *__restrict__ BHA, const int *__restrict__ ELA,
+                                                   const int *__restrict__ BLA, const int *__restrict__ EHB,
+                                                   const int *__restrict__ BHB, const int *__restrict__ ELB,
+                                                   const int *__restrict__ BLB, SoA alpha, SoA beta, SoA Cm, int ldc) {
     alpha = rd(alpha);
     beta = rd(beta);
     Cm = rd(Cm);
     constexpr int R = N / L;
     const long long e = group_id<L>();
-    if (e >= (long long)m * n) return;
+    const long long mn = (long long)m * n;
+    if (e >= mn) return;
     const int row = (int)(e % m), col = (int)(e / m);
     Grp<N, L> g(C_);
-    MP<R> sum, a, b, cv, s2, c2, res;
-    const int log2m = C_->log2m;
-    int k0 = log2m - 1 - (BA[row] + BB[col] + lk);
-    k0 = k0 < 0 ? 0 : (k0 > log2m - 1 ? log2m - 1 : k0);
-    const int *p2 = C_->pow2 + (size_t)k0 * N;
-    int sc[R];
-#pragma unroll
-    for (int q = 0; q < R; q++) {
-        sum.d[q] = S[(size_t)g.ix(q) * m * n + e];
-        sc[q] = mulmod(sum.d[q], p2[g.ix(q)], g.md(q), g.mv(q));
-    }
-    sum.exp = EA[row] + EB[col];
-    Er lo, up;
-    g.eval_full(sc, lo, up);
-    sum.sign = 0;
-    if (up.frac != 0 && up.exp >= -1) {
-        // 2^K0 X / M >= 1/2: the residues hold M - |sum| (|sum| 2^K0 < M/2)
-#pragma unroll
-        for (int q = 0; q < R; q++) {
-            sum.d[q] = sum.d[q] ? g.mi(q) - sum.d[q] : 0;
-            sc[q] = sc[q] ? g.mi(q) - sc[q] : 0;
+    MP<R> sum, v, a, b, cv, s2, c2, res;
+    const bool la = BLA[row] != MF_NONE, lb = BLB[col] != MF_NONE;
+    bool first = true;
+#pragma unroll 1
+    for (int p = 0; p < 4; p++) {
+        const int ba_ = p >> 1, bb_ = p & 1;
+        if ((!ba_ && !la) || (!bb_ && !lb)) continue;
+        mf_pair_value<N, L>(g, v, S + (size_t)p * N * mn, e, mn, ba_ ? EHA[row] : ELA[row], bb_ ? EHB[col] : ELB[col],
+                            ba_ ? BHA[row] : BLA[row], bb_ ? BHB[col] : BLB[col], lk);
+        if (first) {
+            sum = v;
+            first = false;
+        } else {
+            g.add(sum, sum, v);
         }
-        sum.sign = 1;
-        g.eval_full(sc, lo, up);
     }
-    if (lo.frac != 0) lo.exp -= k0;
-    if (up.frac != 0) up.exp -= k0;
-    sum.ev0 = lo;
-    sum.ev1 = up;
-    g.round_if_needed(sum);
     const long long ic = (long long)col * ldc + row;
     load_soa(g, a, alpha, 0);
     load_soa(g, b, beta, 0);

@@ -245,11 +245,14 @@ struct Launch {
     // ---- precision form on INT8 MFMA (kernels_mfma.hpp) ----
     static constexpr long long rup(long long v, long long q) { return (v + q - 1) / q * q; }
     static int mf_limbs(int bits) { return (bits + MF_LB - 1) / MF_LB; }
-    // ws: S (N*m*n ints), EA EB (m + n ints), bits / effective exponents (m + n ints), limb planes
+    // ws: S (4 band pairs x N*m*n ints); per row and column: minimum exponent, top bit, top
+    // bit of the low band, high / low band exponents and bit bounds (7 (m + n) ints); then the
+    // limb planes of both bands of op(A) and op(B)
+    static size_t mf_ints(int m, int n) { return 4 * (size_t)N * m * n + 7 * ((size_t)m + n); }
     static size_t gemm_mfma_ws(int bits, int m, int n, int k) {
         const int nl = mf_limbs(bits);
-        const size_t MPd = rup(m, 64), NPd = rup(n, 64), KPd = rup(k, MF_T);
-        return 4 * ((size_t)N * m * n + 2 * ((size_t)m + n)) + 256 + (size_t)N * nl * (MPd + NPd) * KPd;
+        const size_t MPd = rup(m, 64), NPd = rup(n, 64), KPd = rup(k > 0 ? k : 1, MF_T);
+        return (4 * mf_ints(m, n) + 255) / 256 * 256 + 2 * (size_t)N * nl * (MPd + NPd) * KPd;
     }
     static hipError_t gemm_mfma(const DevConsts *c, int bits, int log2m, int ta, int tb, int m, int n, int k,
                                 SoA alpha, SoA A, int lda, SoA B, int ldb, SoA beta, SoA Cm, int ldc, char *ws,
@@ -258,37 +261,53 @@ struct Launch {
         const int nl = mf_limbs(bits);
         if (nl != 4 && nl != 5) return hipErrorInvalidValue;
         const int MP = (int)rup(m, 64), NP = (int)rup(n, 64), KP = (int)rup(k > 0 ? k : 1, MF_T);
-        int *S = (int *)ws, *EA = S + (size_t)N * m * n, *EB = EA + m, *BA = EB + n, *BB = BA + m;
-        signed char *PA = (signed char *)ws + ((4 * ((size_t)N * m * n + 2 * ((size_t)m + n)) + 255) / 256 * 256);
-        signed char *PB = PA + (size_t)N * nl * MP * KP;
+        int *S = (int *)ws;
+        int *V = S + 4 * (size_t)N * m * n;  // 7 arrays of m + n: rows of op(A) first, then columns of op(B)
+        const size_t mn = (size_t)m + n;
+        int *EMIN = V, *TOP = V + mn, *TOPL = V + 2 * mn, *EH = V + 3 * mn, *BH = V + 4 * mn, *EL = V + 5 * mn,
+            *BL = V + 6 * mn;
+        signed char *PA = (signed char *)ws + (4 * mf_ints(m, n) + 255) / 256 * 256;
+        const size_t bandA = (size_t)N * nl * MP * KP, bandB = (size_t)N * nl * NP * KP;
+        signed char *PB = PA + 2 * bandA;
         const dim3 bl(BLK);
-        hipLaunchKernelGGL((k_fill_int<N, L>), dim3(grid_for((long long)m + n, 1)), bl, 0, st, EA, m + n, 0x7fffffff);
-        hipLaunchKernelGGL((k_fill_int<N, L>), dim3(grid_for((long long)m + n, 1)), bl, 0, st, BA, m + n,
-                           -0x40000000);
+        hipLaunchKernelGGL((k_fill_int<N, L>), dim3(grid_for((long long)mn, 1)), bl, 0, st, EMIN, (int)mn, 0x7fffffff);
+        hipLaunchKernelGGL((k_fill_int<N, L>), dim3(grid_for(2 * (long long)mn, 1)), bl, 0, st, TOP, (int)(2 * mn),
+                           MF_NONE);
+        const unsigned kc = (unsigned)((k + EMIN_K - 1) / EMIN_K);
+        const dim3 ga((m + EMIN_J - 1) / EMIN_J, kc), gb((n + EMIN_J - 1) / EMIN_J, kc);
         if (k > 0) {
-            const unsigned kc = (unsigned)((k + EMIN_K - 1) / EMIN_K);
-            const dim3 ga((m + EMIN_J - 1) / EMIN_J, kc), gb((n + EMIN_J - 1) / EMIN_J, kc);
-            hipLaunchKernelGGL((k_gemm_emin<N, L, SoA, 0, true>), ga, bl, 0, st, m, k, ta, A, lda, EA);
-            hipLaunchKernelGGL((k_gemm_emin<N, L, SoA, 1, true>), gb, bl, 0, st, n, k, tb, B, ldb, EB);
-            hipLaunchKernelGGL((k_mf_bits<N, L, SoA, 0>), ga, bl, 0, st, c, m, k, ta, A, lda, EA, BA);
-            hipLaunchKernelGGL((k_mf_bits<N, L, SoA, 1>), gb, bl, 0, st, c, n, k, tb, B, ldb, EB, BB);
+            hipLaunchKernelGGL((k_gemm_emin<N, L, SoA, 0, true>), ga, bl, 0, st, m, k, ta, A, lda, EMIN);
+            hipLaunchKernelGGL((k_gemm_emin<N, L, SoA, 1, true>), gb, bl, 0, st, n, k, tb, B, ldb, EMIN + m);
+            hipLaunchKernelGGL((k_mf_top<N, L, SoA, 0, false>), ga, bl, 0, st, c, m, k, ta, A, lda, nullptr, TOP);
+            hipLaunchKernelGGL((k_mf_top<N, L, SoA, 1, false>), gb, bl, 0, st, c, n, k, tb, B, ldb, nullptr, TOP + m);
         }
         int lk = 0;
         while ((1LL << lk) < (long long)(k > 0 ? k : 1)) lk++;
         const int half = (log2m - 2 - lk) / 2;
-        hipLaunchKernelGGL((k_mf_shift<N, L>), dim3(grid_for(m, 1)), bl, 0, st, m, half, EA, BA);
-        hipLaunchKernelGGL((k_mf_shift<N, L>), dim3(grid_for(n, 1)), bl, 0, st, n, half, EB, BB);
+        const dim3 gv(grid_for((long long)mn, 1));
+        hipLaunchKernelGGL((k_mf_band<N, L>), gv, bl, 0, st, (int)mn, half, EMIN, TOP, EH, BH);
+        if (k > 0) {
+            hipLaunchKernelGGL((k_mf_top<N, L, SoA, 0, true>), ga, bl, 0, st, c, m, k, ta, A, lda, EH, TOPL);
+            hipLaunchKernelGGL((k_mf_top<N, L, SoA, 1, true>), gb, bl, 0, st, c, n, k, tb, B, ldb, EH + m, TOPL + m);
+        }
+        hipLaunchKernelGGL((k_mf_band<N, L>), gv, bl, 0, st, (int)mn, half, EMIN, TOPL, EL, BL);
         hipLaunchKernelGGL((k_mf_prescale<N, L, SoA, 0>), dim3(grid_for((long long)MP * KP, 1)), bl, 0, st, c, m, MP,
-                           k, KP, nl, ta, A, lda, EA, PA);
+                           k, KP, nl, ta, A, lda, EH, EL, PA);
         hipLaunchKernelGGL((k_mf_prescale<N, L, SoA, 1>), dim3(grid_for((long long)NP * KP, 1)), bl, 0, st, c, n, NP,
-                           k, KP, nl, tb, B, ldb, EB, PB);
+                           k, KP, nl, tb, B, ldb, EH + m, EL + m, PB);
         const dim3 gg(NP / 64, MP / 64, N);
-        if (nl == 4)
-            hipLaunchKernelGGL((k_mf_gemm<N, L, 4>), gg, bl, 0, st, c, m, n, MP, NP, KP, PA, PB, S);
-        else
-            hipLaunchKernelGGL((k_mf_gemm<N, L, 5>), gg, bl, 0, st, c, m, n, MP, NP, KP, PA, PB, S);
-        hipLaunchKernelGGL((k_mf_finish<N, L>), dim3(grid_for((long long)m * n, L)), bl, 0, st, c, m, n, lk, S, EA, EB,
-                           BA, BB, alpha, beta, Cm, ldc);
+        for (int p = 0; p < 4; p++) {
+            const int ba = p >> 1, bb = p & 1;
+            const signed char *pa = PA + (size_t)ba * bandA, *pb = PB + (size_t)bb * bandB;
+            const int *fa = ba ? nullptr : BL, *fb = bb ? nullptr : BL + m;
+            int *Sp = S + (size_t)p * N * m * n;
+            if (nl == 4)
+                hipLaunchKernelGGL((k_mf_gemm<N, L, 4>), gg, bl, 0, st, c, m, n, MP, NP, KP, pa, pb, fa, fb, Sp);
+            else
+                hipLaunchKernelGGL((k_mf_gemm<N, L, 5>), gg, bl, 0, st, c, m, n, MP, NP, KP, pa, pb, fa, fb, Sp);
+        }
+        hipLaunchKernelGGL((k_mf_finish<N, L>), dim3(grid_for((long long)m * n, L)), bl, 0, st, c, m, n, lk, S, EH, BH,
+                           EL, BL, EH + m, BH + m, EL + m, BL + m, alpha, beta, Cm, ldc);
         return hipGetLastError();
     }
     static hipError_t spmv(const DevConsts *c, int pass, int m, const int *irp, const int *ja, const double *as,

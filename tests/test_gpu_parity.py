@@ -375,14 +375,19 @@ def _log2m(c):
 
 
 def mfma_reference(o, log2m, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc):
-    """The precision form restated with exact integers and the oracle's primitives:
-    per row of op(A) / column of op(B) the common exponent (minimum over nonzero
-    elements, raised by the excess over the bit budget), the aligned and truncated
-    operands, the exact signed sum, then the oracle's evaluation, rounding and
-    epilogue (mp_mul, mp_add)."""
+    """The precision form restated with exact integers and the oracle's primitives.
+    Per row of op(A) / column of op(B), over its nonzero elements: emin, the highest bit
+    top = max(eval_up.exp + 2 + log2m + exp); the high band's exponent eh = max(emin,
+    top - half) holds the elements with exponent >= eh exactly; the others form the low
+    band, exponent emin raised by its excess over the budget (truncating). Per entry and
+    band pair (low x low, low x high, high x low, high x high; pairs with an empty band
+    skipped) the exact signed sum of the aligned products, evaluated (oracle, with the
+    2^K0 scaling), rounded, added in that order (mp_add), then the epilogue (mp_mul,
+    mp_add). Returns the result and the high x high sums (rounded)."""
     mm = [int(v) for v in o.moduli()]
     lk = (max(k, 1) - 1).bit_length()
     half = (log2m - 2 - lk) // 2
+    NONE = -(1 << 30)
 
     def opa(row, i):
         return A[i + lda * row] if ta else A[row + lda * i]
@@ -390,14 +395,25 @@ def mfma_reference(o, log2m, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, Cm, l
     def opb(i, col):
         return B[col + ldb * i] if tb else B[i + ldb * col]
 
-    def common(elems):
-        """(effective exponent, aligned bit bound) of a row of op(A) or column of op(B)"""
+    def top_of(r):
+        return int(r["eval"][1]["exp"]) + 2 + log2m + int(r["exp"])
+
+    def bands(elems):
+        """[(low exponent, low bits), (high exponent, high bits)] of a row or column, and the
+        band (0 low, 1 high) of each element"""
         nz = [r for r in elems if float(r["eval"][1]["frac"]) != 0]
         if not nz:
-            return 0, -(1 << 30)
+            return [(0, NONE), (0, NONE)], [1] * len(elems)
+
+        def band(emin, t):
+            span = t - emin
+            return emin + max(0, span - half), min(span, half)
+
         emin = min(int(r["exp"]) for r in nz)
-        bits = max(int(r["eval"][1]["exp"]) + 2 + log2m + int(r["exp"]) - emin for r in nz)
-        return emin + max(0, bits - half), min(bits, half)
+        eh, bh = band(emin, max(top_of(r) for r in nz))
+        low = [r for r in nz if int(r["exp"]) < eh]
+        el, bl = band(emin, max(top_of(r) for r in low)) if low else (0, NONE)
+        return [(el, bl), (eh, bh)], [1 if int(r["exp"]) >= eh else 0 for r in elems]
 
     def aligned(r, eff):
         X = exact.crt(r["digits"], mm)
@@ -405,43 +421,65 @@ def mfma_reference(o, log2m, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, Cm, l
         v = X << d if d >= 0 else X >> (-d)
         return -v if int(r["sign"]) else v
 
-    ea, ba = zip(*[common([opa(row, i) for i in range(k)]) for row in range(m)])
-    eb, bb = zip(*[common([opb(i, col) for i in range(k)]) for col in range(n)])
-    av = [[aligned(opa(row, i), ea[row]) for i in range(k)] for row in range(m)]
-    bv = [[aligned(opb(i, col), eb[col]) for i in range(k)] for col in range(n)]
-    sums = o.empty(m * n)
-    scaled = np.zeros((m * n, len(mm)), np.int32)
-    k0s = np.zeros(m * n, np.int64)
-    for col in range(n):
-        for row in range(m):
-            S = sum(av[row][i] * bv[col][i] for i in range(k))
-            e = row + m * col
-            rec = sums[e]
-            rec["digits"] = [abs(S) % q for q in mm]
-            rec["sign"] = int(S < 0)
-            rec["exp"] = ea[row] + eb[col]
-            # evaluation of |S| 2^K0 (|S| 2^K0 < M/2), exponents lowered by K0
-            k0 = min(max(log2m - 1 - (ba[row] + bb[col] + lk), 0), log2m - 1)
-            scaled[e] = [(abs(S) << k0) % q for q in mm]
-            k0s[e] = k0
-    lo, up = o.eval(scaled, fast=False)
-    lo["exp"] = np.where(lo["frac"] != 0, lo["exp"] - k0s, lo["exp"])
-    up["exp"] = np.where(up["frac"] != 0, up["exp"] - k0s, up["exp"])
-    sums["eval"][:, 0] = lo
-    sums["eval"][:, 1] = up
-    sums = o.round(sums)
+    def side(elems_of, cnt):
+        info = [bands(elems_of(j)) for j in range(cnt)]
+        vals = []
+        for j in range(cnt):
+            (bd, which), elems = info[j], elems_of(j)
+            v = [[0] * k, [0] * k]
+            for i, r in enumerate(elems):
+                v[which[i]][i] = aligned(r, bd[which[i]][0])
+            vals.append(v)
+        return [x[0] for x in info], vals
+
+    ba_, av = side(lambda row: [opa(row, i) for i in range(k)], m)
+    bb_, bv = side(lambda col: [opb(i, col) for i in range(k)], n)
+    pairs = []
+    for p in range(4):
+        pa, pb = p >> 1, p & 1
+        sums = o.empty(m * n)
+        scaled = np.zeros((m * n, len(mm)), np.int32)
+        k0s = np.zeros(m * n, np.int64)
+        live = np.zeros(m * n, bool)
+        for col in range(n):
+            for row in range(m):
+                e = row + m * col
+                (ea, ba), (eb, bb) = ba_[row][pa], bb_[col][pb]
+                live[e] = p == 3 or ((pa == 1 or ba != NONE) and (pb == 1 or bb != NONE))
+                S = sum(av[row][pa][i] * bv[col][pb][i] for i in range(k))
+                rec = sums[e]
+                rec["digits"] = [abs(S) % q for q in mm]
+                rec["sign"] = int(S < 0)
+                rec["exp"] = ea + eb
+                # evaluation of |S| 2^K0 (|S| 2^K0 < M/2), exponents lowered by K0
+                k0 = min(max(log2m - 1 - (max(ba, 0) + max(bb, 0) + lk), 0), log2m - 1)
+                scaled[e] = [(abs(S) << k0) % q for q in mm]
+                k0s[e] = k0
+        lo, up = o.eval(scaled, fast=False)
+        lo["exp"] = np.where(lo["frac"] != 0, lo["exp"] - k0s, lo["exp"])
+        up["exp"] = np.where(up["frac"] != 0, up["exp"] - k0s, up["exp"])
+        sums["eval"][:, 0] = lo
+        sums["eval"][:, 1] = up
+        pairs.append((o.round(sums), live))
+    total = o.empty(m * n)
+    for e in range(m * n):
+        acc = None
+        for sums, live in pairs:
+            if live[e]:
+                acc = sums[e:e + 1] if acc is None else o.add(acc, sums[e:e + 1], D)
+        total[e] = acc[0]
     want = np.array(Cm, copy=True)
     for col in range(n):
         for row in range(m):
             e = row + m * col
-            s2 = o.mul(sums[e:e + 1], alpha, D)
+            s2 = o.mul(total[e:e + 1], alpha, D)
             c2 = o.mul(Cm[col * ldc + row:col * ldc + row + 1], beta, D)
             want[col * ldc + row] = o.add(s2, c2, D)[0]
-    return want, sums
+    return want, total
 
 
-@pytest.mark.parametrize("name,lo,hi", [("8_2double", -8, 8), ("8_2double", -60, 60), ("16_4double", -20, 20),
-                                        ("32_8double", -60, 60), ("16_32bit", -30, 30)])
+@pytest.mark.parametrize("name,lo,hi", [("8_2double", -8, 8), ("8_2double", -60, 60), ("8_2double", -150, 150),
+                                        ("16_4double", -20, 20), ("32_8double", -60, 60), ("16_32bit", -30, 30)])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 1)])
 def test_gemm_mfma(name, lo, hi, ta, tb):
     c, o = ctx(name), orc(name)
@@ -465,7 +503,8 @@ def test_gemm_mfma(name, lo, hi, ta, tb):
                                      C.byref(aa), C.byref(aA), lda, C.byref(aB), ldb, C.byref(ab), C.byref(aC), m,
                                      None, 0, None))
         same(c.array_to_host(aC, m * n), want, f"gemm_mfma {name} [{lo},{hi}] {ta}{tb} L={L}")
-    # accuracy of the sums against the exact product (truncation only when the budget is exceeded)
+    # accuracy of the sums against the exact product: exact band sums, rounded and added (a few
+    # roundings); e in [-150, 150] at 8 moduli exceeds two bands, so the low band truncates there
     mm = [int(v) for v in o.moduli()]
     prec = o.mp_consts()[0]
     worst = 0.0
@@ -476,8 +515,8 @@ def test_gemm_mfma(name, lo, hi, ta, tb):
             got = exact.mp_value(sums[row + m * col], mm)
             if ex != 0:
                 worst = max(worst, float(abs(got - ex) / abs(ex)))
-    if name != "8_2double" or hi <= 8:
-        assert worst <= 2.0 ** (-prec + 2), worst
+    if hi <= 60:
+        assert worst <= 2.0 ** (-prec + 3), worst
 
 
 def _batch_case(c, o, name, seed0):
