@@ -296,7 +296,7 @@ def main():
     ctx.close()
 
 
-def mfma_lines(mp, size=2048, samples=((0, 0), (1000, 7), (2047, 2047))):
+def mfma_lines(mp, size=2048, samples=((0, 0), (1000, 7), (2047, 2047), (5, 1500), (777, 333), (1500, 1999))):
     """Precision form of gemm on INT8 MFMA (mpres_gemm_mfma) beside the plain path on the
     same inputs: M = N = K = size at 8 and 32 moduli, exponents in [-8, 8] and [-60, 60].
     Errors: relative error of sampled entries of alpha*A*B + beta*C against the exact result."""
@@ -332,6 +332,12 @@ def mfma_lines(mp, size=2048, samples=((0, 0), (1000, 7), (2047, 2047))):
                 t_mf += tm.ms()
             t_mf /= reps
             err_mf = sample_error(ctx, mp, A, B, C0, Cm, al, be, size, samples, m_)
+            # the product alone: alpha = 1, beta = 0 make the epilogue exact
+            one, zero = ctx.array_set_d(np.array([1.0])), ctx.array_set_d(np.array([0.0]))
+            ctx.copy_array(Cm, C0, size * size)
+            mp.check(L.mpres_gemm_mfma(ctx.h, mp.NO_TRANS, mp.NO_TRANS, size, size, size, C.byref(one), C.byref(A),
+                                       size, C.byref(B), size, C.byref(zero), C.byref(Cm), size, None, 0, None))
+            err_prod = sample_error(ctx, mp, A, B, C0, Cm, one, zero, size, samples, m_)
             ctx.copy_array(Cm, C0, size * size)
             tm.start()
             mp.check(L.mpres_gemm(ctx.h, mp.NO_TRANS, mp.NO_TRANS, size, size, size, C.byref(al), C.byref(A), size,
@@ -344,7 +350,8 @@ def mfma_lines(mp, size=2048, samples=((0, 0), (1000, 7), (2047, 2047))):
                         "lanes_per_number": ctx.L.mpres_get_lanes(ctx.h, mp.K_MFMA),
                         "mfma_ms": t_mf, "mfma_mpfma_per_s": fmas / (t_mf / 1e3),
                         "plain_ms": t_pl, "plain_mpfma_per_s": fmas / (t_pl / 1e3), "speedup": t_pl / t_mf,
-                        "mfma_max_rel_err": err_mf, "plain_max_rel_err": err_pl,
+                        "mfma_max_rel_err": err_mf, "mfma_product_only_max_rel_err": err_prod,
+                        "plain_max_rel_err": err_pl,
                         "unit_roundoff": 2.0 ** (-ctx.precision)})
             del A, B, C0, Cm
         ctx.close()
