@@ -708,15 +708,48 @@ __global__ __launch_bounds__(BLK) void k_gemm_resid(const DevConsts *__restrict_
 
 // ---------------------------------------------------------------- gemv, split form
 // Main pass without transpose over 16 and 32 moduli. As for gemm, an exact chain's
-// residues are |sum_j s_j X_j 2^(e_j - E)| mod m, so each partial (o, t) runs the
-// interval chain on scalars (sc_fma_step) and accumulates its residues against the
-// fixed exponent E_ref = EA[o] + EB (EA[o]: minimum exponent of row o of A, EB:
-// minimum exponent of bx), then rescales once by 2^(E_ref - E). bx is pre-scaled
+// residues are |sum_j s_j X_j 2^(e_j - E)| mod m, where the sign, the exponent E and
+// the interval evaluation come from the interval chain on scalars alone. So the
+// partial (o, t) is formed by two kernels: k_gemv_chain runs the chain
+// (sc_fma_step) over A's scalar fields and writes sign, exponent and evaluation,
+// or REDO; k_gemv_split then accumulates the residues against the fixed exponent
+// E_ref = EA[o] + EB (EA[o]: minimum exponent of row o of A, EB: minimum exponent
+// of bx) and rescales once by 2^(E_ref - E). bx is pre-scaled
 // (bxp[r][j] = s x_r 2^(e_j - EB) mod m_r); per step the workgroup builds in LDS
 // the 64 multiples bxp[r][j] 2^d (d < 64), so a term costs one product mod m per
 // residue; d >= 64 takes the slow path. A's residues are staged through LDS as
-// in k_gemv_rows. Chains with a rare case are marked REDO for the fixup kernel.
+// in k_gemv_rows. Chains marked REDO are left to the fixup kernel.
 constexpr int GV_D = 64;
+// scalar chain of the partial (o, t), o fastest (one thread each, so that a
+// wavefront reads a run of one column of A's scalar fields per step)
+template <int N, int L>
+__global__ __launch_bounds__(BLK) void k_gemv_chain(const DevConsts *__restrict__ C, int m, int n, SoA A, int lda,
+                                                    AoSRef<N> bx, int b3, AoSRef<N> part) {
+    A = rd(A);
+    const long long g = (long long)blockIdx.x * BLK + threadIdx.x;
+    if (g >= (long long)m * b3) return;
+    const int o = (int)(g % m), t = (int)(g / m);
+    Sc sc{0, 0, Er{0.0, 0}, Er{0.0, 0}};
+    bool ok = true;
+    for (int j = t; j < n && ok; j += b3) {
+        int xs, xe, ys, ye;
+        Er x0, x1, y0, y1;
+        scalars_of<N>(A, o + (long long)j * lda, xs, xe, x0, x1);
+        scalars_of<N>(bx, j, ys, ye, y0, y1);
+        ok = sc_fma_step(C, sc, xs, xe, x0, x1, ys, ye, y0, y1);
+    }
+    const long long v = (long long)o * b3 + t;
+    if (!ok) {
+        *part.sign(v) = REDO;
+        return;
+    }
+    *part.sign(v) = sc.sign;
+    *part.exp(v) = sc.exp;
+    ErMem *ev = part.eval(v);
+    ev[0] = ErMem{sc.ev0.frac, (long long)sc.ev0.exp};
+    ev[1] = ErMem{sc.ev1.frac, (long long)sc.ev1.exp};
+}
+
 constexpr int GV_REDUCE = 1 << 20;  // terms between reductions of the double accumulators
 template <int N, int L>
 __global__ __launch_bounds__(BLK) void k_gemv_split(const DevConsts *__restrict__ C, int m, int n, SoA A, int lda,
@@ -731,7 +764,11 @@ __global__ __launch_bounds__(BLK) void k_gemv_split(const DevConsts *__restrict_
     const long long o0 = (long long)blockIdx.x * BLK, o = o0 + threadIdx.x;
     const int rows = (int)((m - o0) < BLK ? (m - o0) : BLK);
     const bool live = o < m;
-    bool ok = live;
+    const long long v = o * b3 + t;
+    // the chain's result (k_gemv_chain): sign and exponent, or REDO
+    const int csign = live ? *part.sign(v) : REDO;
+    const int cexp = live ? *part.exp(v) : 0;
+    const bool ok = live && csign != REDO;
     const int ea = live ? EA[o] : 0;
     // residue sums s_j x_j (bxp 2^d)_j accumulate unreduced: each term
     // w = x f - q m (exact, |w| < 2m + 1 < 2^28) is added in FP64, which stays
@@ -740,100 +777,84 @@ __global__ __launch_bounds__(BLK) void k_gemv_split(const DevConsts *__restrict_
 #pragma unroll
     for (int r = 0; r < N; r++) acc[r] = 0.0;
     int since = 0;
-    Sc sc{0, 0, Er{0.0, 0}, Er{0.0, 0}};
     // table build: thread q -> residue q % N, multiples d0 .. d0 + DB - 1
     constexpr int DB = GV_D * N / BLK;
     const int tr = threadIdx.x % N, td0 = (threadIdx.x / N) * DB;
     const double tmd = C->md[tr], tmv = C->minv[tr];
     const int tm = C->m[tr];
     const int t2 = C->pow2[(size_t)td0 * N + tr];
-    // step j's table seed and A's scalar fields are fetched during step j - b3;
-    // A's run of residues is staged at the start of step j (all loads in flight)
+    // step j's table seed, A's sign and exponent and A's run of residues are
+    // fetched during step j - b3 (the run into registers), so step j's loads are
+    // in flight while step j - b3 computes
     RunStage<N, 1> stg;
     int nb = 0, nxs = 0, nxe = 0;
-    ErMem nlo{0.0, 0}, nup{0.0, 0};
     auto fetch = [&](int j) {
         nb = bxp[(size_t)tr * n + j];
-        if (live) {
+        if (ok) {
             const long long ia = o + (long long)j * lda;
             nxs = A.s[ia];
             nxe = A.e[ia];
-            nlo = A.ev[ia];
-            nup = A.ev[ia + A.len];
         }
     };
-    if (t < n) fetch(t);
+    if (t < n) {
+        fetch(t);
+        stg.fetch(A.d + (o0 + (long long)t * lda) * N, rows);
+    }
     for (int j = t; j < n; j += b3) {
-        const int *run = A.d + (o0 + (long long)j * lda) * N;
-        stg.fetch(run, rows);
-        stg.store(sd, run, rows);
-        int v = mulmod(nb, t2, tmd, tmv);
+        stg.store(sd, A.d + (o0 + (long long)j * lda) * N, rows);
+        int tv = mulmod(nb, t2, tmd, tmv);
         for (int d = 0; d < DB; d++) {
-            tp[(td0 + d) * SP + tr] = (double)v;
-            v += v;
-            v = v >= tm ? v - tm : v;
+            tp[(td0 + d) * SP + tr] = (double)tv;
+            tv += tv;
+            tv = tv >= tm ? tv - tm : tv;
         }
         const int xs = nxs, xe = nxe;
-        const Er x0{nlo.frac, (int)nlo.exp}, x1{nup.frac, (int)nup.exp};
         __syncthreads();
-        if (j + b3 < n) fetch(j + b3);
+        if (j + b3 < n) {
+            fetch(j + b3);
+            stg.fetch(A.d + (o0 + (long long)(j + b3) * lda) * N, rows);
+        }
         if (ok) {
-            int ys, ye;
-            Er y0, y1;
-            scalars_of<N>(bx, j, ys, ye, y0, y1);
-            ok = sc_fma_step(C, sc, xs, xe, x0, x1, ys, ye, y0, y1);
-            if (ok) {
-                int d = xe - ea;
-                int *xd = sd + threadIdx.x * SP;
-                if (d >= GV_D) {
-                    // rare: fold 2^(d - d mod 64) into this thread's own staged residues
-                    const int dh = d - (d & (GV_D - 1));
-                    for (int r = 0; r < N; r++) xd[r] = mulmod(xd[r], pow2mod(C, dh, r), C->md[r], C->minv[r]);
-                    d &= GV_D - 1;
-                }
-                const double sg = xs ? -1.0 : 1.0;
-                const double *f = tp + d * SP;
-                // groups of 8 residues, fenced so that the compiler does not hoist
-                // all LDS loads (and their registers) ahead of the arithmetic
+            int d = xe - ea;
+            int *xd = sd + threadIdx.x * SP;
+            if (d >= GV_D) {
+                // rare: fold 2^(d - d mod 64) into this thread's own staged residues
+                const int dh = d - (d & (GV_D - 1));
+                for (int r = 0; r < N; r++) xd[r] = mulmod(xd[r], pow2mod(C, dh, r), C->md[r], C->minv[r]);
+                d &= GV_D - 1;
+            }
+            const double sg = xs ? -1.0 : 1.0;
+            const double *f = tp + d * SP;
+            // groups of 8 residues, fenced so that the compiler does not hoist
+            // all LDS loads (and their registers) ahead of the arithmetic
 #pragma unroll
-                for (int r0 = 0; r0 < N; r0 += 8) {
+            for (int r0 = 0; r0 < N; r0 += 8) {
 #pragma unroll
-                    for (int r = r0; r < r0 + 8; r++) {
-                        const double x = (double)xd[r], fr = f[r];
-                        const double p = x * fr, pl = fma(x, fr, -p);
-                        const double q = floor(p * C->minv[r]);
-                        acc[r] = fma(sg, fma(-q, C->md[r], p) + pl, acc[r]);
-                    }
-                    asm volatile("" ::: "memory");
+                for (int r = r0; r < r0 + 8; r++) {
+                    const double x = (double)xd[r], fr = f[r];
+                    const double p = x * fr, pl = fma(x, fr, -p);
+                    const double q = floor(p * C->minv[r]);
+                    acc[r] = fma(sg, fma(-q, C->md[r], p) + pl, acc[r]);
                 }
-                if (++since == GV_REDUCE) {
-                    since = 0;
+                asm volatile("" ::: "memory");
+            }
+            if (++since == GV_REDUCE) {
+                since = 0;
 #pragma unroll
-                    for (int r = 0; r < N; r++) acc[r] = fmod_exact(acc[r], C->md[r], C->minv[r]);
-                }
+                for (int r = 0; r < N; r++) acc[r] = fmod_exact(acc[r], C->md[r], C->minv[r]);
             }
         }
         __syncthreads();
     }
-    if (!live) return;
-    const long long v = o * b3 + t;
-    if (!ok) {
-        *part.sign(v) = REDO;
-        return;
-    }
-    const int dz = ea + EB[0] - sc.exp;
+    if (!ok) return;
+    const int dz = ea + EB[0] - cexp;
     int *dig = part.digits(v);
 #pragma unroll
     for (int r = 0; r < N; r++) dig[r] = (int)fmod_exact(acc[r], C->md[r], C->minv[r]);
     for (int r = 0; r < N; r++) {
         const int w = mulmod(dig[r], pow2mod(C, dz, r), C->md[r], C->minv[r]);
-        dig[r] = (sc.sign && w) ? C->m[r] - w : w;
+        dig[r] = (csign && w) ? C->m[r] - w : w;
     }
-    *part.sign(v) = sc.sign;
-    *part.exp(v) = sc.exp;
-    ErMem *ev = part.eval(v);
-    ev[0] = ErMem{sc.ev0.frac, (long long)sc.ev0.exp};
-    ev[1] = ErMem{sc.ev1.frac, (long long)sc.ev1.exp};
 }
 
 // ---------------------------------------------------------------- SpMV
