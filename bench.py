@@ -35,8 +35,9 @@ PROF_GEMM_RESID = 6        # library timing slot of the gemm residue products
 # HBM bytes per launch of the step's main kernels on this workload, from rocprofv3
 # --pmc FETCH_SIZE and --pmc WRITE_SIZE (profiles/bench_424bit_counters.md), each
 # scaled by its reading for a 256 MiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0)
-TRAFFIC_BYTES = {"chain": 128.4e6, "resid": 649.9e6, "gemv": 2899.6e6}
+TRAFFIC_BYTES = {"chain": 128.4e6, "resid": None, "gemv": 2899.6e6}
 FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 (vector and matrix)
+INT8_PEAK_TOPS = 5000.0    # MI355X dense INT8 MFMA
 
 SET = "32_8double"
 M_GEMV = N_GEMV = 4096
@@ -232,12 +233,13 @@ def main():
                 "traffic": TRAFFIC_BYTES["chain"],
                 "avg_ms": gemm_avg, "work": "M*N*K interval steps, 6 FP64 operations each"}
     elif dom == "resid":
-        # residue multiply-accumulates M*N*K*n, 2 FP64 FMAs (4 flops) each
-        ach = 4.0 * fma_gemm * nmod / (resid_avg / 1e3) / 1e12
-        roof = {"kernel": "k_gemm_resid (modular residue products)", "bound": "mfma", "achieved": ach,
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
-                "traffic": TRAFFIC_BYTES["resid"],
-                "avg_ms": resid_avg, "work": "M*N*K*n residue multiply-accumulates, 2 FP64 FMAs each"}
+        # INT8 MFMA limb products: n moduli x NL^2 limb pairs x M*N*K multiply-adds (2 ops each)
+        nl = 4 if ctx_bits(ctx) <= 28 else 5
+        ach = 2.0 * fma_gemm * nmod * nl * nl / (resid_avg / 1e3) / 1e12
+        roof = {"kernel": "k_mf_gemm + k_gemm_digits (residue products on INT8 MFMA)", "bound": "mfma",
+                "achieved": ach, "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s", "frac": ach / INT8_PEAK_TOPS,
+                "traffic": TRAFFIC_BYTES["resid"], "avg_ms": resid_avg,
+                "work": "M*N*K*n*NL^2 int8 multiply-adds (2 ops each; unit is tera int8 operations/s)"}
     else:
         bytes_ = (M_GEMV * N_GEMV + N_GEMV + 2 * M_GEMV) * S
         ach = bytes_ / (gemv_avg / 1e3) / 1e9
@@ -654,6 +656,11 @@ def counter_ops_run(mp):
     mp.synchronize()
     ctx.close()
     print("[bench] counter run (other operations) done", file=sys.stderr, flush=True)
+
+
+def ctx_bits(ctx):
+    """bit length of the largest modulus"""
+    return max(int(v) for v in ctx_moduli(ctx)).bit_length()
 
 
 def ctx_moduli(ctx):

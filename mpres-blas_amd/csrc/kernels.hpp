@@ -490,10 +490,11 @@ __global__ __launch_bounds__(BLK) void k_gemm_fin(const DevConsts *__restrict__ 
 // so the chain runs on the scalar fields alone (k_gemm_chain), and the residues
 // come from N independent modular matrix products of pre-scaled operands
 //   A'_r[i][row] = s_A x_r 2^(e_A - EA[row]) mod m_r,  B'_r[i][col] likewise,
-//   digits_r = 2^(EA[row] + EB[col] - E) * sum_i A'_r[i][row] B'_r[i][col] mod m_r
-// (k_gemm_resid), EA / EB being the minimum exponents of each row of op(A) and
-// column of op(B). Chains that meet a rare case are marked REDO and recomputed
-// step by step by the fixup kernel.
+//   digits_r = 2^(EA[row] + EB[col] - E) * sum_i A'_r[i][row] B'_r[i][col] mod m_r,
+// EA / EB being the minimum exponents of each row of op(A) and column of op(B).
+// The products run exactly on INT8 MFMA over 7-bit limbs of A' and B'
+// (k_gemm_limbs, k_mf_gemm), and k_gemm_digits applies the rescaling. Chains that
+// meet a rare case are marked REDO and recomputed step by step by the fixup kernel.
 
 template <int N>
 __device__ __forceinline__ int sign_at(const SoA &v, long long i) { return v.s[i]; }
@@ -584,6 +585,58 @@ __global__ __launch_bounds__(BLK) void k_gemm_prescale(const DevConsts *__restri
     }
 }
 
+// Residue planes of the split-form gemm for the INT8 MFMA residue products
+// (k_mf_gemm): the values of k_gemm_prescale, s x_r 2^(e - emin[j]) mod m_r (signed
+// as m_r - v), split into LIMB_BITS-bit limbs, as int8 planes P[r][limb][j][kp] with
+// j < JP (rows of op(A) or columns of op(B), padded with zeros to a multiple of 64)
+// and kp < KP (k padded with zeros to a multiple of LIMB_T).
+constexpr int LIMB_BITS = 7, LIMB_T = 32;
+template <int N, int L, class VM, int SIDE>
+__global__ __launch_bounds__(BLK) void k_gemm_limbs(const DevConsts *__restrict__ C, int cnt, int JP, int k, int KP,
+                                                    int nl, int t, VM X, int ld, const int *__restrict__ emin,
+                                                    signed char *__restrict__ P) {
+    if constexpr (std::is_same<VM, SoA>::value) X = rd(X);
+    const long long g = (long long)blockIdx.x * BLK + threadIdx.x;
+    if (g >= (long long)JP * KP) return;
+    const int kp = (int)(g % KP), j = (int)(g / KP);
+    const size_t plane = (size_t)JP * KP;
+    if (j >= cnt || kp >= k) {
+        for (int r = 0; r < N; r++)
+            for (int a = 0; a < nl; a++) P[((size_t)r * nl + a) * plane + g] = 0;
+        return;
+    }
+    const long long idx = op_index<SIDE>(t, j, kp, ld);
+    const int d = exp_of<N>(X, idx) - emin[j];
+    const int s = sign_at<N>(X, idx);
+    // the element's residues at once (SoA records are 16-byte aligned; AoS 8-byte)
+    int dig[N];
+    if constexpr (std::is_same<VM, SoA>::value) {
+        const int4 *p4 = (const int4 *)digits_of<N>(X, idx);
+#pragma unroll
+        for (int q = 0; q < N / 4; q++) {
+            const int4 v4 = p4[q];
+            dig[4 * q] = v4.x;
+            dig[4 * q + 1] = v4.y;
+            dig[4 * q + 2] = v4.z;
+            dig[4 * q + 3] = v4.w;
+        }
+    } else {
+        const int2 *p2 = (const int2 *)digits_of<N>(X, idx);
+#pragma unroll
+        for (int q = 0; q < N / 2; q++) {
+            const int2 v2 = p2[q];
+            dig[2 * q] = v2.x;
+            dig[2 * q + 1] = v2.y;
+        }
+    }
+    for (int r = 0; r < N; r++) {
+        const int m = C->m[r];
+        int v = dig[r] ? mulmod(dig[r], pow2mod(C, d, r), C->md[r], C->minv[r]) : 0;
+        v = (s && v) ? m - v : v;
+        for (int a = 0; a < nl; a++)
+            P[((size_t)r * nl + a) * plane + g] = (signed char)((v >> (LIMB_BITS * a)) & ((1 << LIMB_BITS) - 1));
+    }
+}
 // scalar chain of every entry (one thread per entry): the sum's sign, exponent and
 // interval evaluation into the workspace, or REDO
 template <int N, int L, class VM>
@@ -633,77 +686,36 @@ __device__ __forceinline__ void resid_digit(const DevConsts *C, int r, int S, in
     sums.digits(e)[r] = (sign && t) ? C->m[r] - t : t;
 }
 
-// Residue products: blockIdx.z = modulus r, 64 x 64 entries per workgroup, each
-// thread 4 x 4. Operands stay exact in FP64: with B-bit moduli, a < 2^B times
-// the halves of b = bh 2^sb + bl (bl < 2^sb, bh < 2^(B-sb)) stay below
-// 2^(53 - log2 kred) per term, so kred terms accumulate exactly; the sums are
-// reduced mod m every kred terms (kred a multiple of RKC, set by the host).
-constexpr int RT = 64, RKC = 16;
+// digits of every entry from its residue sums S[r][e] (the INT8 MFMA residue products).
+// A workgroup's BLK entries are consecutive records; their digits are formed in
+// LDS and written out so that consecutive lanes store consecutive words.
 template <int N, int L>
-__global__ __launch_bounds__(BLK) void k_gemm_resid(const DevConsts *__restrict__ C, int m, int n, int k, int sb,
-                                                    int kred, const int *__restrict__ PA, const int *__restrict__ PB,
-                                                    const int *__restrict__ EA, const int *__restrict__ EB,
-                                                    AoSRef<N> sums) {
-    __shared__ double sa[RKC][RT];
-    __shared__ double sbh[RKC][RT], sbl[RKC][RT];
-    const int r = blockIdx.z;
-    const int row0 = blockIdx.y * RT, col0 = blockIdx.x * RT;
-    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-    const double md = C->md[r], mv = C->minv[r];
-    const int *A = PA + (size_t)r * k * m, *B = PB + (size_t)r * k * n;
-    double ah[4][4], al[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int b = 0; b < 4; b++) ah[a][b] = al[a][b] = 0.0;
-    for (int k0 = 0; k0 < k; k0 += RKC) {
-        for (int q = threadIdx.x; q < RKC * RT; q += BLK) {
-            const int kk = q / RT, c = q % RT, gi = k0 + kk;
-            const int gr = row0 + c, gc = col0 + c;
-            sa[kk][c] = (gi < k && gr < m) ? (double)A[(size_t)gi * m + gr] : 0.0;
-            const int bv = (gi < k && gc < n) ? B[(size_t)gi * n + gc] : 0;
-            sbh[kk][c] = (double)(bv >> sb);
-            sbl[kk][c] = (double)(bv & ((1 << sb) - 1));
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int kk = 0; kk < RKC; kk++) {
-            double a[4], bh[4], bl[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                a[q] = sa[kk][ty + 16 * q];
-                bh[q] = sbh[kk][tx + 16 * q];
-                bl[q] = sbl[kk][tx + 16 * q];
+__global__ __launch_bounds__(BLK) void k_gemm_digits(const DevConsts *__restrict__ C, int m, int n,
+                                                     const int *__restrict__ S, const int *__restrict__ EA,
+                                                     const int *__restrict__ EB, AoSRef<N> sums) {
+    __shared__ int sd[BLK * (N + 1)];
+    __shared__ int ss[BLK];
+    const long long e0 = (long long)blockIdx.x * BLK, e = e0 + threadIdx.x;
+    const long long mn = (long long)m * n;
+    const int cnt = (int)((mn - e0) < BLK ? (mn - e0) : BLK);
+    int sign = REDO;
+    if (e < mn) {
+        const int row = (int)(e % m), col = (int)(e / m);
+        sign = *sums.sign(e);
+        if (sign != REDO) {
+            const int d = EA[row] + EB[col] - *sums.exp(e);
+            for (int r = 0; r < N; r++) {
+                const int t = mulmod(S[(size_t)r * mn + e], pow2mod(C, d, r), C->md[r], C->minv[r]);
+                sd[threadIdx.x * (N + 1) + r] = (sign && t) ? C->m[r] - t : t;
             }
-#pragma unroll
-            for (int x = 0; x < 4; x++)
-#pragma unroll
-                for (int y = 0; y < 4; y++) {
-                    ah[x][y] = fma(a[x], bh[y], ah[x][y]);
-                    al[x][y] = fma(a[x], bl[y], al[x][y]);
-                }
-        }
-        __syncthreads();
-        if ((k0 + RKC) % kred == 0) {
-#pragma unroll
-            for (int x = 0; x < 4; x++)
-#pragma unroll
-                for (int y = 0; y < 4; y++) {
-                    ah[x][y] = fmod_exact(ah[x][y], md, mv);
-                    al[x][y] = fmod_exact(al[x][y], md, mv);
-                }
         }
     }
-#pragma unroll
-    for (int x = 0; x < 4; x++)
-#pragma unroll
-        for (int y = 0; y < 4; y++) {
-            const int row = row0 + ty + 16 * x, col = col0 + tx + 16 * y;
-            if (row >= m || col >= n) continue;
-            const double h = fmod_exact(ah[x][y], md, mv), l = fmod_exact(al[x][y], md, mv);
-            const int S = (int)fmod_exact(h * (double)(1 << sb) + l, md, mv);
-            resid_digit<N>(C, r, S, EA[row], EB[col], sums, (long long)row + (long long)m * col);
-        }
+    ss[threadIdx.x] = sign;
+    __syncthreads();
+    for (int q = threadIdx.x; q < cnt * N; q += BLK) {
+        const int i = q / N, r = q % N;
+        if (ss[i] != REDO) sums.digits(e0 + i)[r] = sd[i * (N + 1) + r];
+    }
 }
 
 // ---------------------------------------------------------------- gemv, split form

@@ -26,8 +26,9 @@ namespace mph {
 typedef int mf_v4i __attribute__((ext_vector_type(4)));
 typedef int mf_v16i __attribute__((ext_vector_type(16)));
 
-constexpr int MF_T = 32;     // MFMA tile (rows, columns and k per instruction)
-constexpr int MF_LB = 7;     // bits per limb (limbs are 0..127, valid int8)
+constexpr int MF_T = LIMB_T;     // MFMA tile (rows, columns and k per instruction)
+constexpr int MF_LB = LIMB_BITS; // bits per limb (limbs are 0..127, valid int8)
+constexpr int MF_KCH = 8192;     // k per INT32 accumulation: NL 127^2 MF_KCH < 2^31 for NL <= 5
 constexpr int MF_NONE = -0x40000000;  // no element (bit bounds of empty rows, columns, bands)
 
 // bit length bound of an element from its interval upper bound: X < 2^(exp+1) M < 2^(exp+2+log2m)
@@ -167,33 +168,46 @@ __global__ __launch_bounds__(BLK) void k_mf_gemm(const DevConsts *__restrict__ C
     const size_t pa = (size_t)MP * KP, pb = (size_t)NP * KP;
     const signed char *A = PA + (size_t)r * NL * pa + (size_t)(row0 + l % 32) * KP + 16 * (l / 32);
     const signed char *B = PB + (size_t)r * NL * pb + (size_t)(col0 + l % 32) * KP + 16 * (l / 32);
+    const double md = C->md[r], mv = C->minv[r];
     mf_v16i acc[NG];
+    double res[16];  // residues of the chunks done so far
 #pragma unroll
-    for (int q = 0; q < NG; q++) acc[q] = mf_v16i{};
+    for (int q = 0; q < 16; q++) res[q] = 0.0;
     if (row0 < MP && col0 < NP) {
-        for (int k0 = 0; k0 < KP; k0 += MF_T) {
-            mf_v4i fa[NL], fb[NL];
+        for (int kc = 0; kc < KP; kc += MF_KCH) {
+            const int ke = (kc + MF_KCH) < KP ? kc + MF_KCH : KP;
 #pragma unroll
-            for (int a = 0; a < NL; a++) {
-                fa[a] = *(const mf_v4i *)(A + a * pa + k0);
-                fb[a] = *(const mf_v4i *)(B + a * pb + k0);
+            for (int q = 0; q < NG; q++) acc[q] = mf_v16i{};
+            for (int k0 = kc; k0 < ke; k0 += MF_T) {
+                mf_v4i fa[NL], fb[NL];
+#pragma unroll
+                for (int a = 0; a < NL; a++) {
+                    fa[a] = *(const mf_v4i *)(A + a * pa + k0);
+                    fb[a] = *(const mf_v4i *)(B + a * pb + k0);
+                }
+#pragma unroll
+                for (int a = 0; a < NL; a++)
+#pragma unroll
+                    for (int b = 0; b < NL; b++)
+                        acc[a + b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fb[b], fa[a], acc[a + b], 0, 0, 0);
             }
+            // the chunk's limb groups recombined (Horner, exact below 2^53) and added mod m
 #pragma unroll
-            for (int a = 0; a < NL; a++)
+            for (int q = 0; q < 16; q++) {
+                double v = 0.0;
 #pragma unroll
-                for (int b = 0; b < NL; b++)
-                    acc[a + b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fb[b], fa[a], acc[a + b], 0, 0, 0);
+                for (int g = NG - 1; g >= 0; g--)
+                    v = (double)modd(v * (double)(1 << MF_LB) + (double)acc[g][q], md, mv);
+                v += res[q];
+                res[q] = v >= md ? v - md : v;
+            }
         }
     }
-    const double md = C->md[r], mv = C->minv[r];
     const int row = row0 + l % 32;
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         const int col = col0 + 8 * (q / 4) + 4 * (l / 32) + q % 4;
-        double v = 0.0;
-#pragma unroll
-        for (int g = NG - 1; g >= 0; g--) v = (double)modd(v * (double)(1 << MF_LB) + (double)acc[g][q], md, mv);
-        if (row < m && col < n) S[(size_t)r * m * n + (size_t)row + (size_t)m * col] = (int)v;
+        if (row < m && col < n) S[(size_t)r * m * n + (size_t)row + (size_t)m * col] = (int)res[q];
     }
 }
 

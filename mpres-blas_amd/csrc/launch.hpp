@@ -171,14 +171,20 @@ struct Launch {
         return hipGetLastError();
     }
     // gemm passes. Over 16 and 32 moduli (split form, see kernels.hpp):
-    //   0 exponent minima and pre-scaled residue planes, 1 scalar chains,
-    //   2 residue products and digits, 3 fixup of the marked chains, 4 epilogue.
+    //   0 exponent minima and the int8 limb planes of the pre-scaled residues, 1 scalar chains,
+    //   2 residue products (INT8 MFMA) and digits, 3 fixup of the marked chains, 4 epilogue.
     // Over 8 moduli: 1 full sums, 4 epilogue. bits: bit length of the largest modulus.
-    // ws: m*n sums, then (split form) the planes of op(A) and op(B) and the minima.
+    // ws: m*n sums, then (split form) the residue sums S (N*m*n ints), the minima (m + n
+    // ints) and the limb planes of op(A) and op(B).
+    static constexpr int GEMM_NL = 5;  // limbs per residue reserved (moduli below 2^35)
+    static int gemm_limbs(int bits) { return bits <= 4 * LIMB_BITS ? 4 : 5; }
     static size_t gemm_ws(int m, int n, int k) {
         const size_t sums = ((size_t)m * n * (4 * N + 40) + 255) / 256 * 256;
         if (!SPLIT) return sums;
-        return sums + 4 * ((size_t)N * k * m + (size_t)N * k * n + m + n);
+        const size_t ints = ((4 * ((size_t)N * m * n + m + n)) + 255) / 256 * 256;
+        const size_t MPd = (m + 63) / 64 * 64, NPd = (n + 63) / 64 * 64,
+                     KPd = (size_t)((k > 0 ? k : 1) + LIMB_T - 1) / LIMB_T * LIMB_T;
+        return sums + ints + (size_t)N * GEMM_NL * (MPd + NPd) * KPd;
     }
     template <class VM>
     static void gemm_pass(int pass, const DevConsts *c, int bits, int ta, int tb, int m, int n, int k, VM A, int lda,
@@ -187,7 +193,12 @@ struct Launch {
         const dim3 gr(grid_for(mn, L)), bl(BLK);
         const AoSRef<N> sums{ws};
         const size_t sb_ = ((size_t)mn * (4 * N + 40) + 255) / 256 * 256;
-        int *PA = (int *)(ws + sb_), *PB = PA + (size_t)N * k * m, *EA = PB + (size_t)N * k * n, *EB = EA + m;
+        int *S = (int *)(ws + sb_), *EA = S + (size_t)N * mn, *EB = EA + m;
+        const size_t ib_ = ((4 * ((size_t)N * mn + m + n)) + 255) / 256 * 256;
+        const int MP = (m + 63) / 64 * 64, NP = (n + 63) / 64 * 64,
+                  KP = ((k > 0 ? k : 1) + LIMB_T - 1) / LIMB_T * LIMB_T;
+        const int nl = gemm_limbs(bits);
+        signed char *PA = (signed char *)(ws + sb_ + ib_), *PB = PA + (size_t)N * nl * MP * KP;
         if constexpr (SPLIT) {
             if (pass == 0) {
                 hipLaunchKernelGGL((k_fill_int<N, L>), dim3(grid_for((long long)m + n, 1)), bl, 0, st, EA, m + n,
@@ -198,20 +209,23 @@ struct Launch {
                                        k, ta, A, lda, EA);
                     hipLaunchKernelGGL((k_gemm_emin<N, L, VM, 1>), dim3((n + EMIN_J - 1) / EMIN_J, kc), bl, 0, st, n,
                                        k, tb, B, ldb, EB);
-                    hipLaunchKernelGGL((k_gemm_prescale<N, L, VM, 0>), dim3(grid_for((long long)m * k, 1)), bl, 0, st,
-                                       c, m, k, ta, A, lda, EA, PA);
-                    hipLaunchKernelGGL((k_gemm_prescale<N, L, VM, 1>), dim3(grid_for((long long)n * k, 1)), bl, 0, st,
-                                       c, n, k, tb, B, ldb, EB, PB);
                 }
+                hipLaunchKernelGGL((k_gemm_limbs<N, L, VM, 0>), dim3(grid_for((long long)MP * KP, 1)), bl, 0, st, c, m,
+                                   MP, k, KP, nl, ta, A, lda, EA, PA);
+                hipLaunchKernelGGL((k_gemm_limbs<N, L, VM, 1>), dim3(grid_for((long long)NP * KP, 1)), bl, 0, st, c, n,
+                                   NP, k, KP, nl, tb, B, ldb, EB, PB);
             } else if (pass == 1) {
                 hipLaunchKernelGGL((k_gemm_chain<N, L, VM>), dim3(grid_for(mn, 1)), bl, 0, st, c, ta, tb, m, n, k, A,
                                    lda, B, ldb, sums);
             } else if (pass == 2) {
-                const int sbits = bits / 2;
-                const int hi = (bits + sbits > 2 * bits - sbits) ? bits + sbits : 2 * bits - sbits;
-                const int kred = 1 << (52 - hi);
-                const dim3 rg((n + RT - 1) / RT, (m + RT - 1) / RT, N);
-                hipLaunchKernelGGL((k_gemm_resid<N, L>), rg, bl, 0, st, c, m, n, k, sbits, kred, PA, PB, EA, EB, sums);
+                const dim3 gg(NP / 64, MP / 64, N);
+                if (nl == 4)
+                    hipLaunchKernelGGL((k_mf_gemm<N, L, 4>), gg, bl, 0, st, c, m, n, MP, NP, KP, PA, PB, nullptr,
+                                       nullptr, S);
+                else
+                    hipLaunchKernelGGL((k_mf_gemm<N, L, 5>), gg, bl, 0, st, c, m, n, MP, NP, KP, PA, PB, nullptr,
+                                       nullptr, S);
+                hipLaunchKernelGGL((k_gemm_digits<N, L>), dim3(grid_for(mn, 1)), bl, 0, st, c, m, n, S, EA, EB, sums);
             } else if (pass == 3) {
                 hipLaunchKernelGGL((k_gemm_sum<N, L, true, true, VM>), gr, bl, 0, st, c, ta, tb, m, n, k, A, lda, B,
                                    ldb, sums);
@@ -246,13 +260,12 @@ struct Launch {
     }
     // ---- precision form on INT8 MFMA (kernels_mfma.hpp) ----
     static constexpr long long rup(long long v, long long q) { return (v + q - 1) / q * q; }
-    static int mf_limbs(int bits) { return (bits + MF_LB - 1) / MF_LB; }
     // ws: S (4 band pairs x N*m*n ints); per row and column: minimum exponent, top bit, top
     // bit of the low band, high / low band exponents and bit bounds (7 (m + n) ints); then the
     // limb planes of both bands of op(A) and op(B)
     static size_t mf_ints(int m, int n) { return 4 * (size_t)N * m * n + 7 * ((size_t)m + n); }
     static size_t gemm_mfma_ws(int bits, int m, int n, int k) {
-        const int nl = mf_limbs(bits);
+        const int nl = gemm_limbs(bits);
         const size_t MPd = rup(m, 64), NPd = rup(n, 64), KPd = rup(k > 0 ? k : 1, MF_T);
         return (4 * mf_ints(m, n) + 255) / 256 * 256 + 2 * (size_t)N * nl * (MPd + NPd) * KPd;
     }
@@ -260,8 +273,7 @@ struct Launch {
                                 SoA alpha, SoA A, int lda, SoA B, int ldb, SoA beta, SoA Cm, int ldc, char *ws,
                                 hipStream_t st) {
         if ((long long)m * n <= 0) return hipSuccess;
-        const int nl = mf_limbs(bits);
-        if (nl != 4 && nl != 5) return hipErrorInvalidValue;
+        const int nl = gemm_limbs(bits);
         const int MP = (int)rup(m, 64), NP = (int)rup(n, 64), KP = (int)rup(k > 0 ? k : 1, MF_T);
         int *S = (int *)ws;
         int *V = S + 4 * (size_t)N * m * n;  // 7 arrays of m + n: rows of op(A) first, then columns of op(B)

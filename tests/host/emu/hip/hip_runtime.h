@@ -2,10 +2,11 @@
 // (tests/host/kernels_emu.cpp, tests/host/batch_emu.cpp), which compile this
 // library's own kernel templates with g++ and run them on the CPU, one lane per
 // number (L = 1). With L = 1 a group is a single lane, so every cross-lane
-// exchange is the identity. kernels_emu.cpp runs the threads of a workgroup one
-// after another (kernels that synchronise a workgroup are not run there);
-// batch_emu.cpp runs the threads of one workgroup as concurrent host threads and
-// sets emu_barrier to a barrier wait, so __syncthreads synchronises them.
+// exchange is the identity. Kernels without workgroup synchronisation run their
+// threads one after another; for kernels that synchronise (kernels_emu.cpp's
+// run_wg, batch_emu.cpp) the threads of one workgroup run as concurrent host
+// threads and emu_barrier is set to a barrier wait, so __syncthreads synchronises
+// them.
 #pragma once
 
 #include <cmath>
@@ -24,6 +25,9 @@ typedef int hipError_t;
 typedef void *hipStream_t;
 enum { hipSuccess = 0, hipErrorInvalidValue = 1 };
 
+struct int2 {
+    int x, y;
+};
 struct int4 {
     int x, y, z, w;
 };

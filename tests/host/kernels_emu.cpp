@@ -4,7 +4,10 @@
 // (block, thread) is run in turn. This checks the kernels' own logic (fast
 // path, marking, fixup, epilogues) against the oracle on the CPU, where a
 // mistake can be found with ordinary tools. Built by tests/test_kernels_emu.py.
+#include <pthread.h>
+
 #include <algorithm>
+#include <thread>
 #include <vector>
 
 #include "../../mpres-blas_amd/csrc/internal.hpp"
@@ -38,6 +41,29 @@ void run(long long groups, F &&f) {
         }
 }
 
+// run a kernel that synchronises its workgroup: the BLK threads of each block run as
+// concurrent host threads with a barrier behind __syncthreads, blocks one after another
+pthread_barrier_t g_bar;
+void bar_wait() { pthread_barrier_wait(&g_bar); }
+template <class F>
+void run_wg(long long groups, F &&f) {
+    const long long blocks = (groups + BLK - 1) / BLK;
+    emu_barrier = bar_wait;
+    for (long long b = 0; b < blocks; b++) {
+        pthread_barrier_init(&g_bar, nullptr, BLK);
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < (unsigned)BLK; t++)
+            th.emplace_back([&, t, b] {
+                blockIdx = EmuIdx{(unsigned)b, 0, 0};
+                threadIdx = EmuIdx{t, 0, 0};
+                f();
+            });
+        for (auto &t : th) t.join();
+        pthread_barrier_destroy(&g_bar);
+    }
+    emu_barrier = nullptr;
+}
+
 template <int N>
 int gemm_v2_n(const Ctx &x, int ta, int tb, int m, int n, int k, char *alpha, char *A, int lda, char *B, int ldb,
               char *beta, char *Cm, int ldc, char *ws, int split) {
@@ -63,7 +89,7 @@ int gemm_v2_split_n(const Ctx &x, int ta, int tb, int m, int n, int k, char *alp
                     char *beta, char *Cm, int ldc, char *ws) {
     const long long mn = (long long)m * n;
     const AoSRef<N> a{A}, b{B}, w{ws};
-    std::vector<int> EA(m), EB(n), PA((size_t)N * k * m), PB((size_t)N * k * n);
+    std::vector<int> EA(m), EB(n);
     // the exponent minima kernels reduce through LDS across a workgroup: restated here
     for (int row = 0; row < m; row++) {
         int e = 0x7fffffff;
@@ -75,19 +101,45 @@ int gemm_v2_split_n(const Ctx &x, int ta, int tb, int m, int n, int k, char *alp
         for (int i = 0; i < k; i++) e = std::min(e, *b.exp(op_index<1>(tb, col, i, ldb)));
         EB[col] = k > 0 ? e : 0;
     }
-    run((long long)m * k, [&] { k_gemm_prescale<N, 1, AoSRef<N>, 0>(&x.c, m, k, ta, a, lda, EA.data(), PA.data()); });
-    run((long long)n * k, [&] { k_gemm_prescale<N, 1, AoSRef<N>, 1>(&x.c, n, k, tb, b, ldb, EB.data(), PB.data()); });
+    // int8 limb planes through the kernel; the INT8 MFMA products restated in 64-bit integers
+    int bits = 0;
+    for (int r = 0; r < N; r++)
+        while ((1LL << bits) <= x.c.m[r]) bits++;
+    const int nl = bits <= 4 * LIMB_BITS ? 4 : 5;
+    const int MP = (m + 63) / 64 * 64, NP = (n + 63) / 64 * 64, KP = ((k > 0 ? k : 1) + LIMB_T - 1) / LIMB_T * LIMB_T;
+    std::vector<signed char> PA((size_t)N * nl * MP * KP, 99), PB((size_t)N * nl * NP * KP, 99);
+    run((long long)MP * KP,
+        [&] { k_gemm_limbs<N, 1, AoSRef<N>, 0>(&x.c, m, MP, k, KP, nl, ta, a, lda, EA.data(), PA.data()); });
+    run((long long)NP * KP,
+        [&] { k_gemm_limbs<N, 1, AoSRef<N>, 1>(&x.c, n, NP, k, KP, nl, tb, b, ldb, EB.data(), PB.data()); });
     run(mn, [&] { k_gemm_chain<N, 1, AoSRef<N>>(&x.c, ta, tb, m, n, k, a, lda, b, ldb, w); });
+    auto val = [&](const std::vector<signed char> &P, int JP, int r, int j, int i) {
+        unsigned long long v = 0;
+        for (int q = 0; q < nl; q++) {
+            const int limb = P[((size_t)r * nl + q) * JP * KP + (size_t)j * KP + i];
+            if (limb < 0) return ~0ULL;  // not a limb: the planes are not fully written
+            v |= (unsigned long long)limb << (LIMB_BITS * q);
+        }
+        return v;
+    };
+    std::vector<int> S((size_t)N * mn);
     for (int r = 0; r < N; r++) {
-        const long long mod = x.c.m[r];
+        const unsigned long long mod = x.c.m[r];
         for (int col = 0; col < n; col++)
             for (int row = 0; row < m; row++) {
-                long long S = 0;
-                for (int i = 0; i < k; i++)
-                    S = (S + (long long)PA[((size_t)r * k + i) * m + row] * PB[((size_t)r * k + i) * n + col]) % mod;
-                resid_digit<N>(&x.c, r, (int)S, EA[row], EB[col], w, (long long)row + (long long)m * col);
+                unsigned long long acc = 0;
+                for (int i = 0; i < KP; i++) {
+                    const unsigned long long va = val(PA, MP, r, row, i), vb = val(PB, NP, r, col, i);
+                    if (va >= mod || vb >= mod) return -2;  // padding must be zero, values residues
+                    acc = (acc + va * vb % mod) % mod;
+                }
+                S[(size_t)r * mn + row + (size_t)m * col] = (int)acc;
             }
+        for (int j = m; j < MP; j++)
+            for (int i = 0; i < KP; i++)
+                if (val(PA, MP, r, j, i) != 0) return -2;
     }
+    run_wg(mn, [&] { k_gemm_digits<N, 1>(&x.c, m, n, S.data(), EA.data(), EB.data(), w); });
     run(mn, [&] { k_gemm_sum<N, 1, true, true, AoSRef<N>>(&x.c, ta, tb, m, n, k, a, lda, b, ldb, w); });
     run(mn, [&] {
         k_gemm_fin<N, 1, true, AoSRef<N>>(&x.c, m, n, AoSRef<N>{alpha}, AoSRef<N>{beta}, AoSRef<N>{Cm}, ldc, w);

@@ -22,7 +22,7 @@ SETS = {"8_2double": (-20, 20), "16_4double": (-150, 150), "32_8double": (-400, 
 @pytest.fixture(scope="module")
 def emu(tmp_path_factory):
     out = tmp_path_factory.mktemp("emu") / "libkemu.so"
-    cmd = ["g++", "-std=c++17", "-O1", "-fPIC", "-shared", "-ffp-contract=off",
+    cmd = ["g++", "-std=c++17", "-O1", "-fPIC", "-shared", "-ffp-contract=off", "-pthread",
            "-I", os.path.join(ROOT, "tests", "host", "emu"),
            os.path.join(ROOT, "tests", "host", "kernels_emu.cpp"),
            os.path.join(ROOT, "mpres-blas_amd", "csrc", "consts.cpp"), "-o", str(out)]
