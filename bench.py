@@ -35,7 +35,11 @@ PROF_GEMM_RESID = 6        # library timing slot of the gemm residue products
 # HBM bytes per launch of the step's main kernels on this workload, from rocprofv3
 # --pmc FETCH_SIZE and --pmc WRITE_SIZE (profiles/bench_424bit_counters.md), each
 # scaled by its reading for a 256 MiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0)
-TRAFFIC_BYTES = {"chain": 128.4e6, "resid": None, "gemv": 2899.6e6}
+TRAFFIC_BYTES = {"chain": 117.7e6, "resid": 436.6e6, "gemv": 2259.0e6}
+# VALU instructions per launch of the gemm scalar chains (SQ_INSTS_VALU, same profile)
+# and the issue peak: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction
+CHAIN_VALU_INSTS = 4.95e8
+VALU_ISSUE_PEAK = 614.4e9
 FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 (vector and matrix)
 INT8_PEAK_TOPS = 5000.0    # MI355X dense INT8 MFMA
 
@@ -231,7 +235,9 @@ def main():
         roof = {"kernel": "k_gemm_chain (scalar chains of the gemm entries)", "bound": "mfma", "achieved": ach,
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
                 "traffic": TRAFFIC_BYTES["chain"],
-                "avg_ms": gemm_avg, "work": "M*N*K interval steps, 6 FP64 operations each"}
+                "avg_ms": gemm_avg, "work": "M*N*K interval steps, 6 FP64 operations each",
+                "binding_resource": "VALU issue: each directed-rounding operation is several instructions",
+                "valu_issue_frac": CHAIN_VALU_INSTS / (gemm_avg / 1e3) / VALU_ISSUE_PEAK}
     elif dom == "resid":
         # INT8 MFMA limb products: n moduli x NL^2 limb pairs x M*N*K multiply-adds (2 ops each)
         nl = 4 if ctx_bits(ctx) <= 28 else 5
@@ -241,11 +247,11 @@ def main():
                 "traffic": TRAFFIC_BYTES["resid"], "avg_ms": resid_avg,
                 "work": "M*N*K*n*NL^2 int8 multiply-adds (2 ops each; unit is tera int8 operations/s)"}
     else:
-        bytes_ = (M_GEMV * N_GEMV + N_GEMV + 2 * M_GEMV) * S
+        bytes_ = M_GEMV * N_GEMV * (4 * nmod + 8)
         ach = bytes_ / (gemv_avg / 1e3) / 1e9
-        roof = {"kernel": "k_gemv_split (gemv partial sums)", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES["gemv"], "avg_ms": gemv_avg,
-                "work": "(m*n + n + 2m) * (4n+40) bytes"}
+        roof = {"kernel": "k_gemv_split (gemv partial sums: residues of A)", "bound": "hbm", "achieved": ach,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES["gemv"],
+                "avg_ms": gemv_avg, "work": "m*n*(4n+8) bytes (residues, sign and exponent of A)"}
 
     # ---- accuracy: sampled outputs against the exact dyadic result ----
     m_ = [int(v) for v in ctx_moduli(ctx)]
