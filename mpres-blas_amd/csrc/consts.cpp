@@ -156,6 +156,11 @@ bool compute_consts(const ModuliSet &s, DevConsts &c, std::vector<int> &pow2) {
         c.unit_upp = {f, e - sh, 0};
         f = std::frexp(1.0, &e);
         c.inv_unit_upp = {f, e, 0};
+        // two-double reciprocals of the unit's fractions (rounding to nearest)
+        c.ul_rh = 1.0 / c.unit_low.frac;
+        c.ul_rl = std::fma(-c.ul_rh, c.unit_low.frac, 1.0) / c.unit_low.frac;
+        c.uu_rh = 1.0 / c.unit_upp.frac;
+        c.uu_rl = std::fma(-c.uu_rh, c.unit_upp.frac, 1.0) / c.unit_upp.frac;
         f = std::frexp(1.0 - std::ldexp(1.0, -53), &e);
         c.inv_unit_low = {f, e, 0};
     }

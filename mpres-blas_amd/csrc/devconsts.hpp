@@ -28,6 +28,7 @@ struct DevConsts {
     int mp_precision; // MP_PRECISION
     double acc;       // RNS_EVAL_ACCURACY
     ErC unit_low, unit_upp, inv_unit_low, inv_unit_upp;  // RNS_EVAL_UNIT / RNS_EVAL_INV_UNIT
+    double ul_rh, ul_rl, uu_rh, uu_rl;  // two-double reciprocals of unit_low.frac, unit_upp.frac (recip2)
     int m[MAXN];           // moduli
     int w[MAXN];           // (M/m_i)^-1 mod m_i
     double md[MAXN];       // m_i as double

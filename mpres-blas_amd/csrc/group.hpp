@@ -121,8 +121,8 @@ __device__ __forceinline__ void sc_mul(const DevConsts *C, Sc &r, int xs, int xe
                                        Er y1) {
     r.exp = xe + ye;
     r.sign = xs ^ ys;
-    r.ev0 = er_md_rd(x0, y0, Er{C->unit_upp.frac, C->unit_upp.exp});
-    r.ev1 = er_md_ru(x1, y1, Er{C->unit_low.frac, C->unit_low.exp});
+    r.ev0 = er_mdk_rd(x0, y0, C->unit_upp.frac, C->unit_upp.exp, C->uu_rh, C->uu_rl);
+    r.ev1 = er_mdk_ru(x1, y1, C->unit_low.frac, C->unit_low.exp, C->ul_rh, C->ul_rl);
 }
 // acc = acc + x*y on the scalar fields, for chains whose every step is exact:
 // false when the product or the sum needs rounding, the sum's interval

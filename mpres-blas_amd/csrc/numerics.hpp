@@ -167,6 +167,35 @@ MPH_HD Er er_add_dir(Er x, Er y, bool up) {
 MPH_HD Er er_md_rd(Er x, Er y, Er z) { return er_adjust(div_rd(mul_rd(x.frac, y.frac), z.frac), x.exp + y.exp - z.exp); }
 MPH_HD Er er_md_ru(Er x, Er y, Er z) { return er_adjust(div_ru(mul_ru(x.frac, y.frac), z.frac), x.exp + y.exp - z.exp); }
 
+// Directed division by a fixed z in [2^-4, 2^4], for 2^-900 < |a| < 2^900 or a = 0, from a
+// two-double reciprocal zh + zl of z (recip2): zh = RN(1/z) and zl = RN(fma(-zh, z, 1) / z)
+// (the fma is exact), so |1/z - zh - zl| <= 2^-105 / z. q = RN(a zh + RN(a zl)) rounds a value
+// within 2^-104 |a/z| of a/z: no double lies strictly between them unless a/z is within that
+// distance of it, so q is one of the two doubles enclosing a/z (or a/z itself). The remainder
+// r = a - q z is exact (fma; q is within one ulp of a/z) and its sign tells on which side q
+// lies. The results equal div_rd(a, z) / div_ru(a, z) without the division sequence.
+MPH_HD void recip2(double z, double &zh, double &zl) {
+    zh = 1.0 / z;
+    zl = fma(-zh, z, 1.0) / z;
+}
+MPH_HD double divk_rd(double a, double z, double zh, double zl) {
+    const double q = fma(a, zh, a * zl);
+    const double r = fma(-q, z, a);
+    return (a == 0) ? a : ((r < 0) ? step_down(q) : q);  // a zero quotient keeps a's sign
+}
+MPH_HD double divk_ru(double a, double z, double zh, double zl) {
+    const double q = fma(a, zh, a * zl);
+    const double r = fma(-q, z, a);
+    return (a == 0) ? a : ((r > 0) ? step_up(q) : q);
+}
+// er_md_rd / er_md_ru with z the interval evaluation's unit (frac in [1/2, 1)), through divk
+MPH_HD Er er_mdk_rd(Er x, Er y, double zf, int ze, double zh, double zl) {
+    return er_adjust(divk_rd(mul_rd(x.frac, y.frac), zf, zh, zl), x.exp + y.exp - ze);
+}
+MPH_HD Er er_mdk_ru(Er x, Er y, double zf, int ze, double zh, double zl) {
+    return er_adjust(divk_ru(mul_ru(x.frac, y.frac), zf, zh, zl), x.exp + y.exp - ze);
+}
+
 // ---------------------------------------------------------------------------
 // Exact modular arithmetic for moduli m < 2^31 (values held as int).
 // ---------------------------------------------------------------------------

@@ -5,6 +5,7 @@ import ctypes as C
 import math
 import os
 import struct
+from fractions import Fraction
 import subprocess
 import tempfile
 
@@ -25,11 +26,13 @@ def nh():
     d = tempfile.mkdtemp()
     so = os.path.join(d, "nh.so")
     subprocess.run(["g++", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-o", so,
-                    os.path.join(ROOT, "tests", "host", "numerics_host.cpp")], check=True)
+                    os.path.join(ROOT, "tests", "host", "numerics_host.cpp"),
+                    os.path.join(ROOT, "mpres-blas_amd", "csrc", "consts.cpp")], check=True)
     L = C.CDLL(so)
-    for f in ("nh_add", "nh_sub", "nh_mul", "nh_div"):
+    for f in ("nh_add", "nh_sub", "nh_mul", "nh_div", "nh_divk"):
         getattr(L, f).restype = C.c_double
         getattr(L, f).argtypes = [C.c_int, C.c_double, C.c_double]
+    L.nh_unit.argtypes = [C.c_int, C.POINTER(C.c_double)]
     L.nh_ceil_log2.argtypes = [C.c_double]
     L.nh_mulmod.argtypes = [C.c_int, C.c_int, C.c_int]
     L.nh_modd.argtypes = [C.c_double, C.c_int]
@@ -102,3 +105,35 @@ def test_mulmod_exact(nh, m):
         assert nh.nh_mulmod(a, b, m) == (a * b) % m
     for v in list(rng.integers(-2**52, 2**52, 3000)) + [0, 2**53 - 1, -(2**53 - 1)]:
         assert nh.nh_modd(float(v), m) == int(v) % m
+
+
+def test_division_by_unit_matches_ieee(nh, oracle_lib):
+    """divk_rd / divk_ru (reciprocal of the fixed divisor in two doubles, exact remainder)
+    against IEEE directed division: the actual unit fractions of every moduli set (and the
+    reciprocals the constants store), random divisors, dividends that are products of two
+    fractions (as in er_md), and quotients that are exact or within one rounding of a double"""
+    L = oracle_lib.lib()
+    rng = np.random.default_rng(23)
+    zs = []
+    for sid in range(len(po.set_names())):
+        u = (C.c_double * 6)()
+        assert nh.nh_unit(sid, u) == 0
+        zs += [u[0], u[1]]
+        for z, zh, zl in ((u[0], u[2], u[3]), (u[1], u[4], u[5])):
+            e = 1 - Fraction(zh) * Fraction(z)  # what the fma forms: exact and representable
+            assert zh == 1.0 / z and Fraction(float(e)) == e and zl == float(e) / z
+    zs += list(rng.uniform(0.5, 1.0, 40)) + list(rng.uniform(1.0, 2.0, 40)) + [0.5, 1.0, math.nextafter(1.0, 0)]
+    checked = 0
+    for z in zs:
+        fx = rng.uniform(1.0, 2.0, 300)
+        fy = rng.uniform(1.0, 2.0, 300)
+        a = [x * y for x, y in zip(fx, fy)]
+        a += list(rng.uniform(1.0, 4.0, 100)) + [0.0, -0.0, 1.0, 2.0, math.nextafter(4.0, 0)]
+        q = rng.uniform(0.5, 8.0, 200)
+        a += [float(v * z) for v in q]  # a/z within a rounding of a double (often exact)
+        for v in a:
+            for up in (0, 1):
+                got, want = nh.nh_divk(up, v, z), L.orc_ddiv(po.POLICY_D, up, v, z)
+                assert bits(got) == bits(want), (up, v, z, got, want)
+                checked += 1
+    assert checked > 100000
