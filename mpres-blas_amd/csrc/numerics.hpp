@@ -117,14 +117,22 @@ struct Er {
     int exp;
 };
 
-// er_adjust, src/extrange.cuh:412-421
+// er_adjust, src/extrange.cuh:412-421: frac scaled into [1, 2) (sign kept), exp
+// corrected; a zero frac gives exp 0. The reference reads the exponent field; here
+// frexp does it (v_frexp_mant / v_frexp_exp on the GPU), which is the same for every
+// input this code passes: normal doubles (products, quotients and sums of fractions in
+// [1/2, 2), or of a fraction and a 53-bit significand) and signed zeros.
 MPH_HD Er er_adjust(double frac, int exp) {
-    const uint64_t u = d2u(frac);
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double m = __builtin_amdgcn_frexp_mant(frac);
+    const int e = __builtin_amdgcn_frexp_exp(frac);
+#else
+    int e;
+    const double m = frexp(frac, &e);
+#endif
     Er r;
-    const bool nz = frac != 0;
-    r.exp = nz ? (exp + (int)((u & ~((uint64_t)1 << 63)) >> 52) - 1023) : 0;
-    const uint64_t v = (u & ((((uint64_t)1) << 52) - 1)) | (u & ((uint64_t)1 << 63)) | ((uint64_t)1023 << 52);
-    r.frac = u2d(v) * (double)nz;
+    r.frac = m + m;
+    r.exp = (frac != 0) ? exp + e - 1 : 0;
     return r;
 }
 // er_set_d, src/extrange.cuh:46-58 (CUDA :94-106)

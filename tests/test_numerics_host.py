@@ -33,6 +33,7 @@ def nh():
         getattr(L, f).restype = C.c_double
         getattr(L, f).argtypes = [C.c_int, C.c_double, C.c_double]
     L.nh_unit.argtypes = [C.c_int, C.POINTER(C.c_double)]
+    L.nh_er_adjust.argtypes = [C.c_double, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.nh_ceil_log2.argtypes = [C.c_double]
     L.nh_mulmod.argtypes = [C.c_int, C.c_int, C.c_int]
     L.nh_modd.argtypes = [C.c_double, C.c_int]
@@ -137,3 +138,25 @@ def test_division_by_unit_matches_ieee(nh, oracle_lib):
                 assert bits(got) == bits(want), (up, v, z, got, want)
                 checked += 1
     assert checked > 100000
+
+
+def test_er_adjust_matches_bit_definition(nh):
+    """er_adjust through frexp against the reference's definition on the bit pattern
+    (exponent field read out, mantissa given the exponent of 1.0): normal inputs of both
+    signs over a wide range, powers of two and their neighbours, and signed zeros (the
+    inputs er_adjust receives)"""
+    def ref(f, e):
+        u = bits(f)
+        ex = (e + ((u & ~(1 << 63)) >> 52) - 1023) if f != 0 else 0
+        fb = (u & (((1 << 52) - 1) | (1 << 63))) | (1023 << 52)
+        return struct.unpack("<d", struct.pack("<Q", fb))[0] * (1.0 if f != 0 else 0.0), ex
+    rng = np.random.default_rng(5)
+    fs = list(np.ldexp(rng.uniform(-2, 2, 20000), rng.integers(-300, 300, 20000)))
+    fs += [0.0, -0.0, 1.0, -1.0, 2.0, 0.5, 3.999999999999999, math.nextafter(1.0, 0), math.nextafter(2.0, 0),
+           math.nextafter(4.0, 0), 2.0**-1000, -2.0**1000]
+    rf, re_ = C.c_double(), C.c_int()
+    for i, f in enumerate(fs):
+        e = int(rng.integers(-2000, 2000))
+        nh.nh_er_adjust(float(f), e, C.byref(rf), C.byref(re_))
+        wf, we = ref(float(f), e)
+        assert bits(rf.value) == bits(wf) and re_.value == we, (f, e, rf.value, re_.value, wf, we)
