@@ -324,24 +324,25 @@ __global__ __launch_bounds__(BLK) void k_gemv_fold(const DevConsts *__restrict__
 // through registers in two halves, so that the next step's loads can be issued
 // before the current step computes: fetch issues the loads, store writes them
 // to LDS. Other runs are copied directly by store.
-template <int N, int L>
+// T: threads of the workgroup (BLK unless the kernel launches fewer).
+template <int N, int L, int T = BLK>
 struct RunStage {
-    static constexpr int W = BLK / L, SP = N + 1;
-    static constexpr bool VEC = (W * N) % (4 * BLK) == 0;
-    static constexpr int V4 = VEC ? W * N / 4 / BLK : 1;  // int4 per thread for a full run
+    static constexpr int W = T / L, SP = N + 1;
+    static constexpr bool VEC = (W * N) % (4 * T) == 0;
+    static constexpr int V4 = VEC ? W * N / 4 / T : 1;  // int4 per thread for a full run
     int4 v[V4];
     __device__ __forceinline__ void fetch(const int *src, int rows) {
         if (VEC && rows == W) {
             const int4 *s4 = (const int4 *)src;
 #pragma unroll
-            for (int q = 0; q < V4; q++) v[q] = s4[q * BLK + threadIdx.x];
+            for (int q = 0; q < V4; q++) v[q] = s4[q * T + threadIdx.x];
         }
     }
     __device__ __forceinline__ void store(int *sd, const int *src, int rows) const {
         if (VEC && rows == W) {
 #pragma unroll
             for (int q = 0; q < V4; q++) {
-                const int i = 4 * (q * BLK + threadIdx.x), e = i / N, f = i % N;
+                const int i = 4 * (q * T + threadIdx.x), e = i / N, f = i % N;
                 int *d = sd + e * SP + f;
                 d[0] = v[q].x;
                 d[1] = v[q].y;
@@ -349,7 +350,7 @@ struct RunStage {
                 d[3] = v[q].w;
             }
         } else {
-            for (int q = threadIdx.x; q < rows * N; q += BLK) sd[(q / N) * SP + q % N] = src[q];
+            for (int q = threadIdx.x; q < rows * N; q += T) sd[(q / N) * SP + q % N] = src[q];
         }
     }
 };
@@ -732,6 +733,7 @@ __global__ __launch_bounds__(BLK) void k_gemm_digits(const DevConsts *__restrict
 // residue; d >= 64 takes the slow path. A's residues are staged through LDS as
 // in k_gemv_rows. Chains marked REDO are left to the fixup kernel.
 constexpr int GV_D = 64;
+constexpr int GV_T = 128;  // rows (threads) per workgroup of k_gemv_split
 // scalar chain of the partial (o, t), o fastest (one thread each, so that a
 // wavefront reads a run of one column of A's scalar fields per step)
 template <int N, int L>
@@ -764,17 +766,17 @@ __global__ __launch_bounds__(BLK) void k_gemv_chain(const DevConsts *__restrict_
 
 constexpr int GV_REDUCE = 1 << 20;  // terms between reductions of the double accumulators
 template <int N, int L>
-__global__ __launch_bounds__(BLK) void k_gemv_split(const DevConsts *__restrict__ C, int m, int n, SoA A, int lda,
+__global__ __launch_bounds__(GV_T) void k_gemv_split(const DevConsts *__restrict__ C, int m, int n, SoA A, int lda,
                                                     AoSRef<N> bx, const int *__restrict__ bxp,
                                                     const int *__restrict__ EA, const int *__restrict__ EB, int b3,
                                                     AoSRef<N> part) {
     constexpr int SP = N + 1;
-    __shared__ int sd[BLK * SP];
-    __shared__ double tp[GV_D * SP];
+    __shared__ int sd[GV_T * SP];
+    __shared__ int tp[GV_D * SP];
     A = rd(A);
     const int t = blockIdx.y;
-    const long long o0 = (long long)blockIdx.x * BLK, o = o0 + threadIdx.x;
-    const int rows = (int)((m - o0) < BLK ? (m - o0) : BLK);
+    const long long o0 = (long long)blockIdx.x * GV_T, o = o0 + threadIdx.x;
+    const int rows = (int)((m - o0) < GV_T ? (m - o0) : GV_T);
     const bool live = o < m;
     const long long v = o * b3 + t;
     // the chain's result (k_gemv_chain): sign and exponent, or REDO
@@ -790,7 +792,7 @@ __global__ __launch_bounds__(BLK) void k_gemv_split(const DevConsts *__restrict_
     for (int r = 0; r < N; r++) acc[r] = 0.0;
     int since = 0;
     // table build: thread q -> residue q % N, multiples d0 .. d0 + DB - 1
-    constexpr int DB = GV_D * N / BLK;
+    constexpr int DB = GV_D * N / GV_T;
     const int tr = threadIdx.x % N, td0 = (threadIdx.x / N) * DB;
     const double tmd = C->md[tr], tmv = C->minv[tr];
     const int tm = C->m[tr];
@@ -798,7 +800,7 @@ __global__ __launch_bounds__(BLK) void k_gemv_split(const DevConsts *__restrict_
     // step j's table seed, A's sign and exponent and A's run of residues are
     // fetched during step j - b3 (the run into registers), so step j's loads are
     // in flight while step j - b3 computes
-    RunStage<N, 1> stg;
+    RunStage<N, 1, GV_T> stg;
     int nb = 0, nxs = 0, nxe = 0;
     auto fetch = [&](int j) {
         nb = bxp[(size_t)tr * n + j];
@@ -816,7 +818,7 @@ __global__ __launch_bounds__(BLK) void k_gemv_split(const DevConsts *__restrict_
         stg.store(sd, A.d + (o0 + (long long)j * lda) * N, rows);
         int tv = mulmod(nb, t2, tmd, tmv);
         for (int d = 0; d < DB; d++) {
-            tp[(td0 + d) * SP + tr] = (double)tv;
+            tp[(td0 + d) * SP + tr] = tv;
             tv += tv;
             tv = tv >= tm ? tv - tm : tv;
         }
@@ -836,14 +838,14 @@ __global__ __launch_bounds__(BLK) void k_gemv_split(const DevConsts *__restrict_
                 d &= GV_D - 1;
             }
             const double sg = xs ? -1.0 : 1.0;
-            const double *f = tp + d * SP;
+            const int *f = tp + d * SP;
             // groups of 8 residues, fenced so that the compiler does not hoist
             // all LDS loads (and their registers) ahead of the arithmetic
 #pragma unroll
             for (int r0 = 0; r0 < N; r0 += 8) {
 #pragma unroll
                 for (int r = r0; r < r0 + 8; r++) {
-                    const double x = (double)xd[r], fr = f[r];
+                    const double x = (double)xd[r], fr = (double)f[r];
                     const double p = x * fr, pl = fma(x, fr, -p);
                     const double q = floor(p * C->minv[r]);
                     acc[r] = fma(sg, fma(-q, C->md[r], p) + pl, acc[r]);

@@ -143,7 +143,7 @@ struct Launch {
             }
         } else if (pass == 1) {
             if (split)
-                hipLaunchKernelGGL((k_gemv_split<N, L>), dim3((m + BLK - 1) / BLK, b3), bl, 0, st, c, m, n, A, lda, bx,
+                hipLaunchKernelGGL((k_gemv_split<N, L>), dim3((m + GV_T - 1) / GV_T, b3), dim3(GV_T), 0, st, c, m, n, A, lda, bx,
                                    bxp, EA, EB, b3, part);
             else if (!TR && b3 <= 65535)  // grid y limit
                 hipLaunchKernelGGL((k_gemv_rows<N, L, !SPLIT>), dim3((m + BLK / L - 1) / (BLK / L), b3), bl, 0, st, c,
