@@ -196,7 +196,9 @@ typedef struct {
     mp_array_t *y;        /* dot: second vector (read); gemv: y (updated) */
     mp_array_t *r;        /* dot: result (one number); gemv: unused */
 } mpres_batch_problem_t;
-/* Geometry of the persistent kernel. It never changes results; 0 = built-in choice. */
+/* Geometry of the persistent kernel. It never changes results; 0 = built-in choice (the
+ * gemv lanes of the context, 4 workgroups per CU, 128 rows: the autotuner's best on 4096
+ * dot/gemv problems at 16 moduli, mpres-blas_amd/tuning/batch_gfx950.json). */
 typedef struct {
     int lanes;            /* lanes per number (a value mpres_lanes_supported accepts) */
     int wg_per_cu;        /* persistent workgroups per compute unit */

@@ -687,9 +687,12 @@ struct mpres_batch {
 namespace {
 
 int batch_resolve(const mpres_ctx *c, const mpres_batch_tuning_t *t, mpres_batch_tuning_t &u, const Ops **ops) {
+    // built-in geometry: the autotuner's best on the batching workload at 16 moduli
+    // (mpres-blas_amd/tuning/batch_gfx950.json): the gemv lanes (4 at 16 moduli), 4
+    // workgroups per CU, 128 gemv rows per item
     u.lanes = (t && t->lanes) ? t->lanes : c->ops[OP_GEMV]->L;
-    u.wg_per_cu = (t && t->wg_per_cu) ? t->wg_per_cu : 2;
-    u.gemv_rows = (t && t->gemv_rows) ? t->gemv_rows : 64;
+    u.wg_per_cu = (t && t->wg_per_cu) ? t->wg_per_cu : 4;
+    u.gemv_rows = (t && t->gemv_rows) ? t->gemv_rows : 128;
     if (u.wg_per_cu < 0 || u.wg_per_cu > 32 || u.gemv_rows < 0)
         return fail(MPRES_ERR_ARG, "invalid batch tuning (%d workgroups per CU, %d rows)", u.wg_per_cu, u.gemv_rows);
     *ops = find_ops(c->host.n, u.lanes);
