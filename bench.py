@@ -35,10 +35,10 @@ PROF_GEMM_RESID = 6        # library timing slot of the gemm residue products
 # HBM bytes per launch of the step's main kernels on this workload, from rocprofv3
 # --pmc FETCH_SIZE and --pmc WRITE_SIZE (profiles/bench_424bit_counters.md), each
 # scaled by its reading for a 256 MiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0)
-TRAFFIC_BYTES = {"chain": 117.7e6, "resid": 436.6e6, "gemv": 2259.0e6}
+TRAFFIC_BYTES = {"chain": 99.4e6, "resid": 436.2e6, "gemv": 2261.0e6}
 # VALU instructions per launch of the gemm scalar chains (SQ_INSTS_VALU, same profile)
 # and the issue peak: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction
-CHAIN_VALU_INSTS = 4.95e8
+CHAIN_VALU_INSTS = 4.28e8
 VALU_ISSUE_PEAK = 614.4e9
 FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 (vector and matrix)
 INT8_PEAK_TOPS = 5000.0    # MI355X dense INT8 MFMA
