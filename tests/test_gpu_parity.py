@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 import exact
+from fractions import Fraction
 import gen
 import pyoracle as po
 
@@ -598,3 +599,26 @@ def test_batch_errors():
     assert empty.info()["items"] == 0
     empty.run()
     empty.close()
+
+
+def test_dot_headline_config():
+    """The CPU-runnable headline case: mp_dot of two vectors of n = 65536, u*2^e with e in
+    [-20, 20], 8 moduli (106 bits), order 256 x 64, inputs converted on the GPU: bit-exact
+    against the oracle, and its error against the exact sum within what the reference's
+    rounded chain allows (measured 1.08e-31, about 9 units of 2^-106)"""
+    c, o = ctx("8_2double"), orc("8_2double")
+    n = 65536
+    hx, hy = gen.uexp(1, n, -20, 20), gen.uexp(2, n, -20, 20)
+    x, y = o.set_d(hx, D), o.set_d(hy, D)
+    want = o.dot(x, y, 256, 64, D)
+    ax, ay, r = c.array_set_d(hx), c.array_set_d(hy), c.array(1)
+    c.set_lanes(mp.K_DOT, 0)
+    note("dot headline", n)
+    same(c.array_to_host(ax, n), x, "set_d x")
+    mp.check(c.L.mpres_dot(c.h, n, C.byref(ax), 1, C.byref(ay), 1, C.byref(r), 256, 64, None, 0, None))
+    got = c.array_to_host(r, 1)
+    same(got, want, "dot 65536")
+    m = [int(v) for v in o.moduli()]
+    ex = sum(Fraction(float(a)) * Fraction(float(b)) for a, b in zip(hx, hy))
+    err = abs(exact.mp_value(got[0], m) - ex) / abs(ex)
+    assert err < 2.0 ** -95, float(err)
