@@ -28,7 +28,8 @@ typedef int mf_v16i __attribute__((ext_vector_type(16)));
 
 constexpr int MF_T = LIMB_T;     // MFMA tile (rows, columns and k per instruction)
 constexpr int MF_LB = LIMB_BITS; // bits per limb (limbs are 0..127, valid int8)
-constexpr int MF_KCH = 8192;     // k per INT32 accumulation: NL 127^2 MF_KCH < 2^31 for NL <= 5
+constexpr int MF_KCH = 8192;     // k per INT32 accumulation chunk: NL 127^2 MF_KCH < 2^31 for NL <= 5
+constexpr int MF_KONE = 16384;   // k up to which one INT32 accumulation suffices (NL <= 5)
 constexpr int MF_NONE = -0x40000000;  // no element (bit bounds of empty rows, columns, bands)
 
 // bit length bound of an element from its interval upper bound: X < 2^(exp+1) M < 2^(exp+2+log2m)
@@ -144,7 +145,9 @@ __global__ __launch_bounds__(BLK) void k_mf_prescale(const DevConsts *__restrict
 // fa / fb (low-band products only): the bit bounds of the rows / columns' low bands; a
 // workgroup whose 64 rows (or columns) all lack a low band has nothing to add, and the
 // finish does not read its entries, so it returns at once.
-template <int N, int L, int NL>
+// CHUNKED (k > MF_KONE only): the INT32 accumulators are folded into residues every MF_KCH
+// k; that needs registers for the running residues, so shorter k takes the single pass.
+template <int N, int L, int NL, bool CHUNKED>
 __global__ __launch_bounds__(BLK) void k_mf_gemm(const DevConsts *__restrict__ C, int m, int n, int MP, int NP, int KP,
                                                  const signed char *__restrict__ PA, const signed char *__restrict__ PB,
                                                  const int *__restrict__ fa, const int *__restrict__ fb,
@@ -170,6 +173,35 @@ __global__ __launch_bounds__(BLK) void k_mf_gemm(const DevConsts *__restrict__ C
     const signed char *B = PB + (size_t)r * NL * pb + (size_t)(col0 + l % 32) * KP + 16 * (l / 32);
     const double md = C->md[r], mv = C->minv[r];
     mf_v16i acc[NG];
+    const int row = row0 + l % 32;
+    if constexpr (!CHUNKED) {
+#pragma unroll
+        for (int q = 0; q < NG; q++) acc[q] = mf_v16i{};
+        if (row0 < MP && col0 < NP) {
+            for (int k0 = 0; k0 < KP; k0 += MF_T) {
+                mf_v4i fa[NL], fb[NL];
+#pragma unroll
+                for (int a = 0; a < NL; a++) {
+                    fa[a] = *(const mf_v4i *)(A + a * pa + k0);
+                    fb[a] = *(const mf_v4i *)(B + a * pb + k0);
+                }
+#pragma unroll
+                for (int a = 0; a < NL; a++)
+#pragma unroll
+                    for (int b = 0; b < NL; b++)
+                        acc[a + b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fb[b], fa[a], acc[a + b], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int col = col0 + 8 * (q / 4) + 4 * (l / 32) + q % 4;
+            double v = 0.0;
+#pragma unroll
+            for (int g = NG - 1; g >= 0; g--) v = (double)modd(v * (double)(1 << MF_LB) + (double)acc[g][q], md, mv);
+            if (row < m && col < n) S[(size_t)r * m * n + (size_t)row + (size_t)m * col] = (int)v;
+        }
+        return;
+    }
     double res[16];  // residues of the chunks done so far
 #pragma unroll
     for (int q = 0; q < 16; q++) res[q] = 0.0;
@@ -203,7 +235,6 @@ __global__ __launch_bounds__(BLK) void k_mf_gemm(const DevConsts *__restrict__ C
             }
         }
     }
-    const int row = row0 + l % 32;
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         const int col = col0 + 8 * (q / 4) + 4 * (l / 32) + q % 4;

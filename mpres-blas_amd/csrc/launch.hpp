@@ -218,13 +218,7 @@ struct Launch {
                 hipLaunchKernelGGL((k_gemm_chain<N, L, VM>), dim3(grid_for(mn, 1)), bl, 0, st, c, ta, tb, m, n, k, A,
                                    lda, B, ldb, sums);
             } else if (pass == 2) {
-                const dim3 gg(NP / 64, MP / 64, N);
-                if (nl == 4)
-                    hipLaunchKernelGGL((k_mf_gemm<N, L, 4>), gg, bl, 0, st, c, m, n, MP, NP, KP, PA, PB, nullptr,
-                                       nullptr, S);
-                else
-                    hipLaunchKernelGGL((k_mf_gemm<N, L, 5>), gg, bl, 0, st, c, m, n, MP, NP, KP, PA, PB, nullptr,
-                                       nullptr, S);
+                mf_products(c, nl, m, n, MP, NP, KP, PA, PB, nullptr, nullptr, S, st);
                 hipLaunchKernelGGL((k_gemm_digits<N, L>), dim3(grid_for(mn, 1)), bl, 0, st, c, m, n, S, EA, EB, sums);
             } else if (pass == 3) {
                 hipLaunchKernelGGL((k_gemm_sum<N, L, true, true, VM>), gr, bl, 0, st, c, ta, tb, m, n, k, A, lda, B,
@@ -269,6 +263,20 @@ struct Launch {
         const size_t MPd = rup(m, 64), NPd = rup(n, 64), KPd = rup(k > 0 ? k : 1, MF_T);
         return (4 * mf_ints(m, n) + 255) / 256 * 256 + 2 * (size_t)N * nl * (MPd + NPd) * KPd;
     }
+    // residue products of limb planes on INT8 MFMA (k_mf_gemm), one launch
+    static void mf_products(const DevConsts *c, int nl, int m, int n, int MP, int NP, int KP, const signed char *pa,
+                            const signed char *pb, const int *fa, const int *fb, int *S, hipStream_t st) {
+        const dim3 gg(NP / 64, MP / 64, N), bl(BLK);
+        const bool ch = KP > MF_KONE;
+        if (nl == 4 && !ch)
+            hipLaunchKernelGGL((k_mf_gemm<N, L, 4, false>), gg, bl, 0, st, c, m, n, MP, NP, KP, pa, pb, fa, fb, S);
+        else if (nl == 4)
+            hipLaunchKernelGGL((k_mf_gemm<N, L, 4, true>), gg, bl, 0, st, c, m, n, MP, NP, KP, pa, pb, fa, fb, S);
+        else if (!ch)
+            hipLaunchKernelGGL((k_mf_gemm<N, L, 5, false>), gg, bl, 0, st, c, m, n, MP, NP, KP, pa, pb, fa, fb, S);
+        else
+            hipLaunchKernelGGL((k_mf_gemm<N, L, 5, true>), gg, bl, 0, st, c, m, n, MP, NP, KP, pa, pb, fa, fb, S);
+    }
     static hipError_t gemm_mfma(const DevConsts *c, int bits, int log2m, int ta, int tb, int m, int n, int k,
                                 SoA alpha, SoA A, int lda, SoA B, int ldb, SoA beta, SoA Cm, int ldc, char *ws,
                                 hipStream_t st) {
@@ -309,16 +317,11 @@ struct Launch {
                            k, KP, nl, ta, A, lda, EH, EL, PA);
         hipLaunchKernelGGL((k_mf_prescale<N, L, SoA, 1>), dim3(grid_for((long long)NP * KP, 1)), bl, 0, st, c, n, NP,
                            k, KP, nl, tb, B, ldb, EH + m, EL + m, PB);
-        const dim3 gg(NP / 64, MP / 64, N);
         for (int p = 0; p < 4; p++) {
             const int ba = p >> 1, bb = p & 1;
             const signed char *pa = PA + (size_t)ba * bandA, *pb = PB + (size_t)bb * bandB;
             const int *fa = ba ? nullptr : BL, *fb = bb ? nullptr : BL + m;
-            int *Sp = S + (size_t)p * N * m * n;
-            if (nl == 4)
-                hipLaunchKernelGGL((k_mf_gemm<N, L, 4>), gg, bl, 0, st, c, m, n, MP, NP, KP, pa, pb, fa, fb, Sp);
-            else
-                hipLaunchKernelGGL((k_mf_gemm<N, L, 5>), gg, bl, 0, st, c, m, n, MP, NP, KP, pa, pb, fa, fb, Sp);
+            mf_products(c, nl, m, n, MP, NP, KP, pa, pb, fa, fb, S + (size_t)p * N * m * n, st);
         }
         hipLaunchKernelGGL((k_mf_finish<N, L>), dim3(grid_for((long long)m * n, L)), bl, 0, st, c, m, n, lk, S, EH, BH,
                            EL, BL, EH + m, BH + m, EL + m, BL + m, alpha, beta, Cm, ldc);

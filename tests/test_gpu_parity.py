@@ -622,3 +622,23 @@ def test_dot_headline_config():
     ex = sum(Fraction(float(a)) * Fraction(float(b)) for a, b in zip(hx, hy))
     err = abs(exact.mp_value(got[0], m) - ex) / abs(ex)
     assert err < 2.0 ** -95, float(err)
+
+
+@pytest.mark.parametrize("name", ["16_4double", "32_8double"])
+def test_gemm_long_k(name):
+    """k beyond one INT32 accumulation of the MFMA residue products (k > 16384): the
+    chunked kernel folds its accumulators every 8192 k; bit-exact against the oracle"""
+    c, o = ctx(name), orc(name)
+    m, n, k = 5, 3, 16500
+    A = o.set_d(gen.uexp(131, m * k, -30, 30), D)
+    B = o.set_d(gen.uexp(132, k * n, -30, 30), D)
+    Cm = o.set_d(gen.uexp(133, m * n, -30, 30), D)
+    alpha, beta = o.set_d([0.75], D), o.set_d([-2.0], D)
+    want = o.gemm(m, n, k, alpha, A, m, B, k, beta, Cm, m, D)
+    c.set_lanes(mp.K_GEMM, 0)
+    note("gemm long k", name, m, n, k)
+    aA, aB, aC = c.array_from_host(A), c.array_from_host(B), c.array_from_host(Cm)
+    aa, ab = c.array_from_host(alpha), c.array_from_host(beta)
+    mp.check(c.L.mpres_gemm(c.h, mp.NO_TRANS, mp.NO_TRANS, m, n, k, C.byref(aa), C.byref(aA), m, C.byref(aB), k,
+                            C.byref(ab), C.byref(aC), m, None, 0, None))
+    same(c.array_to_host(aC, m * n), want, f"gemm {name} k={k}")
