@@ -113,27 +113,44 @@ __device__ __forceinline__ Align align_sc(int mp_j, int xs, int xe, Er xv0, Er x
     a.y1 = Er{yv1.frac * (double)a.fy, yv1.exp + a.theta};
     return a;
 }
+// The constants of the scalar steps, read once into registers by the kernels that run
+// long scalar chains (so that no step waits on a reload from the constants block).
+struct ScC {
+    int mp_h, mp_j, uue, ule;
+    double uuf, ulf, uu_rh, uu_rl, ul_rh, ul_rl;
+};
+__device__ __forceinline__ ScC sc_consts(const DevConsts *C) {
+    return ScC{C->mp_h, C->mp_j, C->unit_upp.exp, C->unit_low.exp, C->unit_upp.frac, C->unit_low.frac,
+               C->uu_rh, C->uu_rl, C->ul_rh, C->ul_rl};
+}
+__device__ __forceinline__ bool sc_needs_round(const ScC &K, const Er &ev1) {
+    return ev1.frac != 0 && ev1.exp >= K.mp_h;
+}
 __device__ __forceinline__ bool sc_needs_round(const DevConsts *C, const Er &ev1) {
     return ev1.frac != 0 && ev1.exp >= C->mp_h;
 }
 // scalar part of cuda::mp_mul (mp_mul_common, src/arith/mul.cuh:35-42)
-__device__ __forceinline__ void sc_mul(const DevConsts *C, Sc &r, int xs, int xe, Er x0, Er x1, int ys, int ye, Er y0,
+__device__ __forceinline__ void sc_mul(const ScC &K, Sc &r, int xs, int xe, Er x0, Er x1, int ys, int ye, Er y0,
                                        Er y1) {
     r.exp = xe + ye;
     r.sign = xs ^ ys;
-    r.ev0 = er_mdk_rd(x0, y0, C->unit_upp.frac, C->unit_upp.exp, C->uu_rh, C->uu_rl);
-    r.ev1 = er_mdk_ru(x1, y1, C->unit_low.frac, C->unit_low.exp, C->ul_rh, C->ul_rl);
+    r.ev0 = er_mdk_rd(x0, y0, K.uuf, K.uue, K.uu_rh, K.uu_rl);
+    r.ev1 = er_mdk_ru(x1, y1, K.ulf, K.ule, K.ul_rh, K.ul_rl);
+}
+__device__ __forceinline__ void sc_mul(const DevConsts *C, Sc &r, int xs, int xe, Er x0, Er x1, int ys, int ye, Er y0,
+                                       Er y1) {
+    sc_mul(sc_consts(C), r, xs, xe, x0, x1, ys, ye, y0, y1);
 }
 // acc = acc + x*y on the scalar fields, for chains whose every step is exact:
 // false when the product or the sum needs rounding, the sum's interval
 // straddles zero, or the alignment drops a nonzero operand. The decisions are
 // those of the lane-group step (Grp::step_tail).
-__device__ __forceinline__ bool sc_fma_step(const DevConsts *C, Sc &acc, int xs, int xe, Er x0, Er x1, int ys, int ye,
+__device__ __forceinline__ bool sc_fma_step(const ScC &C, Sc &acc, int xs, int xe, Er x0, Er x1, int ys, int ye,
                                             Er y0, Er y1) {
     Sc p;
     sc_mul(C, p, xs, xe, x0, x1, ys, ye, y0, y1);
     if (sc_needs_round(C, p.ev1)) return false;
-    const Align a = align_sc(C->mp_j, acc.sign, acc.exp, acc.ev0, acc.ev1, p.sign, p.exp, p.ev0, p.ev1);
+    const Align a = align_sc(C.mp_j, acc.sign, acc.exp, acc.ev0, acc.ev1, p.sign, p.exp, p.ev0, p.ev1);
     if ((!a.nzx && acc.ev1.frac != 0) || (!a.nzy && p.ev1.frac != 0)) return false;
     const Er e0 = er_add_dir(a.sx ? a.x1 : a.x0, a.sy ? a.y1 : a.y0, false);
     const Er e1 = er_add_dir(a.sx ? a.x0 : a.x1, a.sy ? a.y0 : a.y1, true);

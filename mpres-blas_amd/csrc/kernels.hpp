@@ -650,6 +650,7 @@ __global__ __launch_bounds__(BLK) void k_gemm_chain(const DevConsts *__restrict_
     const long long e = (long long)blockIdx.x * BLK + threadIdx.x;
     if (e >= (long long)m * n) return;
     const int row = (int)(e % m), col = (int)(e / m);
+    const ScC K = sc_consts(C);
     Sc acc{0, 0, Er{0.0, 0}, Er{0.0, 0}};
     bool ok = true;
     for (int i = 0; i < k && ok; i++) {
@@ -658,7 +659,7 @@ __global__ __launch_bounds__(BLK) void k_gemm_chain(const DevConsts *__restrict_
         Er x0, x1, y0, y1;
         scalars_of<N>(A, ia, xs, xe, x0, x1);
         scalars_of<N>(B, ib, ys, ye, y0, y1);
-        ok = sc_fma_step(C, acc, xs, xe, x0, x1, ys, ye, y0, y1);
+        ok = sc_fma_step(K, acc, xs, xe, x0, x1, ys, ye, y0, y1);
     }
     if (!ok) {
         *sums.sign(e) = REDO;
@@ -743,6 +744,7 @@ __global__ __launch_bounds__(BLK) void k_gemv_chain(const DevConsts *__restrict_
     const long long g = (long long)blockIdx.x * BLK + threadIdx.x;
     if (g >= (long long)m * b3) return;
     const int o = (int)(g % m), t = (int)(g / m);
+    const ScC K = sc_consts(C);
     Sc sc{0, 0, Er{0.0, 0}, Er{0.0, 0}};
     bool ok = true;
     for (int j = t; j < n && ok; j += b3) {
@@ -750,7 +752,7 @@ __global__ __launch_bounds__(BLK) void k_gemv_chain(const DevConsts *__restrict_
         Er x0, x1, y0, y1;
         scalars_of<N>(A, o + (long long)j * lda, xs, xe, x0, x1);
         scalars_of<N>(bx, j, ys, ye, y0, y1);
-        ok = sc_fma_step(C, sc, xs, xe, x0, x1, ys, ye, y0, y1);
+        ok = sc_fma_step(K, sc, xs, xe, x0, x1, ys, ye, y0, y1);
     }
     const long long v = (long long)o * b3 + t;
     if (!ok) {
