@@ -126,9 +126,6 @@ __device__ __forceinline__ ScC sc_consts(const DevConsts *C) {
 __device__ __forceinline__ bool sc_needs_round(const ScC &K, const Er &ev1) {
     return ev1.frac != 0 && ev1.exp >= K.mp_h;
 }
-__device__ __forceinline__ bool sc_needs_round(const DevConsts *C, const Er &ev1) {
-    return ev1.frac != 0 && ev1.exp >= C->mp_h;
-}
 // scalar part of cuda::mp_mul (mp_mul_common, src/arith/mul.cuh:35-42)
 __device__ __forceinline__ void sc_mul(const ScC &K, Sc &r, int xs, int xe, Er x0, Er x1, int ys, int ye, Er y0,
                                        Er y1) {
@@ -137,10 +134,7 @@ __device__ __forceinline__ void sc_mul(const ScC &K, Sc &r, int xs, int xe, Er x
     r.ev0 = er_mdk_rd(x0, y0, K.uuf, K.uue, K.uu_rh, K.uu_rl);
     r.ev1 = er_mdk_ru(x1, y1, K.ulf, K.ule, K.ul_rh, K.ul_rl);
 }
-__device__ __forceinline__ void sc_mul(const DevConsts *C, Sc &r, int xs, int xe, Er x0, Er x1, int ys, int ye, Er y0,
-                                       Er y1) {
-    sc_mul(sc_consts(C), r, xs, xe, x0, x1, ys, ye, y0, y1);
-}
+
 // acc = acc + x*y on the scalar fields, for chains whose every step is exact:
 // false when the product or the sum needs rounding, the sum's interval
 // straddles zero, or the alignment drops a nonzero operand. The decisions are
@@ -180,8 +174,9 @@ struct Grp {
     Core<N, L> k;
     double cmd[CR], cmv[CR];
     int cmi[CR];
+    ScC K;  // constants of the scalar steps, read once
 
-    __device__ __forceinline__ explicit Grp(const DevConsts *c) {
+    __device__ __forceinline__ explicit Grp(const DevConsts *c) : K(sc_consts(c)) {
         k.C = c;
         k.P2 = c->pow2;
         k.lane = (int)(__lane_id() & (L - 1));
@@ -363,7 +358,7 @@ struct Grp {
     __device__ __forceinline__ void mul_scalars(MP<R> &res, int xs, int xe, Er x0, Er x1, int ys, int ye, Er y0,
                                                 Er y1) const {
         Sc p;
-        sc_mul(k.C, p, xs, xe, x0, x1, ys, ye, y0, y1);
+        sc_mul(K, p, xs, xe, x0, x1, ys, ye, y0, y1);
         res.sign = p.sign;
         res.exp = p.exp;
         res.ev0 = p.ev0;
