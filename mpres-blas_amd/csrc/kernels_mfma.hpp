@@ -141,18 +141,26 @@ __global__ __launch_bounds__(BLK) void k_mf_prescale(const DevConsts *__restrict
 // Limb products per modulus on MFMA. Workgroup: 4 waves, a 64 x 64 block of C
 // (each wave a 32 x 32 tile); blockIdx.z = modulus. Operands are swapped
 // (B^T as the first, A^T as the second) so that a lane holds one row of C.
+// Per k-step of 32 the workgroup's 64-row panel of A and 64-column panel of B (all
+// limbs) pass through LDS once, double-buffered: the next step's pieces are loaded
+// into registers while this step's products run, then stored to the other buffer.
+// Each wave reads its fragments from LDS (consecutive lanes, consecutive 16 bytes).
 // Output: S[r][e], e = row + m * col, the residue of the exact sum mod m_r.
 // fa / fb (low-band products only): the bit bounds of the rows / columns' low bands; a
 // workgroup whose 64 rows (or columns) all lack a low band has nothing to add, and the
 // finish does not read its entries, so it returns at once.
 // CHUNKED (k > MF_KONE only): the INT32 accumulators are folded into residues every MF_KCH
 // k; that needs registers for the running residues, so shorter k takes the single pass.
+// The grid covers MP x NP exactly in 64-blocks (MP, NP multiples of 64, KP of 32).
 template <int N, int L, int NL, bool CHUNKED>
 __global__ __launch_bounds__(BLK) void k_mf_gemm(const DevConsts *__restrict__ C, int m, int n, int MP, int NP, int KP,
                                                  const signed char *__restrict__ PA, const signed char *__restrict__ PB,
                                                  const int *__restrict__ fa, const int *__restrict__ fb,
                                                  int *__restrict__ S) {
-    constexpr int NG = 2 * NL - 1;  // limb shift groups
+    constexpr int NG = 2 * NL - 1;               // limb shift groups
+    constexpr int ITEMS = NL * 128;              // 16-byte pieces of one 64 x 32 panel, all limbs
+    constexpr int PER = (ITEMS + BLK - 1) / BLK; // pieces per thread
+    __shared__ mf_v4i sa[2][ITEMS], sb[2][ITEMS];  // [buffer][(limb * 64 + row) * 2 + half]
     if (fa || fb) {
         __shared__ int live[2];
         if (threadIdx.x < 2) live[threadIdx.x] = 0;
@@ -167,78 +175,94 @@ __global__ __launch_bounds__(BLK) void k_mf_gemm(const DevConsts *__restrict__ C
     }
     const int r = blockIdx.z;
     const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-    const int row0 = blockIdx.y * 64 + 32 * (w & 1), col0 = blockIdx.x * 64 + 32 * (w >> 1);
+    const int rb = blockIdx.y * 64, cb = blockIdx.x * 64;
+    const int row0 = rb + 32 * (w & 1), col0 = cb + 32 * (w >> 1);
     const size_t pa = (size_t)MP * KP, pb = (size_t)NP * KP;
-    const signed char *A = PA + (size_t)r * NL * pa + (size_t)(row0 + l % 32) * KP + 16 * (l / 32);
-    const signed char *B = PB + (size_t)r * NL * pb + (size_t)(col0 + l % 32) * KP + 16 * (l / 32);
+    const signed char *Ab = PA + (size_t)r * NL * pa + (size_t)rb * KP;
+    const signed char *Bb = PB + (size_t)r * NL * pb + (size_t)cb * KP;
+    mf_v4i ga[PER], gb[PER];
+    // piece q: limb q >> 7, row (column) (q >> 1) & 63, half q & 1 of the k-step's 32 bytes
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int p = 0; p < PER; p++) {
+            const int q = threadIdx.x + p * BLK;
+            if (q < ITEMS) {
+                const size_t o = (size_t)((q >> 1) & 63) * KP + k0 + 16 * (q & 1);
+                ga[p] = *(const mf_v4i *)(Ab + (size_t)(q >> 7) * pa + o);
+                gb[p] = *(const mf_v4i *)(Bb + (size_t)(q >> 7) * pb + o);
+            }
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int p = 0; p < PER; p++) {
+            const int q = threadIdx.x + p * BLK;
+            if (q < ITEMS) {
+                sa[buf][q] = ga[p];
+                sb[buf][q] = gb[p];
+            }
+        }
+    };
+    // this lane's fragment pieces: A row 32 (w & 1) + l % 32, B column 32 (w >> 1) + l % 32, half l / 32
+    const int ia = (32 * (w & 1) + l % 32) * 2 + l / 32, ib = (32 * (w >> 1) + l % 32) * 2 + l / 32;
     const double md = C->md[r], mv = C->minv[r];
     mf_v16i acc[NG];
+#pragma unroll
+    for (int q = 0; q < NG; q++) acc[q] = mf_v16i{};
+    double res[CHUNKED ? 16 : 1];  // residues of the chunks done so far
+#pragma unroll
+    for (int q = 0; q < (CHUNKED ? 16 : 1); q++) res[q] = 0.0;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < KP; k0 += MF_T) {
+        const bool more = k0 + MF_T < KP;
+        if (more) gload(k0 + MF_T);
+        mf_v4i xa[NL], xb[NL];
+#pragma unroll
+        for (int a = 0; a < NL; a++) {
+            xa[a] = sa[buf][a * 128 + ia];
+            xb[a] = sb[buf][a * 128 + ib];
+        }
+#pragma unroll
+        for (int a = 0; a < NL; a++)
+#pragma unroll
+            for (int b = 0; b < NL; b++)
+                acc[a + b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(xb[b], xa[a], acc[a + b], 0, 0, 0);
+        if constexpr (CHUNKED) {
+            if ((k0 + MF_T) % MF_KCH == 0 || !more) {
+                // the chunk's limb groups recombined (Horner, exact below 2^53) and added mod m
+#pragma unroll
+                for (int q = 0; q < 16; q++) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int g = NG - 1; g >= 0; g--)
+                        v = (double)modd(v * (double)(1 << MF_LB) + (double)acc[g][q], md, mv);
+                    v += res[q];
+                    res[q] = v >= md ? v - md : v;
+                }
+#pragma unroll
+                for (int q = 0; q < NG; q++) acc[q] = mf_v16i{};
+            }
+        }
+        if (more) sstore(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
     const int row = row0 + l % 32;
-    if constexpr (!CHUNKED) {
-#pragma unroll
-        for (int q = 0; q < NG; q++) acc[q] = mf_v16i{};
-        if (row0 < MP && col0 < NP) {
-            for (int k0 = 0; k0 < KP; k0 += MF_T) {
-                mf_v4i fa[NL], fb[NL];
-#pragma unroll
-                for (int a = 0; a < NL; a++) {
-                    fa[a] = *(const mf_v4i *)(A + a * pa + k0);
-                    fb[a] = *(const mf_v4i *)(B + a * pb + k0);
-                }
-#pragma unroll
-                for (int a = 0; a < NL; a++)
-#pragma unroll
-                    for (int b = 0; b < NL; b++)
-                        acc[a + b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fb[b], fa[a], acc[a + b], 0, 0, 0);
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const int col = col0 + 8 * (q / 4) + 4 * (l / 32) + q % 4;
-            double v = 0.0;
-#pragma unroll
-            for (int g = NG - 1; g >= 0; g--) v = (double)modd(v * (double)(1 << MF_LB) + (double)acc[g][q], md, mv);
-            if (row < m && col < n) S[(size_t)r * m * n + (size_t)row + (size_t)m * col] = (int)v;
-        }
-        return;
-    }
-    double res[16];  // residues of the chunks done so far
-#pragma unroll
-    for (int q = 0; q < 16; q++) res[q] = 0.0;
-    if (row0 < MP && col0 < NP) {
-        for (int kc = 0; kc < KP; kc += MF_KCH) {
-            const int ke = (kc + MF_KCH) < KP ? kc + MF_KCH : KP;
-#pragma unroll
-            for (int q = 0; q < NG; q++) acc[q] = mf_v16i{};
-            for (int k0 = kc; k0 < ke; k0 += MF_T) {
-                mf_v4i fa[NL], fb[NL];
-#pragma unroll
-                for (int a = 0; a < NL; a++) {
-                    fa[a] = *(const mf_v4i *)(A + a * pa + k0);
-                    fb[a] = *(const mf_v4i *)(B + a * pb + k0);
-                }
-#pragma unroll
-                for (int a = 0; a < NL; a++)
-#pragma unroll
-                    for (int b = 0; b < NL; b++)
-                        acc[a + b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fb[b], fa[a], acc[a + b], 0, 0, 0);
-            }
-            // the chunk's limb groups recombined (Horner, exact below 2^53) and added mod m
-#pragma unroll
-            for (int q = 0; q < 16; q++) {
-                double v = 0.0;
-#pragma unroll
-                for (int g = NG - 1; g >= 0; g--)
-                    v = (double)modd(v * (double)(1 << MF_LB) + (double)acc[g][q], md, mv);
-                v += res[q];
-                res[q] = v >= md ? v - md : v;
-            }
-        }
-    }
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         const int col = col0 + 8 * (q / 4) + 4 * (l / 32) + q % 4;
-        if (row < m && col < n) S[(size_t)r * m * n + (size_t)row + (size_t)m * col] = (int)res[q];
+        double v;
+        if constexpr (CHUNKED) {
+            v = res[q];
+        } else {
+            v = 0.0;
+#pragma unroll
+            for (int g = NG - 1; g >= 0; g--) v = (double)modd(v * (double)(1 << MF_LB) + (double)acc[g][q], md, mv);
+        }
+        if (row < m && col < n) S[(size_t)r * m * n + (size_t)row + (size_t)m * col] = (int)v;
     }
 }
 
