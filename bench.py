@@ -35,7 +35,7 @@ PROF_GEMM_RESID = 6        # library timing slot of the gemm residue products
 # HBM bytes per launch of the step's main kernels on this workload, from rocprofv3
 # --pmc FETCH_SIZE and --pmc WRITE_SIZE (profiles/bench_424bit_counters.md), each
 # scaled by its reading for a 256 MiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0)
-TRAFFIC_BYTES = {"chain": 99.4e6, "resid": 436.2e6, "gemv": 2261.0e6}
+TRAFFIC_BYTES = {"chain": 104.2e6, "resid": 435.9e6, "gemv": 2260.7e6}
 # VALU instructions per launch of the gemm scalar chains (SQ_INSTS_VALU, same profile)
 # and the issue peak: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction
 CHAIN_VALU_INSTS = 4.28e8
