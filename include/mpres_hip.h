@@ -131,7 +131,8 @@ int mpres_gemv(mpres_ctx_t *ctx, int trans, int m, int n, const mp_array_t *alph
                const mp_array_t *x, int incx, const mp_array_t *beta, mp_array_t *y, int incy, int block3, void *ws,
                size_t ws_bytes, void *stream);
 /* C = alpha*A*B + beta*C ; cuda::mp_gemm<...>, src/blas/gemm.cuh:69-167 (no-trans only, as the reference).
- * Workspace: the m*n dot-product sums as mp_float_t (ws NULL: the context's pool). */
+ * Workspace: the m*n dot-product sums as mp_float_t (ws NULL: the context's pool).
+ * A->digits and B->digits must be 16-byte aligned (device allocations are), else MPRES_ERR_ARG. */
 size_t mpres_gemm_workspace(const mpres_ctx_t *ctx, int m, int n, int k);
 int mpres_gemm(mpres_ctx_t *ctx, int transa, int transb, int m, int n, int k, const mp_array_t *alpha,
                const mp_array_t *A, int lda, const mp_array_t *B, int ldb, const mp_array_t *beta, mp_array_t *C,
@@ -145,7 +146,8 @@ int mpres_gemm(mpres_ctx_t *ctx, int transa, int transb, int m, int n, int k, co
  * product and partial sum). Rows or columns whose aligned magnitudes exceed the bit budget that
  * keeps |sum| < M/4 are truncated by the excess bits (tile alignment rounding). The epilogue
  * is (sum*alpha) + (C*beta) with mp_mul / mp_add. Results are not bit-identical to mpres_gemm.
- * Requires 0 <= k <= MPRES_MFMA_MAX_K; transposes allowed. */
+ * Requires 0 <= k <= MPRES_MFMA_MAX_K; transposes allowed; A->digits and B->digits 16-byte
+ * aligned, as for mpres_gemm. */
 #define MPRES_MFMA_MAX_K 16384
 size_t mpres_gemm_mfma_workspace(const mpres_ctx_t *ctx, int m, int n, int k);
 int mpres_gemm_mfma(mpres_ctx_t *ctx, int transa, int transb, int m, int n, int k, const mp_array_t *alpha,

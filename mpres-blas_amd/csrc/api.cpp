@@ -91,6 +91,9 @@ size_t mpsize(const mpres_ctx *c) { return (size_t)(4 * c->host.n + 40); }
 
 SoA soa_of(const mp_array_t *a) { return soa_view(a); }
 
+// the split and precision-form gemm kernels read an element's residues as 16-byte vectors
+bool digits_aligned(const mp_array_t *a) { return ((uintptr_t)a->digits & 15) == 0; }
+
 // workspace: caller's buffer or the context pool
 int get_ws(mpres_ctx *c, void *ws, size_t ws_bytes, size_t need, char **out) {
     if (need == 0) {
@@ -562,6 +565,8 @@ int mpres_gemm(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const mp_arr
         return fail(MPRES_ERR_ARG, "invalid leading dimension");  // :82-96
     if (ta != MPRES_NO_TRANS || tb != MPRES_NO_TRANS)
         return fail(MPRES_ERR_UNSUPPORTED, "only non-transposed matrices are supported");  // :114-125
+    if (!digits_aligned(A) || !digits_aligned(B))
+        return fail(MPRES_ERR_ARG, "digits of A and B must be 16-byte aligned");
     char *w;
     int rc = get_ws(c, ws, wsb, mpres_gemm_workspace(c, m, n, k), &w);
     if (rc) return rc;
@@ -588,6 +593,8 @@ int mpres_gemm_mfma(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const m
     if (k < 0 || lda < std::max(1, tra ? k : m) || ldb < std::max(1, trb ? n : k) || ldc < std::max(1, m))
         return fail(MPRES_ERR_ARG, "invalid dimensions");
     if (k > MPRES_MFMA_MAX_K) return fail(MPRES_ERR_UNSUPPORTED, "k = %d exceeds %d", k, MPRES_MFMA_MAX_K);
+    if (!digits_aligned(A) || !digits_aligned(B))
+        return fail(MPRES_ERR_ARG, "digits of A and B must be 16-byte aligned");
     char *w;
     int rc = get_ws(c, ws, wsb, mpres_gemm_mfma_workspace(c, m, n, k), &w);
     if (rc) return rc;

@@ -588,10 +588,22 @@ __global__ __launch_bounds__(BLK) void k_gemm_prescale(const DevConsts *__restri
 
 // Residue planes of the split-form gemm for the INT8 MFMA residue products
 // (k_mf_gemm): the values of k_gemm_prescale, s x_r 2^(e - emin[j]) mod m_r (signed
-// as m_r - v), split into LIMB_BITS-bit limbs, as int8 planes P[r][limb][j][kp] with
-// j < JP (rows of op(A) or columns of op(B), padded with zeros to a multiple of 64)
-// and kp < KP (k padded with zeros to a multiple of LIMB_T).
+// as m_r - v), split into LIMB_BITS-bit limbs, as int8 planes P[r][limb] with j < JP
+// (rows of op(A) or columns of op(B), padded with zeros to a multiple of 64) and kp < KP
+// (k padded with zeros to a multiple of LIMB_T). A plane is k-step major,
+// [kp / LIMB_T][j][kp % LIMB_T], so that one k-step of a panel of rows is one contiguous
+// run. k_gemm_limbs: thread g = j * KP + kp reads element (j, kp) (consecutive threads,
+// consecutive k) and writes plane position limb_pos(j, kp, JP); k_mf_prescale: thread g
+// writes plane position g, element limb_elem(g).
 constexpr int LIMB_BITS = 7, LIMB_T = 32;
+__device__ __forceinline__ long long limb_pos(int j, int kp, int JP) {
+    return ((long long)(kp / LIMB_T) * JP + j) * LIMB_T + kp % LIMB_T;
+}
+__device__ __forceinline__ void limb_elem(long long g, int JP, int &j, int &kp) {
+    const long long rest = g / LIMB_T;
+    j = (int)(rest % JP);
+    kp = (int)(rest / JP) * LIMB_T + (int)(g % LIMB_T);
+}
 template <int N, int L, class VM, int SIDE>
 __global__ __launch_bounds__(BLK) void k_gemm_limbs(const DevConsts *__restrict__ C, int cnt, int JP, int k, int KP,
                                                     int nl, int t, VM X, int ld, const int *__restrict__ emin,
@@ -600,10 +612,11 @@ __global__ __launch_bounds__(BLK) void k_gemm_limbs(const DevConsts *__restrict_
     const long long g = (long long)blockIdx.x * BLK + threadIdx.x;
     if (g >= (long long)JP * KP) return;
     const int kp = (int)(g % KP), j = (int)(g / KP);
+    const long long pos = limb_pos(j, kp, JP);
     const size_t plane = (size_t)JP * KP;
     if (j >= cnt || kp >= k) {
         for (int r = 0; r < N; r++)
-            for (int a = 0; a < nl; a++) P[((size_t)r * nl + a) * plane + g] = 0;
+            for (int a = 0; a < nl; a++) P[((size_t)r * nl + a) * plane + pos] = 0;
         return;
     }
     const long long idx = op_index<SIDE>(t, j, kp, ld);
@@ -635,7 +648,7 @@ __global__ __launch_bounds__(BLK) void k_gemm_limbs(const DevConsts *__restrict_
         int v = dig[r] ? mulmod(dig[r], pow2mod(C, d, r), C->md[r], C->minv[r]) : 0;
         v = (s && v) ? m - v : v;
         for (int a = 0; a < nl; a++)
-            P[((size_t)r * nl + a) * plane + g] = (signed char)((v >> (LIMB_BITS * a)) & ((1 << LIMB_BITS) - 1));
+            P[((size_t)r * nl + a) * plane + pos] = (signed char)((v >> (LIMB_BITS * a)) & ((1 << LIMB_BITS) - 1));
     }
 }
 // scalar chain of every entry (one thread per entry): the sum's sign, exponent and

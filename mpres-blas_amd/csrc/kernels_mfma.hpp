@@ -89,22 +89,28 @@ __global__ __launch_bounds__(BLK) void k_mf_band(int cnt, int half, const int *_
 }
 
 // aligned, signed residues of op(A) rows (SIDE 0) or op(B) columns (SIDE 1) as int8 limb
-// planes P[band][r][limb][j][kp] (band 1 high, 0 low; kp = 0 .. KP-1 and j = 0 .. JP-1 zero
-// padded); an element goes to one band, the other band's planes hold 0 there
+// planes P[band][r][limb] (band 1 high, 0 low; laid out as the planes of k_gemm_limbs, zero
+// padded); an element goes to one band, the other band's planes hold 0 there. A row or
+// column without a low band (bl[j] == MF_NONE) leaves its low-band planes unwritten: no
+// entry of it reads a low-band product (the finish takes only the band pairs present),
+// and its padding beyond cnt is never read either.
 template <int N, int L, class VM, int SIDE>
 __global__ __launch_bounds__(BLK) void k_mf_prescale(const DevConsts *__restrict__ C, int cnt, int JP, int k, int KP,
                                                      int nl, int t, VM X, int ld, const int *__restrict__ eh,
-                                                     const int *__restrict__ el, signed char *__restrict__ P) {
+                                                     const int *__restrict__ el, const int *__restrict__ bl,
+                                                     signed char *__restrict__ P) {
     if constexpr (std::is_same<VM, SoA>::value) X = rd(X);
     const long long g = (long long)blockIdx.x * BLK + threadIdx.x;
     if (g >= (long long)JP * KP) return;
-    const int kp = (int)(g % KP), j = (int)(g / KP);
+    int j, kp;
+    limb_elem(g, JP, j, kp);
     const size_t plane = (size_t)JP * KP, band = (size_t)N * nl * plane;
     signed char *PL = P, *PH = P + band;
+    const bool low = j < cnt && bl[j] != MF_NONE;  // this row / column has a low band
     if (j >= cnt || kp >= k) {
         for (int r = 0; r < N; r++)
             for (int a = 0; a < nl; a++) {
-                PL[((size_t)r * nl + a) * plane + g] = 0;
+                if (low) PL[((size_t)r * nl + a) * plane + g] = 0;
                 PH[((size_t)r * nl + a) * plane + g] = 0;
             }
         return;
@@ -115,10 +121,24 @@ __global__ __launch_bounds__(BLK) void k_mf_prescale(const DevConsts *__restrict
     scalars_of<N>(X, idx, s, e, lo, up);
     const bool high = e >= eh[j];
     signed char *Pw = high ? PH : PL, *Pz = high ? PL : PH;
+    const bool wz = !high || low;  // the other band's plane needs its zero
     const int d = e - (high ? eh[j] : el[j]);
-    const int *dig = digits_of<N>(X, idx);
+    // the element's residues at once (SoA records are 16-byte aligned)
     int x[N];
-    for (int r = 0; r < N; r++) x[r] = dig[r];
+    if constexpr (std::is_same<VM, SoA>::value) {
+        const int4 *p4 = (const int4 *)digits_of<N>(X, idx);
+#pragma unroll
+        for (int q = 0; q < N / 4; q++) {
+            const int4 v4 = p4[q];
+            x[4 * q] = v4.x;
+            x[4 * q + 1] = v4.y;
+            x[4 * q + 2] = v4.z;
+            x[4 * q + 3] = v4.w;
+        }
+    } else {
+        const int *dig = digits_of<N>(X, idx);
+        for (int r = 0; r < N; r++) x[r] = dig[r];
+    }
     if (up.frac == 0 || -d > C->log2m + 1) {
         // zero, or truncated to zero (X < M < 2^(log2m + 1))
         for (int r = 0; r < N; r++) x[r] = 0;
@@ -133,7 +153,7 @@ __global__ __launch_bounds__(BLK) void k_mf_prescale(const DevConsts *__restrict
         v = (s && v) ? m - v : v;
         for (int a = 0; a < nl; a++) {
             Pw[((size_t)r * nl + a) * plane + g] = (signed char)((v >> (MF_LB * a)) & 127);
-            Pz[((size_t)r * nl + a) * plane + g] = 0;
+            if (wz) Pz[((size_t)r * nl + a) * plane + g] = 0;
         }
     }
 }
@@ -178,8 +198,10 @@ __global__ __launch_bounds__(BLK) void k_mf_gemm(const DevConsts *__restrict__ C
     const int rb = blockIdx.y * 64, cb = blockIdx.x * 64;
     const int row0 = rb + 32 * (w & 1), col0 = cb + 32 * (w >> 1);
     const size_t pa = (size_t)MP * KP, pb = (size_t)NP * KP;
-    const signed char *Ab = PA + (size_t)r * NL * pa + (size_t)rb * KP;
-    const signed char *Bb = PB + (size_t)r * NL * pb + (size_t)cb * KP;
+    // planes are k-step major (k_gemm_limbs): step k0's 64-row panel of one limb is the
+    // 64 * MF_T contiguous bytes at k0 * MP + rb * MF_T
+    const signed char *Ab = PA + (size_t)r * NL * pa + (size_t)rb * MF_T;
+    const signed char *Bb = PB + (size_t)r * NL * pb + (size_t)cb * MF_T;
     mf_v4i ga[PER], gb[PER];
     // piece q: limb q >> 7, row (column) (q >> 1) & 63, half q & 1 of the k-step's 32 bytes
     auto gload = [&](int k0) {
@@ -187,9 +209,9 @@ __global__ __launch_bounds__(BLK) void k_mf_gemm(const DevConsts *__restrict__ C
         for (int p = 0; p < PER; p++) {
             const int q = threadIdx.x + p * BLK;
             if (q < ITEMS) {
-                const size_t o = (size_t)((q >> 1) & 63) * KP + k0 + 16 * (q & 1);
-                ga[p] = *(const mf_v4i *)(Ab + (size_t)(q >> 7) * pa + o);
-                gb[p] = *(const mf_v4i *)(Bb + (size_t)(q >> 7) * pb + o);
+                const int o = ((q >> 1) & 63) * MF_T + 16 * (q & 1);
+                ga[p] = *(const mf_v4i *)(Ab + (size_t)(q >> 7) * pa + (size_t)k0 * MP + o);
+                gb[p] = *(const mf_v4i *)(Bb + (size_t)(q >> 7) * pb + (size_t)k0 * NP + o);
             }
         }
     };

@@ -314,9 +314,9 @@ struct Launch {
         }
         hipLaunchKernelGGL((k_mf_band<N, L>), gv, bl, 0, st, (int)mn, half, EMIN, TOPL, EL, BL);
         hipLaunchKernelGGL((k_mf_prescale<N, L, SoA, 0>), dim3(grid_for((long long)MP * KP, 1)), bl, 0, st, c, m, MP,
-                           k, KP, nl, ta, A, lda, EH, EL, PA);
+                           k, KP, nl, ta, A, lda, EH, EL, BL, PA);
         hipLaunchKernelGGL((k_mf_prescale<N, L, SoA, 1>), dim3(grid_for((long long)NP * KP, 1)), bl, 0, st, c, n, NP,
-                           k, KP, nl, tb, B, ldb, EH + m, EL + m, PB);
+                           k, KP, nl, tb, B, ldb, EH + m, EL + m, BL + m, PB);
         for (int p = 0; p < 4; p++) {
             const int ba = p >> 1, bb = p & 1;
             const signed char *pa = PA + (size_t)ba * bandA, *pb = PB + (size_t)bb * bandB;

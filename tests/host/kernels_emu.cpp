@@ -116,7 +116,8 @@ int gemm_v2_split_n(const Ctx &x, int ta, int tb, int m, int n, int k, char *alp
     auto val = [&](const std::vector<signed char> &P, int JP, int r, int j, int i) {
         unsigned long long v = 0;
         for (int q = 0; q < nl; q++) {
-            const int limb = P[((size_t)r * nl + q) * JP * KP + (size_t)j * KP + i];
+            // k-step-major plane: [i / LIMB_T][j][i % LIMB_T]
+            const int limb = P[((size_t)r * nl + q) * JP * KP + ((size_t)(i / LIMB_T) * JP + j) * LIMB_T + i % LIMB_T];
             if (limb < 0) return ~0ULL;  // not a limb: the planes are not fully written
             v |= (unsigned long long)limb << (LIMB_BITS * q);
         }

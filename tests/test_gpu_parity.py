@@ -624,6 +624,26 @@ def test_dot_headline_config():
     assert err < 2.0 ** -95, float(err)
 
 
+def test_gemm_rejects_misaligned_digits():
+    """both gemm entry points read residues as 16-byte vectors: digits of A or B off a
+    16-byte boundary return MPRES_ERR_ARG before anything is launched"""
+    c = ctx("16_4double")
+    A, B, Cm = c.array_set_d(gen.uexp(141, 4, -3, 3)), c.array_set_d(gen.uexp(142, 4, -3, 3)), \
+        c.array_set_d(gen.uexp(143, 4, -3, 3))
+    al, be = c.array_set_d([1.5]), c.array_set_d([0.5])
+    off = mp.MpArray.from_buffer_copy(A)
+    off.digits = A.digits + 4  # still inside A's digits; never read
+    for fn in (c.L.mpres_gemm, c.L.mpres_gemm_mfma):
+        for a, b in ((off, B), (B, off)):
+            rc = fn(c.h, mp.NO_TRANS, mp.NO_TRANS, 2, 2, 2, C.byref(al), C.byref(a), 2, C.byref(b), 2,
+                    C.byref(be), C.byref(Cm), 2, None, 0, None)
+            assert rc == -1, rc  # MPRES_ERR_ARG
+        rc = fn(c.h, mp.NO_TRANS, mp.NO_TRANS, 2, 2, 2, C.byref(al), C.byref(A), 2, C.byref(B), 2,
+                C.byref(be), C.byref(Cm), 2, None, 0, None)
+        assert rc == 0, rc
+    mp.synchronize()
+
+
 @pytest.mark.parametrize("name", ["16_4double", "32_8double"])
 def test_gemm_long_k(name):
     """k beyond one INT32 accumulation of the MFMA residue products (k > 16384): the
