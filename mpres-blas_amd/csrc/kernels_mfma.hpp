@@ -94,11 +94,13 @@ __global__ __launch_bounds__(BLK) void k_mf_band(int cnt, int half, const int *_
 // column without a low band (bl[j] == MF_NONE) leaves its low-band planes unwritten: no
 // entry of it reads a low-band product (the finish takes only the band pairs present),
 // and its padding beyond cnt is never read either.
+// At least four waves per SIMD: left alone the compiler keeps all 32 residues and their
+// loads in flight at 170 registers (two waves), too few to hide the memory latency.
 template <int N, int L, class VM, int SIDE>
-__global__ __launch_bounds__(BLK) void k_mf_prescale(const DevConsts *__restrict__ C, int cnt, int JP, int k, int KP,
-                                                     int nl, int t, VM X, int ld, const int *__restrict__ eh,
-                                                     const int *__restrict__ el, const int *__restrict__ bl,
-                                                     signed char *__restrict__ P) {
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) void
+k_mf_prescale(const DevConsts *__restrict__ C, int cnt, int JP, int k, int KP, int nl, int t, VM X, int ld,
+              const int *__restrict__ eh, const int *__restrict__ el, const int *__restrict__ bl,
+              signed char *__restrict__ P) {
     if constexpr (std::is_same<VM, SoA>::value) X = rd(X);
     const long long g = (long long)blockIdx.x * BLK + threadIdx.x;
     if (g >= (long long)JP * KP) return;
