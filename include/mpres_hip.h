@@ -115,6 +115,19 @@ int mpres_array_to_aos(mpres_ctx_t *ctx, void *dst_aos, const mp_array_t *src, i
 #define MPRES_OP_VADD 4    /* the v1 vector-add kernels: src/mpvector.cuh:234-472 + round :669-714 */
 int mpres_elementwise(mpres_ctx_t *ctx, int op, int count, const void *x, const void *y, void *r, void *stream);
 
+/* ---- comparison ----
+ * r[i] = 1, 0 or -1 as x[i] >, = or < y[i] (MPRES_CMP: cuda::mp_cmp, src/arith/cmp.cuh:79-130)
+ * or as |x[i]| >, = or < |y[i]| (MPRES_CMP_ABS: cuda::mp_cmp_abs, src/arith/cmpabs.cuh:76-119).
+ * x, y: device arrays of count mp_float_t; r: device array of count ints. The operands are
+ * aligned to the larger exponent as the reference does (an operand below 2^MP_J of the other
+ * counts as zero), then compared by interval evaluation, falling back to mixed-radix digits. */
+#define MPRES_CMP 0
+#define MPRES_CMP_ABS 1
+int mpres_cmp(mpres_ctx_t *ctx, int mode, int count, const void *x, const void *y, int *r, void *stream);
+/* r[i] = 1, 0 or -1 as the RNS integers x_i >, = or < y_i: cuda::rns_cmp(x, y),
+ * src/rns.cuh:1242-1260. x, y: device arrays of count x n residues (0 <= x_i < M). */
+int mpres_rns_cmp(mpres_ctx_t *ctx, int count, const int *x, const int *y, int *r, void *stream);
+
 /* ---- BLAS, v1 interface over mp_array_t ---- */
 /* r = x . y ; cuda::mp_dot<gridDim1,blockDim1,gridDim2,gridDim3,blockDim3>, src/blas/dot.cuh:84-107.
  * The sum order is fixed by (gridDim3, blockDim3); the other template arguments do not change the result. */

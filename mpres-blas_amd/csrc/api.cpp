@@ -37,6 +37,20 @@ int fail(int code, const char *fmt, ...) {
     return code;
 }
 
+}  // namespace
+
+int mph::set_error(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+namespace {
+
 #define HIPCHK(expr)                                                                          \
     do {                                                                                      \
         hipError_t e_ = (expr);                                                               \
@@ -162,6 +176,8 @@ struct ProfScope {
 
 }  // namespace
 
+int mph::ctx_device(const mpres_ctx *c) { return c ? c->device : -1; }
+
 extern "C" {
 
 const char *mpres_last_error(void) { return g_err.c_str(); }
@@ -214,8 +230,14 @@ int mpres_ctx_create(mpres_ctx_t **out, int set_id, int device) {
         return fail(MPRES_ERR_HIP, "allocating the constant tables failed");
     }
     c->host.pow2 = c->dpow2;
-    HIPCHK(hipMemcpy(c->dpow2, pow2.data(), pow2.size() * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->dev, &c->host, sizeof(DevConsts), hipMemcpyHostToDevice));
+    hipError_t e = hipMemcpy(c->dpow2, pow2.data(), pow2.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->dev, &c->host, sizeof(DevConsts), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(c->dev);
+        (void)hipFree(c->dpow2);
+        delete c;
+        return fail(MPRES_ERR_HIP, "copying the constant tables: %s", hipGetErrorString(e));
+    }
     *out = c;
     return MPRES_OK;
 }
@@ -423,6 +445,22 @@ int mpres_elementwise(mpres_ctx_t *c, int op, int count, const void *x, const vo
     if (!x || !y || !r) return fail(MPRES_ERR_ARG, "null argument");
     HIPCHK(c->ops[OP_ELEM]->elem(c->dev, op, count, (const char *)x, (const char *)y, (const double *)y, (char *)r,
                                  strm(st)));
+    return MPRES_OK;
+}
+
+int mpres_cmp(mpres_ctx_t *c, int mode, int count, const void *x, const void *y, int *r, void *st) {
+    if (!c || (mode != MPRES_CMP && mode != MPRES_CMP_ABS)) return fail(MPRES_ERR_ARG, "invalid comparison %d", mode);
+    if (count <= 0) return MPRES_OK;
+    if (!x || !y || !r) return fail(MPRES_ERR_ARG, "null argument");
+    HIPCHK(c->ops[OP_ELEM]->cmp(c->dev, mode, count, (const char *)x, (const char *)y, nullptr, nullptr, r, strm(st)));
+    return MPRES_OK;
+}
+
+int mpres_rns_cmp(mpres_ctx_t *c, int count, const int *x, const int *y, int *r, void *st) {
+    if (!c) return fail(MPRES_ERR_ARG, "null context");
+    if (count <= 0) return MPRES_OK;
+    if (!x || !y || !r) return fail(MPRES_ERR_ARG, "null argument");
+    HIPCHK(c->ops[OP_ELEM]->cmp(c->dev, 2, count, nullptr, nullptr, x, y, r, strm(st)));
     return MPRES_OK;
 }
 

@@ -291,6 +291,58 @@ struct Grp {
         return (cmp < 0 ? sy : sx) * (cmp != 0);
     }
 
+    // rns_cmp with given interval evaluations, src/rns.cuh:1275-1289: the bounds decide
+    // when they separate; otherwise equal residues give 0 and mixed-radix digits decide
+    __device__ __forceinline__ int rns_cmp_ev(const int *lx, Er xl, Er xu, const int *ly, Er yl, Er yu) const {
+        if (er_ucmp(xl, yu) > 0) return 1;
+        if (er_ucmp(yl, xu) > 0) return -1;
+        uint32_t diff = 0;
+#pragma unroll
+        for (int r = 0; r < R; r++) diff += lx[r] != ly[r];
+        if (xlane_usum<L>(diff) == 0) return 0;
+        if constexpr (INL)
+            return rns_mrc_compare<N, L, true>(k, lx, ly);
+        else
+            return ool_mrc_compare<N, L>(k, lx, ly);
+    }
+    // cuda::mp_cmp_common (src/arith/cmp.cuh:79-114) and, with ABS, cuda::mp_cmp_abs_common
+    // (src/arith/cmpabs.cuh:76-109): align to the larger exponent (dropping an operand that the
+    // other's aligned evaluation dominates), compare magnitudes, then apply the signs
+    template <bool ABS>
+    __device__ __forceinline__ int cmp(const MP<R> &x, const MP<R> &y) const {
+        const int dexp = x.exp - y.exp;
+        int gamma = dexp > 0 ? dexp : 0, theta = dexp < 0 ? -dexp : 0;
+        const int nzx = (y.ev1.frac == 0) || (theta + y.ev1.exp < k.C->mp_j);
+        const int nzy = (x.ev1.frac == 0) || (gamma + x.ev1.exp < k.C->mp_j);
+        gamma *= nzy;
+        theta *= nzx;
+        const Er xl{x.ev0.frac * (double)nzx, x.ev0.exp + gamma}, xu{x.ev1.frac * (double)nzx, x.ev1.exp + gamma};
+        const Er yl{y.ev0.frac * (double)nzy, y.ev0.exp + theta}, yu{y.ev1.frac * (double)nzy, y.ev1.exp + theta};
+        int lx[R], ly[R];
+        const int *rg = k.P2 + (size_t)prow(gamma) * N, *rt = k.P2 + (size_t)prow(theta) * N;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            lx[r] = nzx ? mmul(x.d[r], rg[ix(r)], r) : 0;
+            ly[r] = nzy ? mmul(y.d[r], rt[ix(r)], r) : 0;
+        }
+        const int c = rns_cmp_ev(lx, xl, xu, ly, yl, yu);
+        if constexpr (ABS) {
+            return c;
+        } else {
+            const int sx = x.sign, sy = y.sign;
+            const bool gt = (sx == 0 && sy == 1) || (sx == 0 && sy == 0 && c == 1) || (sx == 1 && sy == 1 && c == -1);
+            const bool lt = (sx == 1 && sy == 0) || (sx == 0 && sy == 0 && c == -1) || (sx == 1 && sy == 1 && c == 1);
+            return gt ? 1 : lt ? -1 : 0;
+        }
+    }
+    // cuda::rns_cmp(x, y) on bare residues, src/rns.cuh:1242-1260: both evaluations computed
+    __device__ __forceinline__ int rns_cmp(const int *dx, const int *dy) const {
+        Er xl, xu, yl, yu;
+        eval_full(dx, xl, xu);
+        eval_full(dy, yl, yu);
+        return rns_cmp_ev(dx, xl, xu, dy, yl, yu);
+    }
+
     // ---------------- mp_float_t arithmetic ----------------
     __device__ __forceinline__ void zero(MP<R> &a) const {
 #pragma unroll

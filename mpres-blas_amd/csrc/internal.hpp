@@ -7,6 +7,8 @@
 
 #include "devconsts.hpp"
 
+struct mpres_ctx;
+
 namespace mph {
 
 struct ModuliSet {
@@ -19,5 +21,10 @@ struct ModuliSet {
 int num_sets();
 const ModuliSet *get_set(int id);
 bool compute_consts(const ModuliSet &s, DevConsts &c, std::vector<int> &pow2);
+
+// error message of the calling thread (mpres_last_error); returns code
+int set_error(int code, const char *fmt, ...);
+// device of a context
+int ctx_device(const struct ::mpres_ctx *c);
 
 }  // namespace mph

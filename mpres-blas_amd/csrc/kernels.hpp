@@ -144,6 +144,35 @@ __global__ __launch_bounds__(BLK) void k_elem(const DevConsts *__restrict__ C, i
     store_aos(g, c, r, i);
 }
 
+// comparisons: r[i] in {-1, 0, 1}. MODE 0: cuda::mp_cmp (src/arith/cmp.cuh:120-124), 1:
+// cuda::mp_cmp_abs (src/arith/cmpabs.cuh:115-119), both on AoS numbers; 2: cuda::rns_cmp on
+// bare residues, count x N ints each (src/rns.cuh:1242-1260)
+template <int N, int L, int MODE>
+__global__ __launch_bounds__(BLK) void k_cmp(const DevConsts *__restrict__ C, int count, AoSRef<N> x, AoSRef<N> y,
+                                             const int *__restrict__ dx, const int *__restrict__ dy,
+                                             int *__restrict__ out) {
+    const long long i = group_id<L>();
+    if (i >= count) return;
+    Grp<N, L> g(C);
+    int c;
+    if constexpr (MODE == 2) {
+        constexpr int R = N / L;
+        int a[R], b[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            a[r] = dx[i * N + g.ix(r)];
+            b[r] = dy[i * N + g.ix(r)];
+        }
+        c = g.rns_cmp(a, b);
+    } else {
+        MP<N / L> a, b;
+        load_aos(g, a, x, i);
+        load_aos(g, b, y, i);
+        c = g.template cmp<MODE == 1>(a, b);
+    }
+    if (g.lane() == 0) out[i] = c;
+}
+
 // ---------------------------------------------------------------- reductions
 // Strided fold of the reduction kernels (src/mpreduct.cuh:38-56, dot_v2.cuh:37-55):
 // virtual thread v accumulates terms v, v + nvt, v + 2 nvt, ... from zero.

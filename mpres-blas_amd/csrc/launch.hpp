@@ -48,6 +48,19 @@ struct Launch {
         }
         return hipGetLastError();
     }
+    static hipError_t cmp(const DevConsts *c, int mode, int count, const char *x, const char *y, const int *dx,
+                          const int *dy, int *out, hipStream_t st) {
+        if (count <= 0) return hipSuccess;
+        const dim3 gr(grid_for(count, L)), bl(BLK);
+        AoSRef<N> X{(char *)x}, Y{(char *)y};
+        switch (mode) {
+            case 0: hipLaunchKernelGGL((k_cmp<N, L, 0>), gr, bl, 0, st, c, count, X, Y, dx, dy, out); break;
+            case 1: hipLaunchKernelGGL((k_cmp<N, L, 1>), gr, bl, 0, st, c, count, X, Y, dx, dy, out); break;
+            case 2: hipLaunchKernelGGL((k_cmp<N, L, 2>), gr, bl, 0, st, c, count, X, Y, dx, dy, out); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     // pass 0: the main kernel; pass 1: the fixup kernel over the chains the main
     // kernel marked (16 and 32 moduli; nothing to do over 8 moduli)
     static constexpr bool SPLIT = !Grp<N, L>::INL;
@@ -354,6 +367,7 @@ struct Launch {
         o.L = L;
         o.set_d = &set_d;
         o.elem = &elem;
+        o.cmp = &cmp;
         o.fold = &fold;
         o.tree = &tree;
         o.aos_to_soa = &aos_to_soa;

@@ -148,6 +148,13 @@ MPH_HD Er er_set_d(double x) {
     }
     return r;
 }
+// er_ucmp (CUDA), src/extrange.cuh:671-677: comparison of the magnitudes of two
+// extended-range numbers with nonnegative fractions
+MPH_HD int er_ucmp(Er x, Er y) {
+    const int diff = x.exp - y.exp;
+    if (diff == 0) return x.frac == y.frac ? 0 : (x.frac > y.frac ? 1 : -1);
+    return (x.frac != 0 && (diff > 0 || (diff < 0 && y.frac == 0))) ? 1 : -1;
+}
 // er_add_rd / er_add_ru (CUDA), src/extrange.cuh:450-490
 MPH_HD Er er_add_dir(Er x, Er y, bool up) {
     x.exp = (x.frac != 0) ? x.exp : 0;

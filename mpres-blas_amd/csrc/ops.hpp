@@ -40,6 +40,9 @@ struct Ops {
     int N, L;
     hipError_t (*set_d)(const DevConsts *, const double *, int, SoA, char *, bool, hipStream_t);
     hipError_t (*elem)(const DevConsts *, int, int, const char *, const char *, const double *, char *, hipStream_t);
+    // comparisons (mode 0 mp_cmp, 1 mp_cmp_abs on AoS numbers, 2 rns_cmp on bare residues)
+    hipError_t (*cmp)(const DevConsts *, int, int, const char *, const char *, const int *, const int *, int *,
+                      hipStream_t);
     // the reduction launchers take a pass: 0 = main kernel, 1 = fixup kernel (see kernels.hpp);
     // gemv_fold runs passes 0..2 and gemm / gemm_v2 passes 0..4 (see launch.hpp)
     hipError_t (*fold)(const DevConsts *, int, int, long long, long long, SoA, int, SoA, int, const char *,

@@ -96,6 +96,8 @@ def lib():
         L.mpres_aos_to_array.argtypes = [vp, vp, vp, i32, vp]
         L.mpres_array_to_aos.argtypes = [vp, vp, vp, i32, vp]
         L.mpres_elementwise.argtypes = [vp, i32, i32, vp, vp, vp, vp]
+        L.mpres_cmp.argtypes = [vp, i32, i32, vp, vp, vp, vp]
+        L.mpres_rns_cmp.argtypes = [vp, i32, vp, vp, vp, vp]
         L.mpres_dot_workspace.restype = sz
         L.mpres_dot_workspace.argtypes = [vp, i32, i32, i32]
         L.mpres_dot.argtypes = [vp, i32, vp, i32, vp, i32, vp, i32, i32, vp, sz, vp]

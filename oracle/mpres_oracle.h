@@ -90,6 +90,8 @@ void orc_eval(const orc_ctx *c, int policy, int fast, int count, const int *x, o
 void orc_rank(const orc_ctx *c, int policy, int fast, int count, const int *x, int *k);
 void orc_mrd(const orc_ctx *c, int count, const int *x, int *mrd);
 void orc_rns_cmp(const orc_ctx *c, int policy, int count, const int *x, const int *y, int *cmp);
+/* mp_cmp (abs_only = 0) / mp_cmp_abs (abs_only = 1) of AoS numbers: r[i] in {-1, 0, 1} */
+void orc_mp_cmp(const orc_ctx *c, int policy, int abs_only, int count, const void *x, const void *y, int *r);
 
 /* ---- BLAS compositions in the reference's operation order ---- */
 /* dot: products then the two-pass reduction for launch shape (g3,b3), src/blas/dot.cuh:41-107;

@@ -179,6 +179,14 @@ void orc_rns_cmp(const orc_ctx *c, int policy, int count, const int *x, const in
     DISPATCH(c, policy, B_CMP);
 }
 
+/* mp_cmp / mp_cmp_abs, src/arith/cmp.cuh, cmpabs.cuh */
+#define B_MPCMP(s)                                                                         \
+    for (int i = 0; i < count; i++)                                                        \
+        r[i] = G(cmp, s)(c, ((const G(mpf, s) *)x)[i], ((const G(mpf, s) *)y)[i], abs_only);
+void orc_mp_cmp(const orc_ctx *c, int policy, int abs_only, int count, const void *x, const void *y, int *r) {
+    DISPATCH(c, policy, B_MPCMP);
+}
+
 /* ---- compositions ---- */
 #define B_DOT(s)                                                                           \
     G(dot, s)(c, n, (const G(mpf, s) *)x, incx, (const G(mpf, s) *)y, incy, g3, b3, (G(mpf, s) *)r);

@@ -182,7 +182,7 @@ static inline orc_er_t F(er_md_ru)(orc_er_t x, orc_er_t y, orc_er_t z) {
     F(er_adjust)(&r);
     return r;
 }
-/* er_ucmp, src/extrange.cuh:385-391 */
+/* er_ucmp, src/extrange.cuh:393-399 (CUDA :671-677) */
 static inline int F(er_ucmp)(orc_er_t x, orc_er_t y) {
     long diff = x.exp - y.exp;
     if (diff == 0) return x.frac == y.frac ? 0 : (x.frac > y.frac ? 1 : -1);
@@ -515,6 +515,55 @@ static void F(add)(const orc_ctx *c, MPF *r, MPF x, MPF y) {
         r->eval[1].exp = tmp.exp;
     }
     F(round_if_needed)(c, r);
+}
+
+/* mp_cmp (abs_only = 0) and mp_cmp_abs (abs_only = 1): CPU src/arith/cmp.cuh:31-67 and
+ * cmpabs.cuh:31-64; CUDA mp_cmp_common cmp.cuh:79-114 and mp_cmp_abs_common cmpabs.cuh:76-109.
+ * The operands are aligned to the larger exponent (the smaller one is dropped when the larger's
+ * aligned evaluation reaches 2^MP_J), then compared as RNS magnitudes with the aligned
+ * evaluations: rns_cmp(x, exl, exu, y, eyl, eyu), src/rns.cuh:1208-1225 (CUDA :1275-1289).
+ * Both policies compute the same thing: residues by exact products, no directed rounding. */
+static int F(cmp)(const orc_ctx *c, MPF x, MPF y, int abs_only) {
+    int dexp = x.exp - y.exp;
+    int gamma = dexp * (dexp > 0);
+    int theta = -dexp * (dexp < 0);
+    int nzx = ((y.eval[1].frac == 0) || (theta + y.eval[1].exp) < c->mp_j);
+    int nzy = ((x.eval[1].frac == 0) || (gamma + x.eval[1].exp) < c->mp_j);
+    gamma = gamma * nzy;
+    theta = theta * nzx;
+    x.eval[0].exp += gamma;
+    x.eval[1].exp += gamma;
+    y.eval[0].exp += theta;
+    y.eval[1].exp += theta;
+    x.eval[0].frac *= nzx;
+    x.eval[1].frac *= nzx;
+    y.eval[0].frac *= nzy;
+    y.eval[1].frac *= nzy;
+    int lx[N], ly[N];
+    const int pg = F(prow)(c, gamma), pt = F(prow)(c, theta);
+    for (int i = 0; i < N; i++) {
+        lx[i] = nzx * F(mod_mul_exact)(x.digits[i], c->pow2[pg * N + i], c->m[i]);
+        ly[i] = nzy * F(mod_mul_exact)(y.digits[i], c->pow2[pt * N + i], c->m[i]);
+    }
+    int cmp;
+    if (F(er_ucmp)(x.eval[0], y.eval[1]) > 0) {
+        cmp = 1;
+    } else if (F(er_ucmp)(y.eval[0], x.eval[1]) > 0) {
+        cmp = -1;
+    } else {
+        int eq = 1;
+        for (int i = 0; i < N; i++)
+            if (lx[i] != ly[i]) {
+                eq = 0;
+                break;
+            }
+        cmp = eq ? 0 : F(mrc_compare)(c, lx, ly);
+    }
+    if (abs_only) return cmp;
+    const int sx = x.sign, sy = y.sign;
+    const int greater = (sx == 0 && sy == 1) || (sx == 0 && sy == 0 && cmp == 1) || (sx == 1 && sy == 1 && cmp == -1);
+    const int less = (sx == 1 && sy == 0) || (sx == 0 && sy == 0 && cmp == -1) || (sx == 1 && sy == 1 && cmp == 1);
+    return greater ? 1 : less ? -1 : 0;
 }
 
 /* mp_mul: CPU src/arith/mul.cuh:31-42; CUDA :53-74 (and the esi/digits/round

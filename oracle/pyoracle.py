@@ -206,6 +206,12 @@ class Oracle:
         self.L.orc_rns_cmp(self.ctx, policy, x.shape[0], _p(x), _p(y), _p(r))
         return r
 
+    def cmp(self, x, y, abs_only=False, policy=POLICY_D):
+        r = np.zeros(len(x), np.int32)
+        self.L.orc_mp_cmp(self.ctx, policy, int(abs_only), len(x), _p(np.ascontiguousarray(x)),
+                          _p(np.ascontiguousarray(y)), _p(r))
+        return r
+
     # ---- compositions ----
     def dot(self, x, y, g3, b3, policy=POLICY_D, incx=1, incy=1, n=None):
         n = len(x) if n is None else n

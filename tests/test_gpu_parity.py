@@ -119,6 +119,63 @@ def test_elementwise(name):
             same(c.aos_to_host(r, len(X)), w, f"op {op} {name} L={L}")
 
 
+def _cmp_operands(o, seed, cnt):
+    """pairs reaching every branch of mp_cmp: separated evaluations, equal numbers (also
+    with different representations of one value), near-equal magnitudes (overlapping
+    evaluations, mixed-radix decision), signed zeros, opposite signs, an operand dropped
+    against a much larger one"""
+    X, Y = _operands(o, seed, cnt)
+    a = o.set_d(inputs(o, seed + 7, cnt, -300, 300), D)
+    neg = a.copy()
+    neg["sign"] ^= 1
+    z = o.set_d(np.array([0.0, -0.0] * (cnt // 2)), D)
+    tiny = o.mul(a, o.set_d(np.full(cnt, 2.0**-600), D), D)
+    # the same values scaled by a power of two through the residues: equal, other exponent
+    two = o.mul(o.mul(a, o.set_d(np.full(cnt, 2.0**40), D), D), o.set_d(np.full(cnt, 2.0**-40), D), D)
+    up = o.add(a, o.mul(a, o.set_d(np.full(cnt, 2.0**-90), D), D), D)
+    Xs = np.concatenate([X, a, a, a, z, z[::-1], a, tiny, two, up, a])
+    Ys = np.concatenate([Y, a, neg, z, z, a, tiny, a, a, a, up])
+    return Xs, Ys
+
+
+@pytest.mark.parametrize("name", SETS)
+def test_cmp(name):
+    """mpres_cmp (mp_cmp, mp_cmp_abs) and mpres_rns_cmp against the oracle, every lanes variant"""
+    c, o = ctx(name), orc(name)
+    X, Y = _cmp_operands(o, 17, 400)
+    dX, dY = c.aos(X), c.aos(Y)
+    want = {0: o.cmp(X, Y, False, D), 1: o.cmp(X, Y, True, D)}
+    assert set(np.unique(want[0])) == {-1, 0, 1}
+    # bare residues: random integers below M, equal pairs, neighbours (X, X + 1), small X
+    mm = [int(v) for v in o.moduli()]
+    M = exact.moduli_product(mm)
+    rng = np.random.default_rng(19)
+    ints = [int(rng.integers(0, 2**62)) * int(rng.integers(0, 2**62)) % M for _ in range(300)]
+    ints += [int(rng.integers(0, 2**40)) for _ in range(100)]
+    pairs = [(a_, b_) for a_, b_ in zip(ints[:200], ints[200:])]
+    pairs += [(v, v) for v in ints[:50]] + [(v, (v + 1) % M) for v in ints[50:100]] + [((v + 1) % M, v) for v in ints[100:150]]
+    pairs += [(M - 1, M - 2), (0, 0), (0, 1), (M - 1, 0)]
+    dx = np.array([exact.residues(a_, mm) for a_, _ in pairs], np.int32)
+    dy = np.array([exact.residues(b_, mm) for _, b_ in pairs], np.int32)
+    want_r = o.rns_cmp(dx, dy, D)
+    assert list(want_r) == [(a_ > b_) - (a_ < b_) for a_, b_ in pairs]
+    ddx, ddy = mp.DevBuf(host=dx), mp.DevBuf(host=dy)
+    for L in lanes_of(c):
+        c.set_lanes(mp.K_ELEM, L)
+        for mode, w in want.items():
+            note("cmp", name, mode, L)
+            r = mp.DevBuf(nbytes=4 * len(X))
+            mp.check(c.L.mpres_cmp(c.h, mode, len(X), dX.ptr, dY.ptr, r.ptr, None))
+            got = r.to_numpy(np.int32, len(X))
+            bad = np.nonzero(got != w)[0]
+            assert len(bad) == 0, f"cmp {name} mode {mode} L={L}: {len(bad)} differ, first {bad[:5]}"
+        note("rns_cmp", name, L)
+        r = mp.DevBuf(nbytes=4 * len(pairs))
+        mp.check(c.L.mpres_rns_cmp(c.h, len(pairs), ddx.ptr, ddy.ptr, r.ptr, None))
+        got = r.to_numpy(np.int32, len(pairs))
+        assert (got == want_r).all(), f"rns_cmp {name} L={L}: {np.nonzero(got != want_r)[0][:5]}"
+
+
 @pytest.mark.parametrize("name,n,g3,b3", [("8_2double", 5000, 64, 64), ("8_2double", 3001, 256, 64),
                                           ("8_2double", 700, 7, 33), ("16_4double", 2000, 64, 64),
                                           ("32_8double", 1500, 32, 64)])

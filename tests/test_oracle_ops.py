@@ -284,3 +284,39 @@ def test_add_mul_closed_form(orc, pol):
         lv, uv = exact.er_value(r["eval"][0]), exact.er_value(r["eval"][1])
         q = Fraction(X, exact.moduli_product(m))
         assert lv <= q <= uv
+
+
+@pytest.mark.parametrize("pol", [D, E])
+def test_mp_cmp_exact(orc, pol):
+    """mp_cmp / mp_cmp_abs (src/arith/cmp.cuh, cmpabs.cuh) against the exact comparison of the
+    values, on separated, equal, near-equal (mixed-radix decision), dropped and zero operands.
+    Two zeros compare by sign in the reference (+0 > -0: cmp.cuh:64-65), so they are checked
+    against that rule."""
+    m = [int(v) for v in orc.moduli()]
+    rng = np.random.default_rng(31 + pol)
+    cnt = 200
+    a = orc.set_d(rand_doubles(rng, cnt, -300, 300), policy=pol)
+    b = orc.set_d(rand_doubles(rng, cnt, -300, 300), policy=pol)
+    p = orc.mul(a, b, policy=pol)
+    neg = a.copy()
+    neg["sign"] ^= 1
+    z = orc.set_d(np.array([0.0, -0.0] * (cnt // 2)), policy=pol)
+    near = orc.add(p, orc.mul(p, orc.set_d(np.full(cnt, 2.0**-80), policy=pol), policy=pol), policy=pol)
+    # the same value with another exponent (2^40 through the residues and back)
+    same = orc.mul(orc.mul(a, orc.set_d(np.full(cnt, 2.0**40), policy=pol), policy=pol),
+                   orc.set_d(np.full(cnt, 2.0**-40), policy=pol), policy=pol)
+    X = np.concatenate([a, a, a, p, near, z, z, same, a])
+    Y = np.concatenate([b, a, neg, near, p, z, z[::-1], a, z])
+    c = orc.cmp(X, Y, False, pol)
+    ca = orc.cmp(X, Y, True, pol)
+    seen = set()
+    for x, y, cc, cca in zip(X, Y, c, ca):
+        vx, vy = exact.mp_value(x, m), exact.mp_value(y, m)
+        if vx == 0 and vy == 0:
+            want = int(y["sign"]) - int(x["sign"])
+        else:
+            want = (vx > vy) - (vx < vy)
+        assert cc == want, (float(vx), float(vy), cc)
+        assert cca == (abs(vx) > abs(vy)) - (abs(vx) < abs(vy))
+        seen.add(cc)
+    assert seen == {-1, 0, 1}
