@@ -91,6 +91,10 @@ struct mpres_ctx {
     const Ops *ops[OP_COUNT] = {};
     void *ws = nullptr;
     size_t ws_bytes = 0;
+    // the pool's last use: an event recorded on the stream of the last call that used it
+    hipEvent_t ws_ev = nullptr;
+    hipStream_t ws_stream = nullptr;
+    bool ws_used = false;
     // optional timing of each operation's main kernel (mpres_profile)
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_free;
@@ -108,30 +112,52 @@ SoA soa_of(const mp_array_t *a) { return soa_view(a); }
 // the split and precision-form gemm kernels read an element's residues as 16-byte vectors
 bool digits_aligned(const mp_array_t *a) { return ((uintptr_t)a->digits & 15) == 0; }
 
-// workspace: caller's buffer or the context pool
-int get_ws(mpres_ctx *c, void *ws, size_t ws_bytes, size_t need, char **out) {
-    if (need == 0) {
-        *out = nullptr;
-        return MPRES_OK;
-    }
-    if (ws) {
-        if (ws_bytes < need) return fail(MPRES_ERR_WORKSPACE, "workspace of %zu bytes, %zu needed", ws_bytes, need);
-        *out = (char *)ws;
-        return MPRES_OK;
-    }
-    if (c->ws_bytes < need) {
-        if (c->ws) {
-            HIPCHK(hipDeviceSynchronize());
-            HIPCHK(hipFree(c->ws));
-            c->ws = nullptr;
-            c->ws_bytes = 0;
+// workspace: caller's buffer or the context pool. Calls on one stream use the pool in
+// stream order; a call on another stream first waits (on the device) for the pool's
+// last use, and the pool grows only after that use has ended, so calls on different
+// streams never overwrite each other's partial results. WsUse records the new last use
+// once the calling operation has enqueued its kernels.
+struct WsUse {
+    mpres_ctx *c;
+    hipStream_t st;
+    bool pool = false;
+    WsUse(mpres_ctx *c_, hipStream_t st_) : c(c_), st(st_) {}
+    int get(void *ws, size_t ws_bytes, size_t need, char **out) {
+        if (need == 0) {
+            *out = nullptr;
+            return MPRES_OK;
         }
-        HIPCHK(hipMalloc(&c->ws, need));
-        c->ws_bytes = need;
+        if (ws) {
+            if (ws_bytes < need)
+                return fail(MPRES_ERR_WORKSPACE, "workspace of %zu bytes, %zu needed", ws_bytes, need);
+            *out = (char *)ws;
+            return MPRES_OK;
+        }
+        if (!c->ws_ev) HIPCHK(hipEventCreateWithFlags(&c->ws_ev, hipEventDisableTiming));
+        if (c->ws_bytes < need) {
+            if (c->ws) {
+                if (c->ws_used) HIPCHK(hipEventSynchronize(c->ws_ev));
+                HIPCHK(hipFree(c->ws));
+                c->ws = nullptr;
+                c->ws_bytes = 0;
+            }
+            HIPCHK(hipMalloc(&c->ws, need));
+            c->ws_bytes = need;
+        } else if (c->ws_used && c->ws_stream != st) {
+            HIPCHK(hipStreamWaitEvent(st, c->ws_ev, 0));
+        }
+        pool = true;
+        *out = (char *)c->ws;
+        return MPRES_OK;
     }
-    *out = (char *)c->ws;
-    return MPRES_OK;
-}
+    ~WsUse() {
+        if (!pool) return;
+        if (hipEventRecord(c->ws_ev, st) == hipSuccess) {
+            c->ws_stream = st;
+            c->ws_used = true;
+        }
+    }
+};
 
 unsigned next_pow2(unsigned v) {  // src/common.cuh:92-101
     v--;
@@ -247,6 +273,7 @@ int mpres_ctx_destroy(mpres_ctx_t *c) {
     (void)hipSetDevice(c->device);
     (void)hipDeviceSynchronize();
     if (c->ws) (void)hipFree(c->ws);
+    if (c->ws_ev) (void)hipEventDestroy(c->ws_ev);
     for (auto &e : c->prof_live) c->prof_free.push_back(e.second);
     for (auto &e : c->prof_free) {
         (void)hipEventDestroy(e.first);
@@ -335,14 +362,18 @@ int mpres_array_init(mpres_ctx_t *c, mp_array_t *a, size_t size) {
     if (!c || !a || size == 0) return fail(MPRES_ERR_ARG, "invalid array request");
     std::memset(a, 0, sizeof *a);
     const size_t n = (size_t)c->host.n;
-    HIPCHK(hipMalloc(&a->digits, n * size * sizeof(int)));
-    HIPCHK(hipMalloc(&a->sign, size * sizeof(int)));
-    HIPCHK(hipMalloc(&a->exp, size * sizeof(int)));
-    HIPCHK(hipMalloc(&a->eval, 2 * size * sizeof(er_float_t)));
-    HIPCHK(hipMalloc(&a->buf, size * sizeof(mpres_int4_t)));
-    HIPCHK(hipMalloc(&a->len, sizeof(int)));
     const int len = (int)size;
-    HIPCHK(hipMemcpy(a->len, &len, sizeof(int), hipMemcpyHostToDevice));
+    hipError_t e = hipMalloc(&a->digits, n * size * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&a->sign, size * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&a->exp, size * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&a->eval, 2 * size * sizeof(er_float_t));
+    if (e == hipSuccess) e = hipMalloc(&a->buf, size * sizeof(mpres_int4_t));
+    if (e == hipSuccess) e = hipMalloc(&a->len, sizeof(int));
+    if (e == hipSuccess) e = hipMemcpy(a->len, &len, sizeof(int), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        mpres_array_clear(c, a);  // frees what was allocated (hipFree(NULL) is a no-op)
+        return fail(MPRES_ERR_HIP, "allocating an array of %zu elements: %s", size, hipGetErrorString(e));
+    }
     return MPRES_OK;
 }
 
@@ -377,12 +408,18 @@ int mpres_array_host2device(mpres_ctx_t *c, mp_array_t *dst, const void *src, si
     int len = 0;
     HIPCHK(hipMemcpy(&len, dst->len, sizeof(int), hipMemcpyDeviceToHost));
     if ((size_t)len < size) return fail(MPRES_ERR_ARG, "array of length %d, %zu elements given", len, size);
-    HIPCHK(hipMemcpyAsync(dst->digits, dig.data(), dig.size() * sizeof(int), hipMemcpyHostToDevice, strm(st)));
-    HIPCHK(hipMemcpyAsync(dst->sign, sg.data(), size * sizeof(int), hipMemcpyHostToDevice, strm(st)));
-    HIPCHK(hipMemcpyAsync(dst->exp, ex.data(), size * sizeof(int), hipMemcpyHostToDevice, strm(st)));
-    HIPCHK(hipMemcpyAsync(dst->eval, ev.data(), size * sizeof(er_float_t), hipMemcpyHostToDevice, strm(st)));
-    HIPCHK(hipMemcpyAsync(dst->eval + len, ev.data() + size, size * sizeof(er_float_t), hipMemcpyHostToDevice, strm(st)));
-    HIPCHK(hipStreamSynchronize(strm(st)));
+    // the copies read the staging vectors: every path waits for them before the vectors go away
+    hipError_t e = hipMemcpyAsync(dst->digits, dig.data(), dig.size() * sizeof(int), hipMemcpyHostToDevice, strm(st));
+    if (e == hipSuccess) e = hipMemcpyAsync(dst->sign, sg.data(), size * sizeof(int), hipMemcpyHostToDevice, strm(st));
+    if (e == hipSuccess) e = hipMemcpyAsync(dst->exp, ex.data(), size * sizeof(int), hipMemcpyHostToDevice, strm(st));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dst->eval, ev.data(), size * sizeof(er_float_t), hipMemcpyHostToDevice, strm(st));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dst->eval + len, ev.data() + size, size * sizeof(er_float_t), hipMemcpyHostToDevice,
+                           strm(st));
+    const hipError_t w = hipStreamSynchronize(strm(st));
+    if (e == hipSuccess) e = w;
+    if (e != hipSuccess) return fail(MPRES_ERR_HIP, "copying an array to the device: %s", hipGetErrorString(e));
     return MPRES_OK;
 }
 
@@ -395,12 +432,18 @@ int mpres_array_device2host(mpres_ctx_t *c, void *dst, const mp_array_t *src, si
     if ((size_t)len < size) return fail(MPRES_ERR_ARG, "array of length %d, %zu elements asked", len, size);
     std::vector<int> dig(size * n), sg(size), ex(size);
     std::vector<er_float_t> ev(2 * size);
-    HIPCHK(hipMemcpyAsync(dig.data(), src->digits, dig.size() * sizeof(int), hipMemcpyDeviceToHost, strm(st)));
-    HIPCHK(hipMemcpyAsync(sg.data(), src->sign, size * sizeof(int), hipMemcpyDeviceToHost, strm(st)));
-    HIPCHK(hipMemcpyAsync(ex.data(), src->exp, size * sizeof(int), hipMemcpyDeviceToHost, strm(st)));
-    HIPCHK(hipMemcpyAsync(ev.data(), src->eval, size * sizeof(er_float_t), hipMemcpyDeviceToHost, strm(st)));
-    HIPCHK(hipMemcpyAsync(ev.data() + size, src->eval + len, size * sizeof(er_float_t), hipMemcpyDeviceToHost, strm(st)));
-    HIPCHK(hipStreamSynchronize(strm(st)));
+    // the copies write the staging vectors: every path waits for them before the vectors go away
+    hipError_t e = hipMemcpyAsync(dig.data(), src->digits, dig.size() * sizeof(int), hipMemcpyDeviceToHost, strm(st));
+    if (e == hipSuccess) e = hipMemcpyAsync(sg.data(), src->sign, size * sizeof(int), hipMemcpyDeviceToHost, strm(st));
+    if (e == hipSuccess) e = hipMemcpyAsync(ex.data(), src->exp, size * sizeof(int), hipMemcpyDeviceToHost, strm(st));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(ev.data(), src->eval, size * sizeof(er_float_t), hipMemcpyDeviceToHost, strm(st));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(ev.data() + size, src->eval + len, size * sizeof(er_float_t), hipMemcpyDeviceToHost,
+                           strm(st));
+    const hipError_t w = hipStreamSynchronize(strm(st));
+    if (e == hipSuccess) e = w;
+    if (e != hipSuccess) return fail(MPRES_ERR_HIP, "copying an array to the host: %s", hipGetErrorString(e));
     char *p = (char *)dst;
     for (size_t i = 0; i < size; i++) {
         std::memcpy(p + i * S, &dig[i * n], n * sizeof(int));
@@ -471,11 +514,12 @@ size_t mpres_dot_workspace(const mpres_ctx_t *c, int n, int g3, int b3) {
     return mpsize(c) * ((size_t)g3 * b3 + (size_t)g3 + (size_t)b3 + 1);
 }
 
-static int dot_common(mpres_ctx_t *c, int mode, int n, SoA xs, int incx, SoA ys, int incy, const char *xa,
-                      const char *ya, int g3, int b3, void *ws, size_t wsb, hipStream_t st, char **result) {
+static int dot_common(mpres_ctx_t *c, WsUse &wu, int mode, int n, SoA xs, int incx, SoA ys, int incy,
+                      const char *xa, const char *ya, int g3, int b3, void *ws, size_t wsb, hipStream_t st,
+                      char **result) {
     const size_t S = mpsize(c);
     char *w;
-    int rc = get_ws(c, ws, wsb, mpres_dot_workspace(c, n, g3, b3), &w);
+    int rc = wu.get(ws, wsb, mpres_dot_workspace(c, n, g3, b3), &w);
     if (rc) return rc;
     const long long G = (long long)g3 * b3;
     char *p1 = w, *p2 = p1 + S * G, *p3 = p2 + S * g3, *p4 = p3 + S * b3;
@@ -502,7 +546,8 @@ int mpres_dot(mpres_ctx_t *c, int n, const mp_array_t *x, int incx, const mp_arr
     if (n <= 0) return MPRES_OK;  // src/blas/dot.cuh:88-90
     if (g3 <= 0 || b3 <= 0) return fail(MPRES_ERR_ARG, "invalid reduction shape %d x %d", g3, b3);
     char *res;
-    int rc = dot_common(c, 0, n, soa_of(x), incx, soa_of(y), incy, nullptr, nullptr, g3, b3, ws, wsb, strm(st), &res);
+    WsUse wu(c, strm(st));
+    int rc = dot_common(c, wu, 0, n, soa_of(x), incx, soa_of(y), incy, nullptr, nullptr, g3, b3, ws, wsb, strm(st), &res);
     if (rc) return rc;
     HIPCHK(c->ops[OP_DOT]->aos_to_soa(c->dev, res, soa_of(r), 1, strm(st)));
     return MPRES_OK;
@@ -519,7 +564,8 @@ int mpres_dot_v2(mpres_ctx_t *c, int n, const void *x, const void *y, void *r, i
     if (blocks <= 0 || threads <= 0) return fail(MPRES_ERR_ARG, "invalid launch shape %d x %d", blocks, threads);
     char *res;
     SoA none{};
-    int rc = dot_common(c, 1, n, none, 1, none, 1, (const char *)x, (const char *)y, blocks, threads, ws, wsb, strm(st),
+    WsUse wu(c, strm(st));
+    int rc = dot_common(c, wu, 1, n, none, 1, none, 1, (const char *)x, (const char *)y, blocks, threads, ws, wsb, strm(st),
                         &res);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(r, res, mpsize(c), hipMemcpyDeviceToDevice, strm(st)));
@@ -567,7 +613,8 @@ int mpres_gemv(mpres_ctx_t *c, int trans, int m, int n, const mp_array_t *alpha,
     const int lx = tr ? m : n, ly = tr ? n : m;
     const size_t S = mpsize(c);
     char *w;
-    int rc = get_ws(c, ws, wsb, mpres_gemv_workspace(c, trans, m, n, b3), &w);
+    WsUse wu(c, strm(st));
+    int rc = wu.get(ws, wsb, mpres_gemv_workspace(c, trans, m, n, b3), &w);
     if (rc) return rc;
     char *bx = w, *part = bx + S * lx, *sums = part + S * (size_t)ly * b3, *aux = sums + S * (size_t)ly;
     const Ops *o = c->ops[OP_GEMV];
@@ -606,7 +653,8 @@ int mpres_gemm(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const mp_arr
     if (!digits_aligned(A) || !digits_aligned(B))
         return fail(MPRES_ERR_ARG, "digits of A and B must be 16-byte aligned");
     char *w;
-    int rc = get_ws(c, ws, wsb, mpres_gemm_workspace(c, m, n, k), &w);
+    WsUse wu(c, strm(st));
+    int rc = wu.get(ws, wsb, mpres_gemm_workspace(c, m, n, k), &w);
     if (rc) return rc;
     const Ops *o = c->ops[OP_GEMM];
     for (int pass = 0; pass < 5; pass++) {
@@ -634,7 +682,8 @@ int mpres_gemm_mfma(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const m
     if (!digits_aligned(A) || !digits_aligned(B))
         return fail(MPRES_ERR_ARG, "digits of A and B must be 16-byte aligned");
     char *w;
-    int rc = get_ws(c, ws, wsb, mpres_gemm_mfma_workspace(c, m, n, k), &w);
+    WsUse wu(c, strm(st));
+    int rc = wu.get(ws, wsb, mpres_gemm_mfma_workspace(c, m, n, k), &w);
     if (rc) return rc;
     ProfScope ps(c, PROF_GEMM_MFMA, strm(st));
     HIPCHK(c->ops[OP_MFMA]->gemm_mfma(c->dev, c->bits, c->host.log2m, tra, trb, m, n, k, soa_of(alpha), soa_of(A), lda,
@@ -648,7 +697,8 @@ int mpres_gemm_v2(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const voi
     if (m <= 0 || n <= 0) return MPRES_OK;
     if (k < 0 || lda <= 0 || ldb <= 0 || ldc < std::max(1, m)) return fail(MPRES_ERR_ARG, "invalid dimensions");
     char *w;
-    int rc = get_ws(c, ws, wsb, mpres_gemm_workspace(c, m, n, k), &w);
+    WsUse wu(c, strm(st));
+    int rc = wu.get(ws, wsb, mpres_gemm_workspace(c, m, n, k), &w);
     if (rc) return rc;
     const int tra = ta == MPRES_TRANS || ta == MPRES_CONJ_TRANS, trb = tb == MPRES_TRANS || tb == MPRES_CONJ_TRANS;
     for (int pass = 0; pass < 5; pass++)

@@ -167,6 +167,9 @@ def hip():
         H.hipEventSynchronize.argtypes = [vp]
         H.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), vp, vp]
         H.hipEventDestroy.argtypes = [vp]
+        H.hipStreamCreate.argtypes = [C.POINTER(vp)]
+        H.hipStreamDestroy.argtypes = [vp]
+        H.hipStreamSynchronize.argtypes = [vp]
         H.hipGetErrorString.restype = C.c_char_p
         _hip = H
     return _hip

@@ -205,6 +205,35 @@ def test_dot(name, n, g3, b3):
         assert abs(got - ex) / abs(ex) < 2.0 ** (-c.precision + 12)
 
 
+def test_pool_two_streams():
+    """The context's workspace pool shared by calls on two streams: a call on another stream
+    waits for the pool's last use, and the pool grows only after that use has ended, so
+    every result equals its single-stream value"""
+    name, n = "8_2double", 1 << 18
+    c, o = mp.Context(name), orc(name)  # a fresh context: its pool starts empty
+    H = mp.hip()
+    sa, sb = C.c_void_p(), C.c_void_p()
+    mp.hipchk(H.hipStreamCreate(C.byref(sa)))
+    mp.hipchk(H.hipStreamCreate(C.byref(sb)))
+    try:
+        xd, yd = [gen.uexp(s, n, -20, 20) for s in (41, 42)], [gen.uexp(s, n, -20, 20) for s in (43, 44)]
+        ax, ay = [c.array_set_d(v) for v in xd], [c.array_set_d(v) for v in yd]
+        r = [c.array(1) for _ in range(3)]
+        calls = [(0, 64, 64, sa), (1, 256, 64, sb), (0, 256, 64, sa)]  # grows on the second call
+        note("pool two streams", name, n, calls)
+        for i, (j, g3, b3, st) in enumerate(calls):
+            mp.check(c.L.mpres_dot(c.h, n, C.byref(ax[j]), 1, C.byref(ay[j]), 1, C.byref(r[i]), g3, b3, None, 0, st))
+        mp.hipchk(H.hipStreamSynchronize(sa))
+        mp.hipchk(H.hipStreamSynchronize(sb))
+        for i, (j, g3, b3, _) in enumerate(calls):
+            want = o.dot(o.set_d(xd[j], D), o.set_d(yd[j], D), g3, b3, D)
+            same(c.array_to_host(r[i], 1), want, f"pool call {i}")
+    finally:
+        H.hipStreamDestroy(sa)
+        H.hipStreamDestroy(sb)
+        c.close()
+
+
 @pytest.mark.parametrize("name", ["8_2double", "32_8double"])
 def test_axpy(name):
     c, o = ctx(name), orc(name)
