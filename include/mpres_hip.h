@@ -134,6 +134,15 @@ int mpres_rns_cmp(mpres_ctx_t *ctx, int count, const int *x, const int *y, int *
 size_t mpres_dot_workspace(const mpres_ctx_t *ctx, int n, int grid3, int block3);
 int mpres_dot(mpres_ctx_t *ctx, int n, const mp_array_t *x, int incx, const mp_array_t *y, int incy, mp_array_t *r,
               int grid3, int block3, void *ws, size_t ws_bytes, void *stream);
+/* mpres_dot without any allocation: no counterpart in the reference. For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same result as mpres_dot; the partial sums live in a fixed device array of
+ * MPRES_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends and shared by every call on
+ * that GPU, so calls that overlap in time on one GPU race. A reduction shape whose partials
+ * (mpres_dot_workspace) exceed that size returns MPRES_ERR_WORKSPACE. */
+#define MPRES_NOALLOC_SCRATCH_BYTES (4 << 20)
+int mpres_dot_noalloc(mpres_ctx_t *ctx, int n, const mp_array_t *x, int incx, const mp_array_t *y, int incy,
+                      mp_array_t *r, int grid3, int block3, void *stream);
 /* y = alpha*x + y ; cuda::mp_axpy<gridDim1,blockDim1,gridDim2>, src/blas/axpy.cuh:45-74 (order-free) */
 int mpres_axpy(mpres_ctx_t *ctx, int n, const mp_array_t *alpha, const mp_array_t *x, int incx, mp_array_t *y,
                int incy, void *stream);

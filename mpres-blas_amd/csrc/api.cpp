@@ -232,6 +232,7 @@ int mpres_ctx_create(mpres_ctx_t **out, int set_id, int device) {
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(MPRES_ERR_NO_DEVICE, "device %d is %s; this library is built for gfx950", device, prop.gcnArchName);
     HIPCHK(hipSetDevice(device));
+    if (int rc = noalloc_init(device)) return rc;
     auto *c = new mpres_ctx();
     c->device = device;
     c->set_id = set_id;

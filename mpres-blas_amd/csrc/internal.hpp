@@ -26,5 +26,7 @@ bool compute_consts(const ModuliSet &s, DevConsts &c, std::vector<int> &pow2);
 int set_error(int code, const char *fmt, ...);
 // device of a context
 int ctx_device(const struct ::mpres_ctx *c);
+// looks up the no-allocation dot's scratch array on a device (current), once per device
+int noalloc_init(int device);
 
 }  // namespace mph

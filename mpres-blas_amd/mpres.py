@@ -101,6 +101,7 @@ def lib():
         L.mpres_dot_workspace.restype = sz
         L.mpres_dot_workspace.argtypes = [vp, i32, i32, i32]
         L.mpres_dot.argtypes = [vp, i32, vp, i32, vp, i32, vp, i32, i32, vp, sz, vp]
+        L.mpres_dot_noalloc.argtypes = [vp, i32, vp, i32, vp, i32, vp, i32, i32, vp]
         L.mpres_dot_v2_workspace.restype = sz
         L.mpres_dot_v2_workspace.argtypes = [vp, i32, i32, i32]
         L.mpres_dot_v2.argtypes = [vp, i32, vp, vp, vp, i32, i32, vp, sz, vp]
