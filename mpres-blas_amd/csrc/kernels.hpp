@@ -714,6 +714,53 @@ __global__ __launch_bounds__(BLK) void k_gemm_chain(const DevConsts *__restrict_
     ev[1] = ErMem{acc.ev1.frac, (long long)acc.ev1.exp};
 }
 
+// k_gemm_chain for the v1 layout and op(A) = A, op(B) = B, with every element offset of A
+// and B below 2^28 (so byte offsets of the 16-byte evaluations fit 32 bits): each step
+// reads its operands at 32-bit offsets from uniform bases, advanced by lda and 1. UB: m is
+// a multiple of 64, so each wave's 64 entries share one column and the column's B fields
+// are wave-uniform; they are then read through scalar loads and held in scalar registers.
+template <int N, int L, bool UB>
+__global__ __launch_bounds__(BLK) void k_gemm_chain_nn(const DevConsts *__restrict__ C, int m, int n, int k, SoA A,
+                                                       int lda, SoA B, int ldb, AoSRef<N> sums) {
+    A = rd(A);
+    B = rd(B);
+    const long long e = (long long)blockIdx.x * BLK + threadIdx.x;
+    if (e >= (long long)m * n) return;
+    const int row = (int)(e % m);
+    int col = (int)(e / m);
+    if constexpr (UB) col = __builtin_amdgcn_readfirstlane(col);
+    const ScC K = sc_consts(C);
+    const char *__restrict__ as = (const char *)A.s, *__restrict__ ae = (const char *)A.e;
+    const char *__restrict__ a0 = (const char *)A.ev, *__restrict__ a1 = (const char *)(A.ev + A.len);
+    const int *__restrict__ bs = B.s;
+    const int *__restrict__ be = B.e;
+    const ErMem *__restrict__ b0 = B.ev;
+    const ErMem *__restrict__ b1 = B.ev + B.len;
+    Sc acc{0, 0, Er{0.0, 0}, Er{0.0, 0}};
+    bool ok = true;
+    unsigned ia = (unsigned)row;
+    const unsigned ib0 = (unsigned)col * (unsigned)ldb;
+    for (int i = 0; i < k && ok; i++, ia += (unsigned)lda) {
+        const unsigned o4 = ia << 2, o16 = ia << 4;
+        const int xs = *(const int *)(as + o4), xe = *(const int *)(ae + o4);
+        const ErMem xl = *(const ErMem *)(a0 + o16), xu = *(const ErMem *)(a1 + o16);
+        const unsigned ib = ib0 + (unsigned)i;
+        const int ys = bs[ib], ye = be[ib];
+        const ErMem yl = b0[ib], yu = b1[ib];
+        ok = sc_fma_step(K, acc, xs, xe, Er{xl.frac, (int)xl.exp}, Er{xu.frac, (int)xu.exp}, ys, ye,
+                         Er{yl.frac, (int)yl.exp}, Er{yu.frac, (int)yu.exp});
+    }
+    if (!ok) {
+        *sums.sign(e) = REDO;
+        return;
+    }
+    *sums.sign(e) = acc.sign;
+    *sums.exp(e) = acc.exp;
+    ErMem *ev = sums.eval(e);
+    ev[0] = ErMem{acc.ev0.frac, (long long)acc.ev0.exp};
+    ev[1] = ErMem{acc.ev1.frac, (long long)acc.ev1.exp};
+}
+
 // exact x mod m for an integer-valued 0 <= x < 2^53
 __device__ __forceinline__ double fmod_exact(double x, double md, double mv) {
     const double q = floor(x * mv);

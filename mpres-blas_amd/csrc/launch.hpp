@@ -228,6 +228,18 @@ struct Launch {
                 hipLaunchKernelGGL((k_gemm_limbs<N, L, VM, 1>), dim3(grid_for((long long)NP * KP, 1)), bl, 0, st, c, n,
                                    NP, k, KP, nl, tb, B, ldb, EB, PB);
             } else if (pass == 1) {
+                // no transposes and element offsets below 2^28: the specialised chain kernel
+                if constexpr (std::is_same<VM, SoA>::value) {
+                    if (ta == 0 && tb == 0 && (long long)lda * k <= (1LL << 28) && (long long)ldb * n <= (1LL << 28)) {
+                        if (m % 64 == 0)
+                            hipLaunchKernelGGL((k_gemm_chain_nn<N, L, true>), dim3(grid_for(mn, 1)), bl, 0, st, c, m,
+                                               n, k, A, lda, B, ldb, sums);
+                        else
+                            hipLaunchKernelGGL((k_gemm_chain_nn<N, L, false>), dim3(grid_for(mn, 1)), bl, 0, st, c, m,
+                                               n, k, A, lda, B, ldb, sums);
+                        return;
+                    }
+                }
                 hipLaunchKernelGGL((k_gemm_chain<N, L, VM>), dim3(grid_for(mn, 1)), bl, 0, st, c, ta, tb, m, n, k, A,
                                    lda, B, ldb, sums);
             } else if (pass == 2) {

@@ -7,9 +7,11 @@
 //
 // Directed rounding. The reference's CUDA path uses __dadd_rd / __dmul_ru /
 // __ddiv_rd ... (true IEEE directed rounding, e.g. src/extrange.cuh:458-641,
-// src/rns.cuh:807-862). gfx950 has no per-instruction rounding mode, so each
-// directed operation is derived from the round-to-nearest result and its
-// exact error term:
+// src/rns.cuh:807-862). gfx950 has no per-instruction rounding mode; on the GPU
+// a directed add, multiply or fma runs with the MODE register's FP64 round field
+// set around that one instruction (see hw_add below). On the host, and as the
+// definition those instructions are checked against, each directed operation is
+// derived from the round-to-nearest result and its exact error term:
 //   a + b = s + e   (Knuth's TwoSum, exact for finite inputs)
 //   a * b = p + e   (e = fma(a, b, -p), exact without underflow)
 //   a / b = q + r/b (r = fma(-q, b, a), exact without underflow)
@@ -52,31 +54,85 @@ MPH_HD double u2d(uint64_t x) {
 MPH_HD double step_down(double x) { return u2d(x > 0 ? d2u(x) - 1 : d2u(x) + 1); }
 MPH_HD double step_up(double x) { return u2d(x > 0 ? d2u(x) + 1 : d2u(x) - 1); }
 
+#if defined(__HIP_DEVICE_COMPILE__)
+// On the GPU the FP64 round-mode field of the MODE register (bits 3:2: 0 to nearest even,
+// 1 toward +inf, 2 toward -inf) gives IEEE directed rounding directly. Each block writes
+// the field, runs its one instruction and writes round-to-nearest back, with two wait
+// states after every write, so no instruction outside the block runs under a directed
+// mode and the block can be scheduled like any other instruction.
+#define MPH_DIRECTED(MODE, INSN)                                                              \
+    "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), " #MODE "\n\ts_nop 1\n\t" INSN "\n\ts_nop 1\n\t" \
+    "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 0\n\ts_nop 1"
+__device__ __forceinline__ double hw_add(double a, double b, bool up) {
+    double r;
+    if (up)
+        asm(MPH_DIRECTED(1, "v_add_f64 %0, %1, %2") : "=v"(r) : "v"(a), "v"(b));
+    else
+        asm(MPH_DIRECTED(2, "v_add_f64 %0, %1, %2") : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ double hw_mul(double a, double b, bool up) {
+    double r;
+    if (up)
+        asm(MPH_DIRECTED(1, "v_mul_f64 %0, %1, %2") : "=v"(r) : "v"(a), "v"(b));
+    else
+        asm(MPH_DIRECTED(2, "v_mul_f64 %0, %1, %2") : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ double hw_fma(double a, double b, double c, bool up) {
+    double r;
+    if (up)
+        asm(MPH_DIRECTED(1, "v_fma_f64 %0, %1, %2, %3") : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    else
+        asm(MPH_DIRECTED(2, "v_fma_f64 %0, %1, %2, %3") : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+#undef MPH_DIRECTED
+#endif
+
+// On the host (and in the host tests of this header) the same results come from the
+// error-free transforms described at the top of this file.
 MPH_HD double add_rd(double a, double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return hw_add(a, b, false);
+#else
     double s = a + b;
     double bb = s - a;
     double e = (a - (s - bb)) + (b - bb);
     if (e < 0) return step_down(s);
     if (s == 0) return (d2u(a) == 0 && d2u(b) == 0) ? 0.0 : -0.0;
     return s;
+#endif
 }
 MPH_HD double add_ru(double a, double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return hw_add(a, b, true);
+#else
     double s = a + b;
     double bb = s - a;
     double e = (a - (s - bb)) + (b - bb);
     return (e > 0) ? step_up(s) : s;
+#endif
 }
 MPH_HD double sub_rd(double a, double b) { return add_rd(a, -b); }
 MPH_HD double sub_ru(double a, double b) { return add_ru(a, -b); }
 MPH_HD double mul_rd(double a, double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return hw_mul(a, b, false);
+#else
     double p = a * b;
     double e = fma(a, b, -p);
     return (e < 0) ? step_down(p) : p;
+#endif
 }
 MPH_HD double mul_ru(double a, double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return hw_mul(a, b, true);
+#else
     double p = a * b;
     double e = fma(a, b, -p);
     return (e > 0) ? step_up(p) : p;
+#endif
 }
 MPH_HD double div_rd(double a, double b) {
     double q = a / b;
@@ -193,15 +249,27 @@ MPH_HD void recip2(double z, double &zh, double &zl) {
     zh = 1.0 / z;
     zl = fma(-zh, z, 1.0) / z;
 }
+// On the GPU the step is one directed fma: q + r 2^-64 lies strictly between q and its
+// neighbour on r's side when r != 0 (|r 2^-64| is far below half an ulp of q, and the fma
+// forms the sum exactly before its one rounding), so rounding it down (up) gives q's lower
+// (upper) neighbour exactly when q lies above (below) a/z, and q itself otherwise.
 MPH_HD double divk_rd(double a, double z, double zh, double zl) {
     const double q = fma(a, zh, a * zl);
     const double r = fma(-q, z, a);
-    return (a == 0) ? a : ((r < 0) ? step_down(q) : q);  // a zero quotient keeps a's sign
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (a == 0) ? a : hw_fma(r, 0x1p-64, q, false);  // a zero quotient keeps a's sign
+#else
+    return (a == 0) ? a : ((r < 0) ? step_down(q) : q);
+#endif
 }
 MPH_HD double divk_ru(double a, double z, double zh, double zl) {
     const double q = fma(a, zh, a * zl);
     const double r = fma(-q, z, a);
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (a == 0) ? a : hw_fma(r, 0x1p-64, q, true);
+#else
     return (a == 0) ? a : ((r > 0) ? step_up(q) : q);
+#endif
 }
 // er_md_rd / er_md_ru with z the interval evaluation's unit (frac in [1/2, 1)), through divk
 MPH_HD Er er_mdk_rd(Er x, Er y, double zf, int ze, double zh, double zl) {
