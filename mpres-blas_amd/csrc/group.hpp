@@ -131,8 +131,7 @@ __device__ __forceinline__ void sc_mul(const ScC &K, Sc &r, int xs, int xe, Er x
                                        Er y1) {
     r.exp = xe + ye;
     r.sign = xs ^ ys;
-    r.ev0 = er_mdk_rd(x0, y0, K.uuf, K.uue, K.uu_rh, K.uu_rl);
-    r.ev1 = er_mdk_ru(x1, y1, K.ulf, K.ule, K.ul_rh, K.ul_rl);
+    er_mdk_pair(x0, y0, x1, y1, K.uuf, K.uue, K.uu_rh, K.uu_rl, K.ulf, K.ule, K.ul_rh, K.ul_rl, r.ev0, r.ev1);
 }
 
 // acc = acc + x*y on the scalar fields, for chains whose every step is exact:
@@ -146,8 +145,8 @@ __device__ __forceinline__ bool sc_fma_step(const ScC &C, Sc &acc, int xs, int x
     if (sc_needs_round(C, p.ev1)) return false;
     const Align a = align_sc(C.mp_j, acc.sign, acc.exp, acc.ev0, acc.ev1, p.sign, p.exp, p.ev0, p.ev1);
     if ((!a.nzx && acc.ev1.frac != 0) || (!a.nzy && p.ev1.frac != 0)) return false;
-    const Er e0 = er_add_dir(a.sx ? a.x1 : a.x0, a.sy ? a.y1 : a.y0, false);
-    const Er e1 = er_add_dir(a.sx ? a.x0 : a.x1, a.sy ? a.y0 : a.y1, true);
+    Er e0, e1;
+    er_add_pair(a.sx ? a.x1 : a.x0, a.sy ? a.y1 : a.y0, a.sx ? a.x0 : a.x1, a.sy ? a.y0 : a.y1, e0, e1);
     const int sign = e0.frac < 0;
     if (sign != (e1.frac < 0)) return false;
     acc.sign = sign;
@@ -374,8 +373,8 @@ struct Grp {
     // cuda::mp_add (mp_add_common + rounding), src/arith/add.cuh:126-200
     __device__ __forceinline__ void add(MP<R> &res, const MP<R> &x, const MP<R> &y) const {
         const Align a = align(x, y);
-        Er e0 = er_add_dir(a.sx ? a.x1 : a.x0, a.sy ? a.y1 : a.y0, false);
-        Er e1 = er_add_dir(a.sx ? a.x0 : a.x1, a.sy ? a.y0 : a.y1, true);
+        Er e0, e1;
+        er_add_pair(a.sx ? a.x1 : a.x0, a.sy ? a.y1 : a.y0, a.sx ? a.x0 : a.x1, a.sy ? a.y0 : a.y1, e0, e1);
         int sign = e0.frac < 0;
         if (sign != (e1.frac < 0)) {
             sign = sign_estimate(x.d, y.d, a.sx, a.sy, a.gamma, a.theta, a.nzx, a.nzy);
@@ -501,8 +500,8 @@ struct Grp {
             return true;
         }
         const Align a = align(acc, p);
-        const Er e0 = er_add_dir(a.sx ? a.x1 : a.x0, a.sy ? a.y1 : a.y0, false);
-        const Er e1 = er_add_dir(a.sx ? a.x0 : a.x1, a.sy ? a.y0 : a.y1, true);
+        Er e0, e1;
+        er_add_pair(a.sx ? a.x1 : a.x0, a.sy ? a.y1 : a.y0, a.sx ? a.x0 : a.x1, a.sy ? a.y0 : a.y1, e0, e1);
         const int sign = e0.frac < 0;
         if (sign != (e1.frac < 0)) {
             if constexpr (!FULL) return false;
@@ -542,8 +541,8 @@ struct Grp {
             return true;
         } else {
             const Align a = align(x, y);
-            const Er e0 = er_add_dir(a.sx ? a.x1 : a.x0, a.sy ? a.y1 : a.y0, false);
-            const Er e1 = er_add_dir(a.sx ? a.x0 : a.x1, a.sy ? a.y0 : a.y1, true);
+            Er e0, e1;
+            er_add_pair(a.sx ? a.x1 : a.x0, a.sy ? a.y1 : a.y0, a.sx ? a.x0 : a.x1, a.sy ? a.y0 : a.y1, e0, e1);
             const int sign = e0.frac < 0;
             if (sign != (e1.frac < 0)) return false;
             const bool shx = a.gamma > 0;
@@ -570,8 +569,8 @@ struct Grp {
     // (no MRC on a straddling interval; residues via a double-reciprocal quotient)
     __device__ __forceinline__ void vadd(MP<R> &res, const MP<R> &x, const MP<R> &y) const {
         const Align a = align(x, y);
-        Er e0 = er_add_dir(a.sx ? a.x1 : a.x0, a.sy ? a.y1 : a.y0, false);
-        Er e1 = er_add_dir(a.sx ? a.x0 : a.x1, a.sy ? a.y0 : a.y1, true);
+        Er e0, e1;
+        er_add_pair(a.sx ? a.x1 : a.x0, a.sy ? a.y1 : a.y0, a.sx ? a.x0 : a.x1, a.sy ? a.y0 : a.y1, e0, e1);
         const int minus = e0.frac < 0 && e1.frac < 0;
         res.sign = minus;
         res.exp = (a.ex == 0) ? a.ey : a.ex;

@@ -738,17 +738,25 @@ __global__ __launch_bounds__(BLK) void k_gemm_chain_nn(const DevConsts *__restri
     const ErMem *__restrict__ b1 = B.ev + B.len;
     Sc acc{0, 0, Er{0.0, 0}, Er{0.0, 0}};
     bool ok = true;
-    unsigned ia = (unsigned)row;
     const unsigned ib0 = (unsigned)col * (unsigned)ldb;
-    for (int i = 0; i < k && ok; i++, ia += (unsigned)lda) {
-        const unsigned o4 = ia << 2, o16 = ia << 4;
-        const int xs = *(const int *)(as + o4), xe = *(const int *)(ae + o4);
-        const ErMem xl = *(const ErMem *)(a0 + o16), xu = *(const ErMem *)(a1 + o16);
-        const unsigned ib = ib0 + (unsigned)i;
-        const int ys = bs[ib], ye = be[ib];
-        const ErMem yl = b0[ib], yu = b1[ib];
-        ok = sc_fma_step(K, acc, xs, xe, Er{xl.frac, (int)xl.exp}, Er{xu.frac, (int)xu.exp}, ys, ye,
-                         Er{yl.frac, (int)yl.exp}, Er{yu.frac, (int)yu.exp});
+    // the operands of step i + 1 are loaded before step i runs (the last step reloads its
+    // own), so no load latency sits on the chain
+    struct Opnd {
+        int xs, xe, ys, ye;
+        ErMem xl, xu, yl, yu;
+    };
+    auto load = [&](int i) {
+        const unsigned ia = (unsigned)row + (unsigned)i * (unsigned)lda;
+        const unsigned o4 = ia << 2, o16 = ia << 4, ib = ib0 + (unsigned)i;
+        return Opnd{*(const int *)(as + o4), *(const int *)(ae + o4), bs[ib], be[ib],
+                    *(const ErMem *)(a0 + o16), *(const ErMem *)(a1 + o16), b0[ib], b1[ib]};
+    };
+    Opnd cur = load(0);
+    for (int i = 0; i < k && ok; i++) {
+        const Opnd nxt = load(i + 1 < k ? i + 1 : i);
+        ok = sc_fma_step(K, acc, cur.xs, cur.xe, Er{cur.xl.frac, (int)cur.xl.exp}, Er{cur.xu.frac, (int)cur.xu.exp},
+                         cur.ys, cur.ye, Er{cur.yl.frac, (int)cur.yl.exp}, Er{cur.yu.frac, (int)cur.yu.exp});
+        cur = nxt;
     }
     if (!ok) {
         *sums.sign(e) = REDO;
@@ -826,22 +834,46 @@ constexpr int GV_D = 64;
 constexpr int GV_T = 128;  // rows (threads) per workgroup of k_gemv_split
 // scalar chain of the partial (o, t), o fastest (one thread each, so that a
 // wavefront reads a run of one column of A's scalar fields per step)
-template <int N, int L>
+// NARROW: every element offset of A is below 2^28, so the 16-byte evaluations are read
+// at 32-bit byte offsets from uniform bases. UB: m is a multiple of 64, so each wave's
+// 64 chains share t and read the same bx entry; it then comes through scalar loads.
+// The operands of step j + b3 are loaded before step j runs (the last step reloads its
+// own), so the chain does not wait on memory.
+template <int N, int L, bool NARROW, bool UB>
 __global__ __launch_bounds__(BLK) void k_gemv_chain(const DevConsts *__restrict__ C, int m, int n, SoA A, int lda,
                                                     AoSRef<N> bx, int b3, AoSRef<N> part) {
     A = rd(A);
     const long long g = (long long)blockIdx.x * BLK + threadIdx.x;
     if (g >= (long long)m * b3) return;
-    const int o = (int)(g % m), t = (int)(g / m);
+    const int o = (int)(g % m);
+    int t = (int)(g / m);
+    if constexpr (UB) t = __builtin_amdgcn_readfirstlane(t);
     const ScC K = sc_consts(C);
+    const char *__restrict__ as = (const char *)A.s, *__restrict__ ae = (const char *)A.e;
+    const char *__restrict__ a0 = (const char *)A.ev, *__restrict__ a1 = (const char *)(A.ev + A.len);
+    const char *__restrict__ xb = bx.base;
+    struct Opnd {
+        int xs, xe, ys, ye;
+        ErMem xl, xu, yl, yu;
+    };
+    auto load = [&](int j) {
+        using Off = typename std::conditional<NARROW, unsigned, unsigned long long>::type;
+        const Off ia = (Off)o + (Off)j * (Off)lda, o4 = ia << 2, o16 = ia << 4;
+        const char *y = xb + (size_t)j * AoSRef<N>::STRIDE;
+        return Opnd{*(const int *)(as + o4), *(const int *)(ae + o4), *(const int *)(y + 4 * N),
+                    *(const int *)(y + 4 * N + 4), *(const ErMem *)(a0 + o16), *(const ErMem *)(a1 + o16),
+                    *(const ErMem *)(y + 4 * N + 8), *(const ErMem *)(y + 4 * N + 24)};
+    };
     Sc sc{0, 0, Er{0.0, 0}, Er{0.0, 0}};
     bool ok = true;
-    for (int j = t; j < n && ok; j += b3) {
-        int xs, xe, ys, ye;
-        Er x0, x1, y0, y1;
-        scalars_of<N>(A, o + (long long)j * lda, xs, xe, x0, x1);
-        scalars_of<N>(bx, j, ys, ye, y0, y1);
-        ok = sc_fma_step(K, sc, xs, xe, x0, x1, ys, ye, y0, y1);
+    if (t < n) {
+        Opnd cur = load(t);
+        for (int j = t; j < n && ok; j += b3) {
+            const Opnd nxt = load(j + b3 < n ? j + b3 : j);
+            ok = sc_fma_step(K, sc, cur.xs, cur.xe, Er{cur.xl.frac, (int)cur.xl.exp}, Er{cur.xu.frac, (int)cur.xu.exp},
+                             cur.ys, cur.ye, Er{cur.yl.frac, (int)cur.yl.exp}, Er{cur.yu.frac, (int)cur.yu.exp});
+            cur = nxt;
+        }
     }
     const long long v = (long long)o * b3 + t;
     if (!ok) {

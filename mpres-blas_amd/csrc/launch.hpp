@@ -151,8 +151,15 @@ struct Launch {
                 hipLaunchKernelGGL((k_gemm_emin<N, L, AoSRef<N>, 1>), dim3(1, kc), bl, 0, st, 1, n, 0, bx, 0, EB);
                 hipLaunchKernelGGL((k_gemm_prescale<N, L, AoSRef<N>, 1>), dim3(grid_for(n, 1)), bl, 0, st, c, 1, n, 0,
                                    bx, 0, EB, bxp);
-                hipLaunchKernelGGL((k_gemv_chain<N, L>), dim3(grid_for((long long)m * b3, 1)), bl, 0, st, c, m, n, A,
-                                   lda, bx, b3, part);
+                const dim3 gc(grid_for((long long)m * b3, 1));
+                const bool narrow = (long long)lda * n <= (1LL << 28);
+                if (narrow && m % 64 == 0)
+                    hipLaunchKernelGGL((k_gemv_chain<N, L, true, true>), gc, bl, 0, st, c, m, n, A, lda, bx, b3, part);
+                else if (narrow)
+                    hipLaunchKernelGGL((k_gemv_chain<N, L, true, false>), gc, bl, 0, st, c, m, n, A, lda, bx, b3, part);
+                else
+                    hipLaunchKernelGGL((k_gemv_chain<N, L, false, false>), gc, bl, 0, st, c, m, n, A, lda, bx, b3,
+                                       part);
             }
         } else if (pass == 1) {
             if (split)

@@ -87,6 +87,30 @@ __device__ __forceinline__ double hw_fma(double a, double b, double c, bool up) 
         asm(MPH_DIRECTED(2, "v_fma_f64 %0, %1, %2, %3") : "=v"(r) : "v"(a), "v"(b), "v"(c));
     return r;
 }
+// Pairs: the lower bound's operation rounded down and the upper bound's rounded up in
+// one block (three mode writes for two instructions). The lower result is written
+// before the upper operands are read, hence the early-clobber output.
+#define MPH_PAIR(INSN_LO, INSN_HI)                                                          \
+    "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 2\n\ts_nop 1\n\t" INSN_LO "\n\ts_nop 0\n\t"     \
+    "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 1\n\ts_nop 1\n\t" INSN_HI "\n\ts_nop 0\n\t"     \
+    "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 0\n\ts_nop 1"
+__device__ __forceinline__ void hw_add_pair(double a, double b, double c, double d, double &lo, double &hi) {
+    asm(MPH_PAIR("v_add_f64 %0, %2, %3", "v_add_f64 %1, %4, %5")
+        : "=&v"(lo), "=v"(hi)
+        : "v"(a), "v"(b), "v"(c), "v"(d));
+}
+__device__ __forceinline__ void hw_mul_pair(double a, double b, double c, double d, double &lo, double &hi) {
+    asm(MPH_PAIR("v_mul_f64 %0, %2, %3", "v_mul_f64 %1, %4, %5")
+        : "=&v"(lo), "=v"(hi)
+        : "v"(a), "v"(b), "v"(c), "v"(d));
+}
+__device__ __forceinline__ void hw_fma_pair(double a, double b, double c, double d, double e, double f, double &lo,
+                                            double &hi) {
+    asm(MPH_PAIR("v_fma_f64 %0, %2, %3, %4", "v_fma_f64 %1, %5, %6, %7")
+        : "=&v"(lo), "=v"(hi)
+        : "v"(a), "v"(b), "v"(c), "v"(d), "v"(e), "v"(f));
+}
+#undef MPH_PAIR
 #undef MPH_DIRECTED
 #endif
 
@@ -211,13 +235,12 @@ MPH_HD int er_ucmp(Er x, Er y) {
     if (diff == 0) return x.frac == y.frac ? 0 : (x.frac > y.frac ? 1 : -1);
     return (x.frac != 0 && (diff > 0 || (diff < 0 && y.frac == 0))) ? 1 : -1;
 }
-// er_add_rd / er_add_ru (CUDA), src/extrange.cuh:450-490
-MPH_HD Er er_add_dir(Er x, Er y, bool up) {
+// er_add_rd / er_add_ru (CUDA), src/extrange.cuh:450-490: the operands aligned to the
+// larger exponent e, then one directed addition and er_adjust
+MPH_HD void er_add_align(Er x, Er y, double &a, double &b, int &e) {
     x.exp = (x.frac != 0) ? x.exp : 0;
     y.exp = (y.frac != 0) ? y.exp : 0;
     const int dexp = (x.frac != 0 && y.frac != 0) ? x.exp - y.exp : 0;
-    double a, b;
-    int e;
     if (dexp > 0) {
         e = x.exp;
         a = x.frac;
@@ -231,8 +254,28 @@ MPH_HD Er er_add_dir(Er x, Er y, bool up) {
         a = x.frac;
         b = y.frac;
     }
+}
+MPH_HD Er er_add_dir(Er x, Er y, bool up) {
+    double a, b;
+    int e;
+    er_add_align(x, y, a, b, e);
     const double f = up ? add_ru(a, b) : add_rd(a, b);
     return er_adjust(f, e);
+}
+// lo = er_add_rd(xl, yl), hi = er_add_ru(xu, yu)
+MPH_HD void er_add_pair(Er xl, Er yl, Er xu, Er yu, Er &lo, Er &hi) {
+    double al, bl, au, bu;
+    int el, eu;
+    er_add_align(xl, yl, al, bl, el);
+    er_add_align(xu, yu, au, bu, eu);
+#if defined(__HIP_DEVICE_COMPILE__)
+    double fl, fu;
+    hw_add_pair(al, bl, au, bu, fl, fu);
+#else
+    const double fl = add_rd(al, bl), fu = add_ru(au, bu);
+#endif
+    lo = er_adjust(fl, el);
+    hi = er_adjust(fu, eu);
 }
 // er_md_rd / er_md_ru: x*y/z, src/extrange.cuh:627-644
 MPH_HD Er er_md_rd(Er x, Er y, Er z) { return er_adjust(div_rd(mul_rd(x.frac, y.frac), z.frac), x.exp + y.exp - z.exp); }
@@ -277,6 +320,25 @@ MPH_HD Er er_mdk_rd(Er x, Er y, double zf, int ze, double zh, double zl) {
 }
 MPH_HD Er er_mdk_ru(Er x, Er y, double zf, int ze, double zh, double zl) {
     return er_adjust(divk_ru(mul_ru(x.frac, y.frac), zf, zh, zl), x.exp + y.exp - ze);
+}
+// lo = er_mdk_rd(x0, y0), hi = er_mdk_ru(x1, y1) with the units (zf0, ze0) and (zf1, ze1)
+MPH_HD void er_mdk_pair(Er x0, Er y0, Er x1, Er y1, double zf0, int ze0, double zh0, double zl0, double zf1, int ze1,
+                        double zh1, double zl1, Er &lo, Er &hi) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double pl, ph;
+    hw_mul_pair(x0.frac, y0.frac, x1.frac, y1.frac, pl, ph);
+    const double ql = fma(pl, zh0, pl * zl0), qh = fma(ph, zh1, ph * zl1);
+    const double rl = fma(-ql, zf0, pl), rh = fma(-qh, zf1, ph);
+    double dl, dh;
+    hw_fma_pair(rl, 0x1p-64, ql, rh, 0x1p-64, qh, dl, dh);  // the steps of divk_rd / divk_ru
+    dl = (pl == 0) ? pl : dl;
+    dh = (ph == 0) ? ph : dh;
+#else
+    const double dl = divk_rd(mul_rd(x0.frac, y0.frac), zf0, zh0, zl0);
+    const double dh = divk_ru(mul_ru(x1.frac, y1.frac), zf1, zh1, zl1);
+#endif
+    lo = er_adjust(dl, x0.exp + y0.exp - ze0);
+    hi = er_adjust(dh, x1.exp + y1.exp - ze1);
 }
 
 // ---------------------------------------------------------------------------
