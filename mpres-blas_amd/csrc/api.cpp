@@ -91,6 +91,10 @@ struct mpres_ctx {
     const Ops *ops[OP_COUNT] = {};
     void *ws = nullptr;
     size_t ws_bytes = 0;
+    // a second stream (non-blocking, default priority) for work of one call that does not
+    // depend on the rest (the gemm residue path), with its fork and join events
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     // the pool's last use: an event recorded on the stream of the last call that used it
     hipEvent_t ws_ev = nullptr;
     hipStream_t ws_stream = nullptr;
@@ -156,6 +160,32 @@ struct WsUse {
             c->ws_stream = st;
             c->ws_used = true;
         }
+    }
+};
+
+// Runs f(stream) on the context's side stream, ordered after the work already queued on
+// st; join() makes st wait for it. Without a side stream (its creation failed) f runs on st.
+struct Fork {
+    mpres_ctx *c;
+    hipStream_t st, s1;
+    Fork(mpres_ctx *c_, hipStream_t st_) : c(c_), st(st_), s1(st_) {
+        if (!c->side) {
+            if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) c->side = nullptr;
+            if (c->side && (hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess ||
+                            hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess)) {
+                (void)hipStreamDestroy(c->side);
+                c->side = nullptr;
+            }
+        }
+        if (c->side && hipEventRecord(c->fork_ev, st) == hipSuccess &&
+            hipStreamWaitEvent(c->side, c->fork_ev, 0) == hipSuccess)
+            s1 = c->side;
+    }
+    hipError_t join() {
+        if (s1 == st) return hipSuccess;
+        hipError_t e = hipEventRecord(c->join_ev, s1);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, c->join_ev, 0);
+        return e;
     }
 };
 
@@ -275,6 +305,9 @@ int mpres_ctx_destroy(mpres_ctx_t *c) {
     (void)hipDeviceSynchronize();
     if (c->ws) (void)hipFree(c->ws);
     if (c->ws_ev) (void)hipEventDestroy(c->ws_ev);
+    if (c->side) (void)hipStreamDestroy(c->side);
+    if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+    if (c->join_ev) (void)hipEventDestroy(c->join_ev);
     for (auto &e : c->prof_live) c->prof_free.push_back(e.second);
     for (auto &e : c->prof_free) {
         (void)hipEventDestroy(e.first);
@@ -658,11 +691,22 @@ int mpres_gemm(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const mp_arr
     int rc = wu.get(ws, wsb, mpres_gemm_workspace(c, m, n, k), &w);
     if (rc) return rc;
     const Ops *o = c->ops[OP_GEMM];
-    for (int pass = 0; pass < 5; pass++) {
-        ProfScope ps(c, pass == 1 ? OP_GEMM : pass == 2 ? PROF_GEMM_RESID : -1, strm(st));
-        HIPCHK(o->gemm(c->dev, pass, c->bits, m, n, k, soa_of(alpha), soa_of(A), lda, soa_of(B), ldb, soa_of(beta),
+    // the residue path (pass 0) runs on the side stream while the chains (pass 1) run on st
+    Fork fk(c, strm(st));
+    {
+        ProfScope ps(c, PROF_GEMM_RESID, fk.s1);
+        HIPCHK(o->gemm(c->dev, 0, c->bits, m, n, k, soa_of(alpha), soa_of(A), lda, soa_of(B), ldb, soa_of(beta),
+                       soa_of(C), ldc, w, fk.s1));
+    }
+    {
+        ProfScope ps(c, OP_GEMM, strm(st));
+        HIPCHK(o->gemm(c->dev, 1, c->bits, m, n, k, soa_of(alpha), soa_of(A), lda, soa_of(B), ldb, soa_of(beta),
                        soa_of(C), ldc, w, strm(st)));
     }
+    HIPCHK(fk.join());
+    for (int pass = 2; pass < 5; pass++)
+        HIPCHK(o->gemm(c->dev, pass, c->bits, m, n, k, soa_of(alpha), soa_of(A), lda, soa_of(B), ldb, soa_of(beta),
+                       soa_of(C), ldc, w, strm(st)));
     return MPRES_OK;
 }
 
@@ -702,9 +746,14 @@ int mpres_gemm_v2(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const voi
     int rc = wu.get(ws, wsb, mpres_gemm_workspace(c, m, n, k), &w);
     if (rc) return rc;
     const int tra = ta == MPRES_TRANS || ta == MPRES_CONJ_TRANS, trb = tb == MPRES_TRANS || tb == MPRES_CONJ_TRANS;
-    for (int pass = 0; pass < 5; pass++)
-        HIPCHK(c->ops[OP_GEMM]->gemm_v2(c->dev, pass, c->bits, tra, trb, m, n, k, (const char *)alpha, (const char *)A, lda,
-                                        (const char *)B, ldb, (const char *)beta, (char *)C, ldc, w, strm(st)));
+    // as in mpres_gemm: the residue path (pass 0) on the side stream, the chains on st
+    Fork fk(c, strm(st));
+    for (int pass = 0; pass < 5; pass++) {
+        if (pass == 2) HIPCHK(fk.join());
+        HIPCHK(c->ops[OP_GEMM]->gemm_v2(c->dev, pass, c->bits, tra, trb, m, n, k, (const char *)alpha, (const char *)A,
+                                        lda, (const char *)B, ldb, (const char *)beta, (char *)C, ldc, w,
+                                        pass == 0 ? fk.s1 : strm(st)));
+    }
     return MPRES_OK;
 }
 

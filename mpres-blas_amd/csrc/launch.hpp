@@ -206,6 +206,11 @@ struct Launch {
                      KPd = (size_t)((k > 0 ? k : 1) + LIMB_T - 1) / LIMB_T * LIMB_T;
         return sums + ints + (size_t)N * GEMM_NL * (MPd + NPd) * KPd;
     }
+    // Passes of the split-form gemm: 0 = exponent minima, int8 limb planes and the INT8 MFMA
+    // residue products; 1 = the scalar chains; 2 = digits from both (needs 0 and 1); 3 = the
+    // fixup of REDO chains; 4 = the alpha/beta epilogue. Passes 0 and 1 are independent, so a
+    // caller may run pass 0 on another stream. Over 8 moduli only pass 1 (the lane-group
+    // reduction) and pass 4 do work.
     template <class VM>
     static void gemm_pass(int pass, const DevConsts *c, int bits, int ta, int tb, int m, int n, int k, VM A, int lda,
                           VM B, int ldb, char *ws, hipStream_t st) {
@@ -234,6 +239,7 @@ struct Launch {
                                    MP, k, KP, nl, ta, A, lda, EA, PA);
                 hipLaunchKernelGGL((k_gemm_limbs<N, L, VM, 1>), dim3(grid_for((long long)NP * KP, 1)), bl, 0, st, c, n,
                                    NP, k, KP, nl, tb, B, ldb, EB, PB);
+                mf_products(c, nl, m, n, MP, NP, KP, PA, PB, nullptr, nullptr, S, st);
             } else if (pass == 1) {
                 // no transposes and element offsets below 2^28: the specialised chain kernel
                 if constexpr (std::is_same<VM, SoA>::value) {
@@ -250,7 +256,6 @@ struct Launch {
                 hipLaunchKernelGGL((k_gemm_chain<N, L, VM>), dim3(grid_for(mn, 1)), bl, 0, st, c, ta, tb, m, n, k, A,
                                    lda, B, ldb, sums);
             } else if (pass == 2) {
-                mf_products(c, nl, m, n, MP, NP, KP, PA, PB, nullptr, nullptr, S, st);
                 hipLaunchKernelGGL((k_gemm_digits<N, L>), dim3(grid_for(mn, 1)), bl, 0, st, c, m, n, S, EA, EB, sums);
             } else if (pass == 3) {
                 hipLaunchKernelGGL((k_gemm_sum<N, L, true, true, VM>), gr, bl, 0, st, c, ta, tb, m, n, k, A, lda, B,
