@@ -994,6 +994,112 @@ __global__ __launch_bounds__(GV_T) void k_gemv_split(const DevConsts *__restrict
     }
 }
 
+// Same partials as k_gemv_split, with the N residues of a row spread over N / GR
+// threads (GR residues each) so that m * b3 * N / GR threads hide the memory latency
+// that one thread per row cannot at these grid sizes. Thread q of a workgroup holds
+// residues (q % TPR) * GR .. + GR - 1 of row q / TPR: a wave's two 16-byte loads per
+// step cover a contiguous run of whole rows of column j, so A's residues need no LDS
+// staging. The multiples table (bxp 2^d, d < 64) is double-buffered in LDS: one
+// barrier per step. Rare terms (d >= 64) fold 2^(d - d mod 64) into the thread's own
+// residues first, as in k_gemv_split.
+constexpr int GR = 8, GV_RT = 512;  // residues per thread, threads per workgroup of k_gemv_resid
+template <int N, int L>
+__global__ __launch_bounds__(GV_RT) void k_gemv_resid(const DevConsts *__restrict__ C, int m, int n, SoA A, int lda,
+                                                      const int *__restrict__ bxp, const int *__restrict__ EA,
+                                                      const int *__restrict__ EB, int b3, AoSRef<N> part) {
+    static_assert(N % GR == 0 && (GV_D * N) % GV_RT == 0, "residue groups");
+    constexpr int TPR = N / GR, RW = GV_RT / TPR, SP = N + 1, DB = GV_D * N / GV_RT;
+    __shared__ int tp[2][GV_D * SP];
+    A = rd(A);
+    const int t = blockIdx.y;
+    const int g0 = (threadIdx.x % TPR) * GR;
+    const long long o = (long long)blockIdx.x * RW + threadIdx.x / TPR;
+    const bool live = o < m;
+    const long long v = o * b3 + t;
+    // the chain's result (k_gemv_chain): sign and exponent, or REDO
+    const int csign = live ? *part.sign(v) : REDO;
+    const int cexp = live ? *part.exp(v) : 0;
+    const bool ok = live && csign != REDO;
+    const int ea = live ? EA[o] : 0;
+    double md[GR], mv[GR], acc[GR];
+#pragma unroll
+    for (int r = 0; r < GR; r++) {
+        md[r] = C->md[g0 + r];
+        mv[r] = C->minv[g0 + r];
+        acc[r] = 0.0;
+    }
+    int since = 0;
+    // table build: thread q -> residue q % N, multiples d0 .. d0 + DB - 1
+    const int tr = threadIdx.x % N, td0 = (threadIdx.x / N) * DB;
+    const double tmd = C->md[tr], tmv = C->minv[tr];
+    const int tm = C->m[tr];
+    const int t2 = C->pow2[(size_t)td0 * N + tr];
+    // step j's table seed, sign, exponent and residues are loaded during step j - b3
+    int nb = 0, nxs = 0, nxe = 0;
+    int4 nd0 = make_int4(0, 0, 0, 0), nd1 = nd0;
+    auto fetch = [&](int j) {
+        nb = bxp[(size_t)tr * n + j];
+        if (ok) {
+            const long long ia = o + (long long)j * lda;
+            nxs = A.s[ia];
+            nxe = A.e[ia];
+            const int4 *p = (const int4 *)(A.d + ia * N + g0);
+            nd0 = p[0];
+            nd1 = p[1];
+        }
+    };
+    if (t < n) fetch(t);
+    int buf = 0;
+    for (int j = t; j < n; j += b3, buf ^= 1) {
+        int *tb = tp[buf];
+        int tv = mulmod(nb, t2, tmd, tmv);
+#pragma unroll
+        for (int d = 0; d < DB; d++) {
+            tb[(td0 + d) * SP + tr] = tv;
+            tv += tv;
+            tv = tv >= tm ? tv - tm : tv;
+        }
+        const int xs = nxs, xe = nxe;
+        int xd[GR] = {nd0.x, nd0.y, nd0.z, nd0.w, nd1.x, nd1.y, nd1.z, nd1.w};
+        if (j + b3 < n) fetch(j + b3);
+        __syncthreads();
+        if (ok) {
+            int d = xe - ea;
+            if (d >= GV_D) {
+                const int dh = d - (d & (GV_D - 1));
+#pragma unroll
+                for (int r = 0; r < GR; r++) xd[r] = mulmod(xd[r], pow2mod(C, dh, g0 + r), md[r], mv[r]);
+                d &= GV_D - 1;
+            }
+            const double sg = xs ? -1.0 : 1.0;
+            const int *f = tb + d * SP + g0;
+#pragma unroll
+            for (int r = 0; r < GR; r++) {
+                const double x = (double)xd[r], fr = (double)f[r];
+                const double p = x * fr, pl = fma(x, fr, -p);
+                const double q = floor(p * mv[r]);
+                acc[r] = fma(sg, fma(-q, md[r], p) + pl, acc[r]);
+            }
+            if (++since == GV_REDUCE) {
+                since = 0;
+#pragma unroll
+                for (int r = 0; r < GR; r++) acc[r] = fmod_exact(acc[r], md[r], mv[r]);
+            }
+        }
+    }
+    if (!ok) return;
+    const int dz = ea + EB[0] - cexp;
+    int out[GR];
+#pragma unroll
+    for (int r = 0; r < GR; r++) {
+        const int w = mulmod((int)fmod_exact(acc[r], md[r], mv[r]), pow2mod(C, dz, g0 + r), md[r], mv[r]);
+        out[r] = (csign && w) ? C->m[g0 + r] - w : w;
+    }
+    int *dig = part.digits(v) + g0;
+#pragma unroll
+    for (int r = 0; r < GR; r++) dig[r] = out[r];
+}
+
 // ---------------------------------------------------------------- SpMV
 // Row partial sums of tpr strided lanes (src/sparse/spmv/spmv_csrv.cuh:61-65; tpr = 1
 // is the scalar kernel spmv_csr.cuh:53-62), combined by the 16/8/4/2/1 tree

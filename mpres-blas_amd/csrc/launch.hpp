@@ -162,9 +162,11 @@ struct Launch {
                                        part);
             }
         } else if (pass == 1) {
-            if (split)
-                hipLaunchKernelGGL((k_gemv_split<N, L>), dim3((m + GV_T - 1) / GV_T, b3), dim3(GV_T), 0, st, c, m, n, A, lda, bx,
-                                   bxp, EA, EB, b3, part);
+            if (split) {
+                constexpr int RW = GV_RT * GR / N;  // rows per workgroup of k_gemv_resid
+                hipLaunchKernelGGL((k_gemv_resid<N, L>), dim3((m + RW - 1) / RW, b3), dim3(GV_RT), 0, st, c, m, n, A,
+                                   lda, bxp, EA, EB, b3, part);
+            }
             else if (!TR && b3 <= 65535)  // grid y limit
                 hipLaunchKernelGGL((k_gemv_rows<N, L, !SPLIT>), dim3((m + BLK / L - 1) / (BLK / L), b3), bl, 0, st, c,
                                    m, n, A, lda, bx, b3, part);
