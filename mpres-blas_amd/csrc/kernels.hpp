@@ -823,15 +823,14 @@ __global__ __launch_bounds__(BLK) void k_gemm_digits(const DevConsts *__restrict
 // the interval evaluation come from the interval chain on scalars alone. So the
 // partial (o, t) is formed by two kernels: k_gemv_chain runs the chain
 // (sc_fma_step) over A's scalar fields and writes sign, exponent and evaluation,
-// or REDO; k_gemv_split then accumulates the residues against the fixed exponent
+// or REDO; k_gemv_resid then accumulates the residues against the fixed exponent
 // E_ref = EA[o] + EB (EA[o]: minimum exponent of row o of A, EB: minimum exponent
 // of bx) and rescales once by 2^(E_ref - E). bx is pre-scaled
 // (bxp[r][j] = s x_r 2^(e_j - EB) mod m_r); per step the workgroup builds in LDS
 // the 64 multiples bxp[r][j] 2^d (d < 64), so a term costs one product mod m per
-// residue; d >= 64 takes the slow path. A's residues are staged through LDS as
-// in k_gemv_rows. Chains marked REDO are left to the fixup kernel.
+// residue; d >= 64 takes the slow path. Chains marked REDO are left to the fixup
+// kernel.
 constexpr int GV_D = 64;
-constexpr int GV_T = 128;  // rows (threads) per workgroup of k_gemv_split
 // scalar chain of the partial (o, t), o fastest (one thread each, so that a
 // wavefront reads a run of one column of A's scalar fields per step)
 // NARROW: every element offset of A is below 2^28, so the 16-byte evaluations are read
@@ -888,120 +887,14 @@ __global__ __launch_bounds__(BLK) void k_gemv_chain(const DevConsts *__restrict_
 }
 
 constexpr int GV_REDUCE = 1 << 20;  // terms between reductions of the double accumulators
-template <int N, int L>
-__global__ __launch_bounds__(GV_T) void k_gemv_split(const DevConsts *__restrict__ C, int m, int n, SoA A, int lda,
-                                                    AoSRef<N> bx, const int *__restrict__ bxp,
-                                                    const int *__restrict__ EA, const int *__restrict__ EB, int b3,
-                                                    AoSRef<N> part) {
-    constexpr int SP = N + 1;
-    __shared__ int sd[GV_T * SP];
-    __shared__ int tp[GV_D * SP];
-    A = rd(A);
-    const int t = blockIdx.y;
-    const long long o0 = (long long)blockIdx.x * GV_T, o = o0 + threadIdx.x;
-    const int rows = (int)((m - o0) < GV_T ? (m - o0) : GV_T);
-    const bool live = o < m;
-    const long long v = o * b3 + t;
-    // the chain's result (k_gemv_chain): sign and exponent, or REDO
-    const int csign = live ? *part.sign(v) : REDO;
-    const int cexp = live ? *part.exp(v) : 0;
-    const bool ok = live && csign != REDO;
-    const int ea = live ? EA[o] : 0;
-    // residue sums s_j x_j (bxp 2^d)_j accumulate unreduced: each term
-    // w = x f - q m (exact, |w| < 2m + 1 < 2^28) is added in FP64, which stays
-    // exact for GV_REDUCE terms between reductions
-    double acc[N];
-#pragma unroll
-    for (int r = 0; r < N; r++) acc[r] = 0.0;
-    int since = 0;
-    // table build: thread q -> residue q % N, multiples d0 .. d0 + DB - 1
-    constexpr int DB = GV_D * N / GV_T;
-    const int tr = threadIdx.x % N, td0 = (threadIdx.x / N) * DB;
-    const double tmd = C->md[tr], tmv = C->minv[tr];
-    const int tm = C->m[tr];
-    const int t2 = C->pow2[(size_t)td0 * N + tr];
-    // step j's table seed, A's sign and exponent and A's run of residues are
-    // fetched during step j - b3 (the run into registers), so step j's loads are
-    // in flight while step j - b3 computes
-    RunStage<N, 1, GV_T> stg;
-    int nb = 0, nxs = 0, nxe = 0;
-    auto fetch = [&](int j) {
-        nb = bxp[(size_t)tr * n + j];
-        if (ok) {
-            const long long ia = o + (long long)j * lda;
-            nxs = A.s[ia];
-            nxe = A.e[ia];
-        }
-    };
-    if (t < n) {
-        fetch(t);
-        stg.fetch(A.d + (o0 + (long long)t * lda) * N, rows);
-    }
-    for (int j = t; j < n; j += b3) {
-        stg.store(sd, A.d + (o0 + (long long)j * lda) * N, rows);
-        int tv = mulmod(nb, t2, tmd, tmv);
-        for (int d = 0; d < DB; d++) {
-            tp[(td0 + d) * SP + tr] = tv;
-            tv += tv;
-            tv = tv >= tm ? tv - tm : tv;
-        }
-        const int xs = nxs, xe = nxe;
-        __syncthreads();
-        if (j + b3 < n) {
-            fetch(j + b3);
-            stg.fetch(A.d + (o0 + (long long)(j + b3) * lda) * N, rows);
-        }
-        if (ok) {
-            int d = xe - ea;
-            int *xd = sd + threadIdx.x * SP;
-            if (d >= GV_D) {
-                // rare: fold 2^(d - d mod 64) into this thread's own staged residues
-                const int dh = d - (d & (GV_D - 1));
-                for (int r = 0; r < N; r++) xd[r] = mulmod(xd[r], pow2mod(C, dh, r), C->md[r], C->minv[r]);
-                d &= GV_D - 1;
-            }
-            const double sg = xs ? -1.0 : 1.0;
-            const int *f = tp + d * SP;
-            // groups of 8 residues, fenced so that the compiler does not hoist
-            // all LDS loads (and their registers) ahead of the arithmetic
-#pragma unroll
-            for (int r0 = 0; r0 < N; r0 += 8) {
-#pragma unroll
-                for (int r = r0; r < r0 + 8; r++) {
-                    const double x = (double)xd[r], fr = (double)f[r];
-                    const double p = x * fr, pl = fma(x, fr, -p);
-                    const double q = floor(p * C->minv[r]);
-                    acc[r] = fma(sg, fma(-q, C->md[r], p) + pl, acc[r]);
-                }
-                asm volatile("" ::: "memory");
-            }
-            if (++since == GV_REDUCE) {
-                since = 0;
-#pragma unroll
-                for (int r = 0; r < N; r++) acc[r] = fmod_exact(acc[r], C->md[r], C->minv[r]);
-            }
-        }
-        __syncthreads();
-    }
-    if (!ok) return;
-    const int dz = ea + EB[0] - cexp;
-    int *dig = part.digits(v);
-#pragma unroll
-    for (int r = 0; r < N; r++) dig[r] = (int)fmod_exact(acc[r], C->md[r], C->minv[r]);
-    for (int r = 0; r < N; r++) {
-        const int w = mulmod(dig[r], pow2mod(C, dz, r), C->md[r], C->minv[r]);
-        dig[r] = (csign && w) ? C->m[r] - w : w;
-    }
-}
-
-// Same partials as k_gemv_split, with the N residues of a row spread over N / GR
-// threads (GR residues each) so that m * b3 * N / GR threads hide the memory latency
-// that one thread per row cannot at these grid sizes. Thread q of a workgroup holds
+// The residues of the partials whose scalar chains k_gemv_chain ran. The N residues of
+// a row are spread over N / GR threads (GR residues each) so that m * b3 * N / GR
+// threads hide the memory latency that one thread per row cannot at these grid sizes. Thread q of a workgroup holds
 // residues (q % TPR) * GR .. + GR - 1 of row q / TPR: a wave's two 16-byte loads per
 // step cover a contiguous run of whole rows of column j, so A's residues need no LDS
 // staging. The multiples table (bxp 2^d, d < 64) is double-buffered in LDS: one
-// barrier per step. Rare terms (d >= 64) fold 2^(d - d mod 64) into the thread's own
-// residues first, as in k_gemv_split.
+// barrier per step. Rare terms (d >= 64) first fold 2^(d - d mod 64) into the
+// thread's own residues.
 constexpr int GR = 8, GV_RT = 512;  // residues per thread, threads per workgroup of k_gemv_resid
 template <int N, int L>
 __global__ __launch_bounds__(GV_RT) void k_gemv_resid(const DevConsts *__restrict__ C, int m, int n, SoA A, int lda,

@@ -237,23 +237,15 @@ MPH_HD int er_ucmp(Er x, Er y) {
 }
 // er_add_rd / er_add_ru (CUDA), src/extrange.cuh:450-490: the operands aligned to the
 // larger exponent e, then one directed addition and er_adjust
+// (written without branches: for dexp = 0, ldexp(y.frac, 0) = y.frac and e is x.exp
+// unless x.exp is 0)
 MPH_HD void er_add_align(Er x, Er y, double &a, double &b, int &e) {
-    x.exp = (x.frac != 0) ? x.exp : 0;
-    y.exp = (y.frac != 0) ? y.exp : 0;
-    const int dexp = (x.frac != 0 && y.frac != 0) ? x.exp - y.exp : 0;
-    if (dexp > 0) {
-        e = x.exp;
-        a = x.frac;
-        b = ldexp(y.frac, -dexp);
-    } else if (dexp < 0) {
-        e = y.exp;
-        a = y.frac;
-        b = ldexp(x.frac, dexp);
-    } else {
-        e = (x.exp == 0) ? y.exp : x.exp;
-        a = x.frac;
-        b = y.frac;
-    }
+    const int xe = (x.frac != 0) ? x.exp : 0, ye = (y.frac != 0) ? y.exp : 0;
+    const int dexp = (x.frac != 0 && y.frac != 0) ? xe - ye : 0;
+    const bool xg = dexp >= 0;
+    a = xg ? x.frac : y.frac;
+    b = ldexp(xg ? y.frac : x.frac, xg ? -dexp : dexp);
+    e = xg ? ((dexp == 0 && xe == 0) ? ye : xe) : ye;
 }
 MPH_HD Er er_add_dir(Er x, Er y, bool up) {
     double a, b;
