@@ -31,14 +31,16 @@ sys.path[:0] = [os.path.join(ROOT, "tests")]
 import gen  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
-PROF_GEMM_RESID = 6        # library timing slot of the gemm residue products
-# HBM bytes per launch of the step's main kernels on this workload, from rocprofv3
-# --pmc FETCH_SIZE and --pmc WRITE_SIZE (profiles/bench_424bit_counters.md), each
-# scaled by its reading for a 256 MiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0)
-TRAFFIC_BYTES = {"chain": 104.2e6, "resid": 435.9e6, "gemv": 2260.7e6}
-# VALU instructions per launch of the gemm scalar chains (SQ_INSTS_VALU, same profile)
-# and the issue peak: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction
-CHAIN_VALU_INSTS = 4.28e8
+PROF_GEMM_RESID = 6        # library timing slot of the gemm residue path (side stream)
+# Constants from the committed counter profile of this workload (PROFILE below): HBM bytes
+# per launch from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE, each scaled by its
+# reading for a 256 MiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0), and the VALU
+# wave-instructions per launch of the gemm chains (SQ_INSTS_VALU). They are not measured
+# in this run, and the JSON line names them as profile values.
+PROFILE = "profiles/bench_424bit_counters.md"
+TRAFFIC_BYTES = {"chain": 153.9e6, "gemv": 2337.0e6}
+CHAIN_VALU_INSTS = 3.336e8
+# VALU issue peak: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction
 VALU_ISSUE_PEAK = 614.4e9
 FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 (vector and matrix)
 INT8_PEAK_TOPS = 5000.0    # MI355X dense INT8 MFMA
@@ -73,47 +75,104 @@ def exact_value(rec, m):
     return -v if int(rec["sign"]) else v
 
 
-def cpu_baseline(threads, min_seconds=10.0, max_reps=20):
-    """The reference's CPU mp_float_t algorithm (the oracle's policy-E restatement),
-    composed in the GPU's order, on the same workload as the GPU step (gemv 4096 x
-    4096 + gemm 512^3, same input distribution), repeated until about min_seconds
-    of CPU time have been measured."""
+def cpu_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cpus": usable}
+
+
+def _median_runs(fn, reps):
+    """one untimed run, then the median wall time of reps runs"""
+    fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts
+
+
+def cpu_baseline(threads, reps=5):
+    """The reference's CPU mp_float_t algorithm (the oracle's policy-E restatement), composed
+    in the GPU's order, on this host. All threads: the full step (gemv 4096 x 4096 + gemm
+    512^3, same input distribution), OpenMP over rows / entries. One thread: a row sample of
+    the same step (gemv rows 0..255, gemm rows 0..31 of the same matrices; each output's
+    arithmetic is that of the full problem). Each figure is the median of `reps` runs after
+    one untimed run."""
     sys.path[:0] = [os.path.join(ROOT, "oracle")]
     import pyoracle as po
 
-    po.set_threads(threads)
     o = po.Oracle(SET)
     E = po.POLICY_E
-    A = o.set_d(gen.uexp(301, M_GEMV * N_GEMV, ELO, EHI), E)
+    Ad = gen.uexp(301, M_GEMV * N_GEMV, ELO, EHI)
+    A = o.set_d(Ad, E)
     x = o.set_d(gen.uexp(302, N_GEMV, ELO, EHI), E)
-    y = o.set_d(gen.uexp(303, M_GEMV, ELO, EHI), E)
+    yd = gen.uexp(303, M_GEMV, ELO, EHI)
+    y = o.set_d(yd, E)
     al = o.set_d(gen.uexp(304, 1, ELO, EHI), E)
     be = o.set_d(gen.uexp(305, 1, ELO, EHI), E)
-    Ag = o.set_d(gen.uexp(311, M_GEMM * K_GEMM, ELO, EHI), E)
+    Agd = gen.uexp(311, M_GEMM * K_GEMM, ELO, EHI)
+    Ag = o.set_d(Agd, E)
     Bg = o.set_d(gen.uexp(312, K_GEMM * N_GEMM, ELO, EHI), E)
-    Cg = o.set_d(gen.uexp(313, M_GEMM * N_GEMM, ELO, EHI), E)
-    t_gemv = t_gemm = 0.0
-    reps = 0
-    while reps < max_reps and t_gemv + t_gemm < min_seconds:
-        t0 = time.perf_counter()
+    Cgd = gen.uexp(313, M_GEMM * N_GEMM, ELO, EHI)
+    Cg = o.set_d(Cgd, E)
+
+    def full():
         o.gemv(0, M_GEMV, N_GEMV, al, A, M_GEMV, x, be, y, B3, policy=E)
-        t1 = time.perf_counter()
         o.gemm(M_GEMM, N_GEMM, K_GEMM, al, Ag, M_GEMM, Bg, K_GEMM, be, Cg, M_GEMM, policy=E)
-        t2 = time.perf_counter()
-        t_gemv += t1 - t0
-        t_gemm += t2 - t1
-        reps += 1
-        print(f"[bench] CPU step {reps}: {t2 - t0:.2f} s", file=sys.stderr, flush=True)
+
+    rv, rg = 256, 32  # sampled rows of the gemv and gemm outputs
+    As = o.set_d(np.ascontiguousarray(Ad.reshape(N_GEMV, M_GEMV)[:, :rv]).ravel(), E)
+    ys = o.set_d(yd[:rv], E)
+    Ags = o.set_d(np.ascontiguousarray(Agd.reshape(K_GEMM, M_GEMM)[:, :rg]).ravel(), E)
+    Cgs = o.set_d(np.ascontiguousarray(Cgd.reshape(N_GEMM, M_GEMM)[:, :rg]).ravel(), E)
+
+    def sample():
+        o.gemv(0, rv, N_GEMV, al, As, rv, x, be, ys, B3, policy=E)
+        o.gemm(rg, N_GEMM, K_GEMM, al, Ags, rg, Bg, K_GEMM, be, Cgs, rg, policy=E)
+
     fmas = M_GEMV * N_GEMV + M_GEMM * N_GEMM * K_GEMM
-    return {
-        "value": reps * fmas / (t_gemv + t_gemm),
+    fmas_s = rv * N_GEMV + rg * N_GEMM * K_GEMM
+    po.set_threads(threads)
+    t_all, runs_all = _median_runs(full, reps)
+    print(f"[bench] CPU {threads} threads: {t_all:.2f} s per step", file=sys.stderr, flush=True)
+    po.set_threads(1)
+    t_one, runs_one = _median_runs(sample, reps)
+    print(f"[bench] CPU 1 thread: {t_one:.2f} s per sample", file=sys.stderr, flush=True)
+    po.set_threads(threads)
+    out = {
+        "value": fmas / t_all,
         "unit": "mp-FMA/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{reps} repetitions of the full step (gemv 4096x4096: {t_gemv / reps:.2f} s, gemm 512^3: "
-                  f"{t_gemm / reps:.2f} s each), CPU mp_float_t algorithm (emulated directed rounding), "
+        "sample": f"full step (gemv 4096x4096 + gemm 512^3, {fmas:.3g} mp-FMAs), median of {reps} runs after one "
+                  f"untimed run, CPU mp_float_t algorithm (emulated directed rounding) composed in the GPU's order, "
                   f"OpenMP over rows/entries with {threads} threads",
+        "runs_s": runs_all,
+        "single_thread": {"value": fmas_s / t_one, "unit": "mp-FMA/s", "cores": 1,
+                          "sample": f"gemv rows 0..{rv - 1} and gemm rows 0..{rg - 1} of the same step "
+                                    f"({fmas_s:.3g} mp-FMAs), median of {reps} runs after one untimed run",
+                          "runs_s": runs_one},
     }
+    out.update(cpu_info())
+    return out
+
+
+def cpu_rate(fn, count, reps=5):
+    """mp-FMA/s of the oracle's CPU algorithm for one call of fn (median of reps runs)"""
+    t, _ = _median_runs(fn, reps)
+    return count / t
 
 
 def main():
@@ -223,35 +282,35 @@ def main():
     fma_gemm = M_GEMM * N_GEMM * K_GEMM
     value = (fma_gemv + fma_gemm) / (ms_per_step / 1e3)
 
-    # ---- roofline of the dominant kernel ----
+    # ---- roofline of the dominant kernel (the gemm chains or the gemv residue kernel) ----
     gemm_avg = gemm_ms / max(gemm_n, 1)
     resid_avg = resid_ms / max(resid_n, 1)
     gemv_avg = gemv_ms / max(gemv_n, 1)
-    dom = max((gemm_avg, "chain"), (resid_avg, "resid"), (gemv_avg, "gemv"))[1]
-    if dom == "chain":
+    if gemm_avg >= gemv_avg:
         # one interval step per mp-FMA: per bound one product (mul, div by the
-        # eval unit) and one sum, i.e. 6 FP64 operations
+        # eval unit) and one sum, i.e. 6 directed-rounded FP64 operations
         ach = 6.0 * fma_gemm / (gemm_avg / 1e3) / 1e12
-        roof = {"kernel": "k_gemm_chain (scalar chains of the gemm entries)", "bound": "mfma", "achieved": ach,
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
-                "traffic": TRAFFIC_BYTES["chain"],
-                "avg_ms": gemm_avg, "work": "M*N*K interval steps, 6 FP64 operations each",
-                "binding_resource": "VALU issue: each directed-rounding operation is several instructions",
-                "valu_issue_frac": CHAIN_VALU_INSTS / (gemm_avg / 1e3) / VALU_ISSUE_PEAK}
-    elif dom == "resid":
-        # INT8 MFMA limb products: n moduli x NL^2 limb pairs x M*N*K multiply-adds (2 ops each)
-        nl = 4 if ctx_bits(ctx) <= 28 else 5
-        ach = 2.0 * fma_gemm * nmod * nl * nl / (resid_avg / 1e3) / 1e12
-        roof = {"kernel": "k_mf_gemm + k_gemm_digits (residue products on INT8 MFMA)", "bound": "mfma",
-                "achieved": ach, "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s", "frac": ach / INT8_PEAK_TOPS,
-                "traffic": TRAFFIC_BYTES["resid"], "avg_ms": resid_avg,
-                "work": "M*N*K*n*NL^2 int8 multiply-adds (2 ops each; unit is tera int8 operations/s)"}
+        roof = {"kernel": "k_gemm_chain_nn (scalar interval chains of the gemm entries)", "bound": "fp64-valu",
+                "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
+                "traffic": TRAFFIC_BYTES["chain"], "traffic_source": PROFILE,
+                "avg_ms": gemm_avg, "work": "M*N*K interval steps, 6 directed-rounded FP64 operations each",
+                "binding_resource": "VALU issue: an interval step is about 160 VALU instructions (alignment, "
+                                    "exponent and sign decisions, directed rounding) for its 6 FP64 operations",
+                "valu_issue_frac": CHAIN_VALU_INSTS / (gemm_avg / 1e3) / VALU_ISSUE_PEAK,
+                "valu_insts_source": PROFILE}
     else:
         bytes_ = M_GEMV * N_GEMV * (4 * nmod + 8)
         ach = bytes_ / (gemv_avg / 1e3) / 1e9
-        roof = {"kernel": "k_gemv_split (gemv partial sums: residues of A)", "bound": "hbm", "achieved": ach,
+        roof = {"kernel": "k_gemv_resid (gemv partial sums: residues of A)", "bound": "hbm", "achieved": ach,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES["gemv"],
-                "avg_ms": gemv_avg, "work": "m*n*(4n+8) bytes (residues, sign and exponent of A)"}
+                "traffic_source": PROFILE, "avg_ms": gemv_avg,
+                "work": "m*n*(4n+8) bytes (residues, sign and exponent of A)"}
+    # the second of the two, HBM-bound: the gemv residue kernel
+    gemv_bytes = M_GEMV * N_GEMV * (4 * nmod + 8)
+    roof_gemv = {"kernel": "k_gemv_resid (gemv partial sums: residues of A)", "bound": "hbm",
+                 "achieved": gemv_bytes / (gemv_avg / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": gemv_bytes / (gemv_avg / 1e3) / 1e9 / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES["gemv"],
+                 "traffic_source": PROFILE, "avg_ms": gemv_avg}
 
     # ---- accuracy: sampled outputs against the exact dyadic result ----
     m_ = [int(v) for v in ctx_moduli(ctx)]
@@ -262,7 +321,7 @@ def main():
         mfma = mfma_lines(mp)
     others = None
     if not args.no_other_ops:
-        others = plain_lines(mp)
+        others = plain_lines(mp, args.cpu_threads, cpu=not args.no_cpu)
     batched = None
     if not args.no_batch:
         batched = batch_lines(mp)
@@ -291,10 +350,13 @@ def main():
         "breakdown": {"gemv_ms": gemv_op_ms, "gemm_ms": gemm_op_ms,
                       "gemv_mpfma_per_s": fma_gemv / (gemv_op_ms / 1e3),
                       "gemm_mpfma_per_s": fma_gemm / (gemm_op_ms / 1e3),
-                      "gemv_fold_kernel_ms": gemv_avg, "gemm_chain_kernel_ms": gemm_avg,
-                      "gemm_resid_kernel_ms": resid_avg},
+                      "gemv_resid_kernel_ms": gemv_avg, "gemm_chain_kernel_ms": gemm_avg,
+                      "gemm_resid_path_ms": resid_avg,
+                      "gemm_resid_path": "exponent minima, int8 limb planes and INT8 MFMA residue products on the "
+                                         "library's side stream, overlapped with the chains (wall time)"},
         "accuracy": err,
         "roofline": roof,
+        "roofline_gemv": roof_gemv,
         "mfma_form": mfma,
         "other_ops": others,
         "batched_form": batched,
@@ -401,10 +463,15 @@ def rel_err(v, exact):
     return float(abs(v - exact) / abs(exact))
 
 
-def plain_lines(mp):
+def plain_lines(mp, threads, cpu=True):
     """The plain path's HBM-side operations: mp_dot and mp_axpy at 106 bits (8 moduli), the
     CSR SpMV at 16 moduli. Bytes are algorithmic (every operand read once, every result
-    written once); errors are relative errors against the exact result of the fp64 inputs."""
+    written once); errors are relative errors against the exact result of the fp64 inputs.
+    Beside the dot and SpMV lines: the oracle's CPU algorithm (policy E) on the same inputs,
+    with `threads` threads and with one."""
+    sys.path[:0] = [os.path.join(ROOT, "oracle")]
+    import pyoracle as po
+
     out = []
     ctx = mp.Context("8_2double")
     S = ctx.size
@@ -418,11 +485,23 @@ def plain_lines(mp):
                                                         None, 0, None)), reps)
         err = rel_err(exact_value(ctx.array_to_host(r, 1)[0], m_), exact_dot(hx, hy)) if n <= 65536 else None
         b = 2.0 * n * S
-        out.append({"op": "mp_dot", "moduli_set": "8_2double", "bits": ctx.precision, "n": n,
-                    "order": "gridDim3=256, blockDim3=64", "lanes_per_number": L.mpres_get_lanes(ctx.h, mp.K_DOT),
-                    "ms": ms, "mpfma_per_s": n / (ms / 1e3),
-                    "bytes": b, "gbs": b / (ms / 1e3) / 1e9, "hbm_frac": b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
-                    "max_rel_err": err})
+        line = {"op": "mp_dot", "moduli_set": "8_2double", "bits": ctx.precision, "n": n,
+                "order": "gridDim3=256, blockDim3=64", "lanes_per_number": L.mpres_get_lanes(ctx.h, mp.K_DOT),
+                "ms": ms, "mpfma_per_s": n / (ms / 1e3),
+                "bytes": b, "gbs": b / (ms / 1e3) / 1e9, "hbm_frac": b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                "max_rel_err": err}
+        if cpu and n <= 65536:
+            o = po.Oracle("8_2double")
+            ox, oy = o.set_d(hx, po.POLICY_E), o.set_d(hy, po.POLICY_E)
+            fn = lambda: o.dot(ox, oy, 256, 64, po.POLICY_E)  # noqa: E731
+            po.set_threads(threads)
+            rt = cpu_rate(fn, n)
+            po.set_threads(1)
+            r1 = cpu_rate(fn, n)
+            po.set_threads(threads)
+            line["cpu"] = {"mpfma_per_s": rt, "threads": threads, "single_thread_mpfma_per_s": r1, "kind": "port",
+                           "sample": "the same dot, CPU mp_float_t algorithm, median of 5 runs"}
+        out.append(line)
         del x, y, r
     n = 1 << 24
     print(f"[bench] axpy 106-bit n={n}", file=sys.stderr, flush=True)
@@ -430,11 +509,21 @@ def plain_lines(mp):
         gen.uexp(5, 1, -20, 20))
     ms = time_call(mp, lambda: mp.check(L.mpres_axpy(ctx.h, n, C.byref(al), C.byref(x), 1, C.byref(y), 1, None)), 5)
     b = 3.0 * n * S
+    # accuracy: one call on a fresh y, sampled elements against the exact alpha*x_i + y_i
+    hx, hy, ha = gen.uexp(3, n, -20, 20), gen.uexp(4, n, -20, 20), gen.uexp(5, 1, -20, 20)[0]
+    y1 = ctx.array_set_d(hy)
+    mp.check(L.mpres_axpy(ctx.h, n, C.byref(al), C.byref(x), 1, C.byref(y1), 1, None))
+    mp.synchronize()
+    aerr = 0.0
+    for i in (0, 1, 4097, n // 2, n - 1):
+        got = exact_value(fetch_range(ctx, mp, y1, i, 1)[0], m_)
+        aerr = max(aerr, rel_err(got, Fraction(float(ha)) * Fraction(float(hx[i])) + Fraction(float(hy[i]))))
     out.append({"op": "mp_axpy", "moduli_set": "8_2double", "bits": ctx.precision, "n": n, "ms": ms,
                 "lanes_per_number": L.mpres_get_lanes(ctx.h, mp.K_AXPY),
                 "mpfma_per_s": n / (ms / 1e3), "bytes": b, "gbs": b / (ms / 1e3) / 1e9,
-                "hbm_frac": b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS})
-    del x, y, al
+                "hbm_frac": b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, "max_rel_err_sampled": aerr,
+                "error_samples": "5 elements of y = alpha*x + y after one call, against the exact value"})
+    del x, y, al, y1
     ctx.close()
 
     m = 1000000
@@ -465,6 +554,19 @@ def plain_lines(mp):
                     "lanes_per_number": L.mpres_get_lanes(ctx.h, mp.K_SPMV),
                     "mpfma_per_s": nnz / (ms / 1e3), "bytes": b, "gbs": b / (ms / 1e3) / 1e9,
                     "hbm_frac": b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, "max_rel_err_sampled_rows": err})
+    if cpu:
+        o = po.Oracle("16_4double")
+        ox = o.set_d(hx, po.POLICY_E)
+        po.set_threads(threads)
+        rt = cpu_rate(lambda: o.spmv(m, irp, ja, as_, ox, 1, po.POLICY_E), nnz)
+        rows1 = np.arange(0, 100000, dtype=np.int32)
+        nnz1 = int(irp[100000] - irp[0])
+        po.set_threads(1)
+        r1 = cpu_rate(lambda: o.spmv(m, irp, ja, as_, ox, 1, po.POLICY_E, rows=rows1), nnz1)
+        po.set_threads(threads)
+        out[-4]["cpu"] = {"mpfma_per_s": rt, "threads": threads, "single_thread_mpfma_per_s": r1, "kind": "port",
+                          "sample": f"CSR SpMV, CPU mp_float_t algorithm: all {m} rows with {threads} threads, "
+                                    f"rows 0..99999 ({nnz1} nonzeros) with one; medians of 5 runs"}
     ctx.close()
     return out
 

@@ -61,6 +61,22 @@ typedef struct {                                                      /* src/typ
 } mp_collection_t;
 typedef struct { double *as; int *ja; int *irp; } csr_t;              /* src/types.cuh:109-113 */
 
+/* The layouts above are the reference's (LP64: long and pointers are 8 bytes). */
+#ifdef __cplusplus
+#define MPRES_STATIC_ASSERT(c, msg) static_assert(c, msg)
+#else
+#define MPRES_STATIC_ASSERT(c, msg) _Static_assert(c, msg)
+#endif
+MPRES_STATIC_ASSERT(sizeof(long) == 8 && sizeof(void *) == 8, "LP64 data model required");
+MPRES_STATIC_ASSERT(sizeof(er_float_t) == 16 && offsetof(er_float_t, exp) == 8, "er_float_t layout");
+MPRES_STATIC_ASSERT(sizeof(mpres_int4_t) == 16, "int4 layout");
+MPRES_STATIC_ASSERT(sizeof(mp_array_t) == 48 && offsetof(mp_array_t, sign) == 8 && offsetof(mp_array_t, exp) == 16 &&
+                        offsetof(mp_array_t, eval) == 24 && offsetof(mp_array_t, buf) == 32 &&
+                        offsetof(mp_array_t, len) == 40,
+                    "mp_array_t layout");
+MPRES_STATIC_ASSERT(sizeof(mp_collection_t) == 32 && offsetof(mp_collection_t, eval) == 24, "mp_collection_t layout");
+MPRES_STATIC_ASSERT(sizeof(csr_t) == 24 && offsetof(csr_t, ja) == 8 && offsetof(csr_t, irp) == 16, "csr_t layout");
+
 /* mblas_trans_type, src/blas/mblas_enum.cuh:25-31 */
 #define MPRES_NO_TRANS 111
 #define MPRES_TRANS 112
