@@ -144,6 +144,22 @@ int mpres_cmp(mpres_ctx_t *ctx, int mode, int count, const void *x, const void *
  * src/rns.cuh:1242-1260. x, y: device arrays of count x n residues (0 <= x_i < M). */
 int mpres_rns_cmp(mpres_ctx_t *ctx, int count, const int *x, const int *y, int *r, void *stream);
 
+/* ---- export (replaces the host-side mp_get_d / mp_get_mpfr, src/arith/assign.cuh:154-211) ----
+ * out[i] = x[i] as a double: (-1)^sign X 2^exp with X the RNS integer of the residues (CRT
+ * mod M), rounded to 53 bits to nearest, ties to even, then to the double format (subnormal
+ * results round once more, overflow gives an infinity) - the reference's MPFR conversion at
+ * 53-bit precision. x: count mp_float_t (mpres_get_d) or an mp_array_t (mpres_array_get_d);
+ * out: device array of count doubles. */
+int mpres_get_d(mpres_ctx_t *ctx, int count, const void *x, double *out, void *stream);
+int mpres_array_get_d(mpres_ctx_t *ctx, int count, const mp_array_t *x, double *out, void *stream);
+/* The exact value of each number for MPFR or decimal output: limbs[i * L + l] = 32-bit limb l
+ * (least significant first) of X, with L = mpres_export_limbs(ctx); sign[i], exp[i] as stored.
+ * x[i] = (-1)^sign[i] X 2^exp[i]. With GMP/MPFR: mpz_import(z, L, -1, 4, 0, 0, limbs + i * L),
+ * mpfr_set_z_2exp(r, z, exp[i], MPFR_RNDN), then negate if sign[i]. All arrays on the device. */
+int mpres_export_limbs(const mpres_ctx_t *ctx);
+int mpres_get_integer(mpres_ctx_t *ctx, int count, const void *x, unsigned *limbs, int *sign, int *exp,
+                      void *stream);
+
 /* ---- BLAS, v1 interface over mp_array_t ---- */
 /* r = x . y ; cuda::mp_dot<gridDim1,blockDim1,gridDim2,gridDim3,blockDim3>, src/blas/dot.cuh:84-107.
  * The sum order is fixed by (gridDim3, blockDim3); the other template arguments do not change the result. */

@@ -541,6 +541,39 @@ int mpres_rns_cmp(mpres_ctx_t *c, int count, const int *x, const int *y, int *r,
     return MPRES_OK;
 }
 
+// ---------------------------------------------------------------- export
+int mpres_export_limbs(const mpres_ctx_t *c) {
+    if (!c) return 0;
+    return (31 * c->host.n + 31) / 32 + 1;  // export_limbs<N>() of the kernels
+}
+
+int mpres_get_d(mpres_ctx_t *c, int count, const void *x, double *out, void *st) {
+    if (!c) return fail(MPRES_ERR_ARG, "null context");
+    if (count <= 0) return MPRES_OK;
+    if (!x || !out) return fail(MPRES_ERR_ARG, "null argument");
+    HIPCHK(c->ops[OP_ELEM]->exp_out(c->dev, 0, count, (const char *)x, SoA{}, false, out, nullptr, nullptr, nullptr,
+                                    strm(st)));
+    return MPRES_OK;
+}
+
+int mpres_array_get_d(mpres_ctx_t *c, int count, const mp_array_t *x, double *out, void *st) {
+    if (!c) return fail(MPRES_ERR_ARG, "null context");
+    if (count <= 0) return MPRES_OK;
+    if (!x || !out) return fail(MPRES_ERR_ARG, "null argument");
+    HIPCHK(c->ops[OP_ELEM]->exp_out(c->dev, 0, count, nullptr, soa_of(x), true, out, nullptr, nullptr, nullptr,
+                                    strm(st)));
+    return MPRES_OK;
+}
+
+int mpres_get_integer(mpres_ctx_t *c, int count, const void *x, unsigned *limbs, int *sign, int *exp, void *st) {
+    if (!c) return fail(MPRES_ERR_ARG, "null context");
+    if (count <= 0) return MPRES_OK;
+    if (!x || !limbs || !sign || !exp) return fail(MPRES_ERR_ARG, "null argument");
+    HIPCHK(c->ops[OP_ELEM]->exp_out(c->dev, 1, count, (const char *)x, SoA{}, false, nullptr, limbs, sign, exp,
+                                    strm(st)));
+    return MPRES_OK;
+}
+
 // ---------------------------------------------------------------- dot
 size_t mpres_dot_workspace(const mpres_ctx_t *c, int n, int g3, int b3) {
     (void)n;

@@ -120,6 +120,131 @@ __global__ __launch_bounds__(BLK) void k_set_d(const DevConsts *__restrict__ C, 
         store_aos(g, a, ao, i);
 }
 
+// sign and exponent of element i in either layout
+template <int N>
+__device__ __forceinline__ int sign_at(const SoA &v, long long i) { return v.s[i]; }
+template <int N>
+__device__ __forceinline__ int sign_at(const AoSRef<N> &v, long long i) { return *v.sign(i); }
+template <int N>
+__device__ __forceinline__ int exp_of(const SoA &v, long long i) { return v.e[i]; }
+template <int N>
+__device__ __forceinline__ int exp_of(const AoSRef<N> &v, long long i) { return *v.exp(i); }
+
+// ---------------------------------------------------------------- export
+// The integer X (0 <= X < M) that a number's residues stand for, as the reference's
+// mp_get_d / mp_get_mpfr form it (src/arith/assign.cuh:154-211, CRT with the orthogonal
+// base, mod M): here by mixed-radix conversion, X = a_0 + a_1 m_0 + a_2 m_0 m_1 + ...,
+// then Horner's rule over 32-bit limbs, least significant first. Residues outside
+// [0, m_i) are reduced first. One thread per number, loops unrolled (arrays in registers).
+template <int N>
+constexpr int export_limbs() {
+    return (31 * N + 31) / 32 + 1;  // moduli below 2^31: X < 2^(31 N)
+}
+template <int N>
+__device__ __forceinline__ void mp_integer(const DevConsts *__restrict__ C, const int *d, uint32_t *X) {
+    constexpr int LW = export_limbs<N>();
+    int a[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const int m = C->m[i];
+        int v = d[i] % m;
+        a[i] = v < 0 ? v + m : v;
+    }
+#pragma unroll
+    for (int j = 0; j < N - 1; j++)
+#pragma unroll
+        for (int i = j + 1; i < N; i++) {
+            int t = a[i] - a[j];
+            t = t < 0 ? t + C->m[i] : t;
+            a[i] = mulmod(t, C->mrcinv[j][i], C->md[i], C->minv[i]);
+        }
+#pragma unroll
+    for (int l = 0; l < LW; l++) X[l] = 0;
+    X[0] = (uint32_t)a[N - 1];
+#pragma unroll
+    for (int i = N - 2; i >= 0; i--) {
+        uint64_t carry = (uint64_t)a[i];
+        const uint64_t m = (uint64_t)C->m[i];
+#pragma unroll
+        for (int l = 0; l < LW; l++) {
+            const uint64_t t = (uint64_t)X[l] * m + carry;
+            X[l] = (uint32_t)t;
+            carry = t >> 32;
+        }
+    }
+}
+// (-1)^sign X 2^exp as a double: X rounded to 53 significant bits to nearest (ties to
+// even), then scaled by a power of two with one more rounding to nearest when the result
+// is subnormal (overflow gives an infinity) - the reference's mpfr_set_str at MPFR's
+// default 53-bit precision followed by mpfr_get_d (src/arith/assign.cuh:167-175).
+template <int N>
+__device__ __forceinline__ double integer_to_double(const uint32_t *X, int sign, int exp) {
+    constexpr int LW = export_limbs<N>();
+    int top = -1;
+#pragma unroll
+    for (int l = 0; l < LW; l++) top = X[l] ? l : top;
+    if (top < 0) return sign ? -0.0 : 0.0;
+    uint32_t t0 = 0, t1 = 0, t2 = 0;  // limbs top, top - 1, top - 2
+    bool below = false;               // any nonzero limb under top - 2
+#pragma unroll
+    for (int l = 0; l < LW; l++) {
+        t0 = (l == top) ? X[l] : t0;
+        t1 = (l == top - 1) ? X[l] : t1;
+        t2 = (l == top - 2) ? X[l] : t2;
+        below = below || (l < top - 2 && X[l] != 0);
+    }
+    const int lz = __clz(t0);
+    uint64_t w = ((uint64_t)t0 << 32) | t1;  // the 64 bits from X's leading one down
+    bool sticky = below;
+    if (lz) {
+        w = (w << lz) | (t2 >> (32 - lz));
+        sticky = sticky || (t2 << lz) != 0;
+    } else {
+        sticky = sticky || t2 != 0;
+    }
+    uint64_t sig = w >> 11;
+    const uint32_t rem = (uint32_t)(w & 0x7ff);
+    sticky = sticky || (rem & 0x3ff) != 0;
+    if ((rem & 0x400) && (sticky || (sig & 1))) sig++;  // may reach 2^53, still exact
+    const int e2 = 32 * top + (32 - lz) - 53 + exp;
+    const double r = ldexp((double)sig, e2);
+    return sign ? -r : r;
+}
+template <int N>
+__device__ __forceinline__ void gather_digits(const SoA &v, long long i, int *d) {
+#pragma unroll
+    for (int r = 0; r < N; r++) d[r] = v.d[i * N + r];
+}
+template <int N>
+__device__ __forceinline__ void gather_digits(const AoSRef<N> &v, long long i, int *d) {
+#pragma unroll
+    for (int r = 0; r < N; r++) d[r] = v.digits(i)[r];
+}
+// mp_get_d over count numbers (MODE 0), or the exact integers (MODE 1): export_limbs<N>()
+// limbs per number, with the sign and the exponent
+template <int N, int L, class VM, int MODE>
+__global__ __launch_bounds__(BLK) void k_export(const DevConsts *__restrict__ C, int count, VM x,
+                                               double *__restrict__ out, uint32_t *__restrict__ limbs,
+                                               int *__restrict__ sg, int *__restrict__ ex) {
+    if constexpr (std::is_same<VM, SoA>::value) x = rd(x);
+    const long long i = (long long)blockIdx.x * BLK + threadIdx.x;
+    if (i >= count) return;
+    constexpr int LW = export_limbs<N>();
+    int d[N];
+    gather_digits<N>(x, i, d);
+    uint32_t X[LW];
+    mp_integer<N>(C, d, X);
+    const int s = sign_at<N>(x, i), e = exp_of<N>(x, i);
+    if constexpr (MODE == 0) {
+        out[i] = integer_to_double<N>(X, s, e);
+    } else {
+#pragma unroll
+        for (int l = 0; l < LW; l++) limbs[i * LW + l] = X[l];
+        sg[i] = s;
+        ex[i] = e;
+    }
+}
+
 // ---------------------------------------------------------------- element-wise
 template <int N, int L, int OP>
 __global__ __launch_bounds__(BLK) void k_elem(const DevConsts *__restrict__ C, int count, AoSRef<N> x, AoSRef<N> y,
@@ -526,14 +651,6 @@ __global__ __launch_bounds__(BLK) void k_gemm_fin(const DevConsts *__restrict__ 
 // (k_gemm_limbs, k_mf_gemm), and k_gemm_digits applies the rescaling. Chains that
 // meet a rare case are marked REDO and recomputed step by step by the fixup kernel.
 
-template <int N>
-__device__ __forceinline__ int sign_at(const SoA &v, long long i) { return v.s[i]; }
-template <int N>
-__device__ __forceinline__ int sign_at(const AoSRef<N> &v, long long i) { return *v.sign(i); }
-template <int N>
-__device__ __forceinline__ int exp_of(const SoA &v, long long i) { return v.e[i]; }
-template <int N>
-__device__ __forceinline__ int exp_of(const AoSRef<N> &v, long long i) { return *v.exp(i); }
 
 // index of op(A)[row, i] (SIDE 0) or op(B)[i, col] (SIDE 1) in column-major storage
 template <int SIDE>

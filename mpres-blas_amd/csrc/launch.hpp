@@ -48,6 +48,25 @@ struct Launch {
         }
         return hipGetLastError();
     }
+    static hipError_t exp_out(const DevConsts *c, int mode, int count, const char *x, SoA so, bool soa, double *out,
+                              unsigned *limbs, int *sg, int *ex, hipStream_t st) {
+        if (count <= 0) return hipSuccess;
+        const dim3 gr(grid_for(count, 1)), bl(BLK);
+        uint32_t *lb = (uint32_t *)limbs;
+        if (soa) {
+            if (mode == 0)
+                hipLaunchKernelGGL((k_export<N, L, SoA, 0>), gr, bl, 0, st, c, count, so, out, lb, sg, ex);
+            else
+                hipLaunchKernelGGL((k_export<N, L, SoA, 1>), gr, bl, 0, st, c, count, so, out, lb, sg, ex);
+        } else {
+            const AoSRef<N> X{(char *)x};
+            if (mode == 0)
+                hipLaunchKernelGGL((k_export<N, L, AoSRef<N>, 0>), gr, bl, 0, st, c, count, X, out, lb, sg, ex);
+            else
+                hipLaunchKernelGGL((k_export<N, L, AoSRef<N>, 1>), gr, bl, 0, st, c, count, X, out, lb, sg, ex);
+        }
+        return hipGetLastError();
+    }
     static hipError_t cmp(const DevConsts *c, int mode, int count, const char *x, const char *y, const int *dx,
                           const int *dy, int *out, hipStream_t st) {
         if (count <= 0) return hipSuccess;
@@ -394,6 +413,7 @@ struct Launch {
         o.set_d = &set_d;
         o.elem = &elem;
         o.cmp = &cmp;
+        o.exp_out = &exp_out;
         o.fold = &fold;
         o.tree = &tree;
         o.aos_to_soa = &aos_to_soa;

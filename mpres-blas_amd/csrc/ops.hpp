@@ -40,6 +40,9 @@ struct Ops {
     int N, L;
     hipError_t (*set_d)(const DevConsts *, const double *, int, SoA, char *, bool, hipStream_t);
     hipError_t (*elem)(const DevConsts *, int, int, const char *, const char *, const double *, char *, hipStream_t);
+    // export: mp_get_d (mode 0) or the exact integers (mode 1) of count numbers, AoS (x) or SoA (so)
+    hipError_t (*exp_out)(const DevConsts *, int, int, const char *, SoA, bool, double *, unsigned *, int *, int *,
+                          hipStream_t);
     // comparisons (mode 0 mp_cmp, 1 mp_cmp_abs on AoS numbers, 2 rns_cmp on bare residues)
     hipError_t (*cmp)(const DevConsts *, int, int, const char *, const char *, const int *, const int *, int *,
                       hipStream_t);

@@ -97,6 +97,10 @@ def lib():
         L.mpres_array_to_aos.argtypes = [vp, vp, vp, i32, vp]
         L.mpres_elementwise.argtypes = [vp, i32, i32, vp, vp, vp, vp]
         L.mpres_cmp.argtypes = [vp, i32, i32, vp, vp, vp, vp]
+        L.mpres_get_d.argtypes = [vp, i32, vp, vp, vp]
+        L.mpres_array_get_d.argtypes = [vp, i32, vp, vp, vp]
+        L.mpres_export_limbs.argtypes = [vp]
+        L.mpres_get_integer.argtypes = [vp, i32, vp, vp, vp, vp, vp]
         L.mpres_rns_cmp.argtypes = [vp, i32, vp, vp, vp, vp]
         L.mpres_dot_workspace.restype = sz
         L.mpres_dot_workspace.argtypes = [vp, i32, i32, i32]

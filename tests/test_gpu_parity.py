@@ -4,6 +4,7 @@ field (residues, sign, exponent, both interval bounds), on seeded inputs at
 sizes the oracle finishes in seconds, for every compiled lanes-per-number
 variant."""
 import ctypes as C
+import math
 import importlib.util
 import os
 import sys
@@ -203,6 +204,90 @@ def test_dot(name, n, g3, b3):
     got = exact.mp_value(want[0], m)
     if ex != 0:
         assert abs(got - ex) / abs(ex) < 2.0 ** (-c.precision + 12)
+
+
+def _get_d_exact(rec, m):
+    """(-1)^sign X 2^exp as the reference's MPFR conversion gives it: X rounded to 53 bits to
+    nearest (ties to even), then to a double (C ldexp: one more rounding when subnormal)"""
+    X = exact.crt(rec["digits"], m)
+    s, e = int(rec["sign"]), int(rec["exp"])
+    if X == 0:
+        return -0.0 if s else 0.0
+    b = X.bit_length()
+    if b > 53:
+        q, r = divmod(X, 1 << (b - 53))
+        h = 1 << (b - 54)
+        if r > h or (r == h and q & 1):
+            q += 1
+        X, e = q, e + b - 53
+    try:
+        v = math.ldexp(float(X), e)
+    except OverflowError:
+        v = math.inf
+    return -v if s else v
+
+
+@pytest.mark.parametrize("name", SETS)
+def test_get_d_and_integer(name):
+    """mpres_get_d / mpres_array_get_d / mpres_get_integer (the reference's mp_get_d and
+    mp_get_mpfr on the device): round trips of doubles, random residues scaled into the
+    normal, subnormal and overflow ranges, and products of full-length numbers"""
+    c, o = ctx(name), orc(name)
+    m = [int(v) for v in o.moduli()]
+    M = exact.moduli_product(m)
+    rng = np.random.default_rng(81)
+    # 1. round trips: every double survives set_d -> get_d (zeros come back as +0, as set_d stores them)
+    xd = np.concatenate([inputs(o, 82, 2000, -1000, 1000), [2.0**-1074, -2.0**-1060, 2.0**1023 * 1.5, -0.0,
+                                                             np.nextafter(2.0**-1022, 0), 1.0 - 2.0**-53]])
+    recs = o.set_d(xd, D)
+    # 2. random X in [0, M) with exponents that put |X 2^e| in the normal, subnormal and overflow ranges
+    cnt = 1500
+    rnd = np.zeros(cnt, o.dtype)
+    for i in range(cnt):
+        X = int.from_bytes(rng.bytes((M.bit_length() + 7) // 8), "little") % M
+        if i % 7 == 0:
+            X >>= int(rng.integers(0, M.bit_length()))  # short integers too
+        b = max(X.bit_length(), 1)
+        kind = i % 3
+        e = int(rng.integers(-900, 900)) - b if kind == 0 else (-1074 - b + int(rng.integers(-5, 60)) if kind == 1
+                                                                 else 1024 - b + int(rng.integers(-2, 3)))
+        rnd[i]["digits"] = [X % mi for mi in m]
+        rnd[i]["sign"] = int(rng.integers(0, 2))
+        rnd[i]["exp"] = e
+    # ties: X = (2^53 + 1) 2^k + 2^(k-1) patterns and exact halves
+    for j, X in enumerate([(2**53 + 1) * 2**10 + 2**9, 2**53 * 2**10 + 2**9, (2**53 + 3) * 2**4 + 2**3 + 1]):
+        if X < M:
+            rnd[j]["digits"] = [X % mi for mi in m]
+            rnd[j]["exp"] = -20
+    # 3. products of full-length numbers (op 2: mp_mul)
+    a, b = o.set_d(inputs(o, 83, 500), D), o.set_d(inputs(o, 84, 500), D)
+    prods = o.mul(a, b, D)
+    allr = np.concatenate([recs, rnd, prods])
+    n = len(allr)
+    want = np.array([_get_d_exact(r, m) for r in allr])
+    dx = c.aos(allr)
+    out = mp.DevBuf(nbytes=8 * n)
+    note("get_d", name, n)
+    mp.check(c.L.mpres_get_d(c.h, n, dx.ptr, out.ptr, None))
+    got = out.to_numpy(np.float64, n)
+    assert got.tobytes() == want.tobytes(), f"{int((got.view(np.int64) != want.view(np.int64)).sum())} differ"
+    nz = np.abs(xd) >= 2.0**-1022  # normal doubles round-trip exactly (set_d stores no subnormals)
+    assert np.array_equal(got[:len(xd)][nz], xd[nz])
+    # the same through the mp_array_t layout
+    arr = c.array_from_host(allr)
+    out2 = mp.DevBuf(nbytes=8 * n)
+    mp.check(c.L.mpres_array_get_d(c.h, n, C.byref(arr), out2.ptr, None))
+    assert out2.to_numpy(np.float64, n).tobytes() == want.tobytes()
+    # exact integers
+    lw = c.L.mpres_export_limbs(c.h)
+    lb, sg, ex = mp.DevBuf(nbytes=4 * lw * n), mp.DevBuf(nbytes=4 * n), mp.DevBuf(nbytes=4 * n)
+    mp.check(c.L.mpres_get_integer(c.h, n, dx.ptr, lb.ptr, sg.ptr, ex.ptr, None))
+    limbs = lb.to_numpy(np.uint32, lw * n).reshape(n, lw)
+    assert np.array_equal(sg.to_numpy(np.int32, n), allr["sign"])
+    assert np.array_equal(ex.to_numpy(np.int32, n), allr["exp"])
+    for i in range(n):
+        X = int.from_bytes(limbs[i].tobytes(), "little")
+        assert X == exact.crt(allr[i]["digits"], m), f"record {i}"
 
 
 def test_pool_two_streams():
