@@ -144,6 +144,19 @@ int mpres_cmp(mpres_ctx_t *ctx, int mode, int count, const void *x, const void *
  * src/rns.cuh:1242-1260. x, y: device arrays of count x n residues (0 <= x_i < M). */
 int mpres_rns_cmp(mpres_ctx_t *ctx, int count, const int *x, const int *y, int *r, void *stream);
 
+/* ---- the operations the reference runs on the host with MPFR, on the GPU ----
+ * r[i] = x[i] / y[i]: cuda::mp_div (src/arith/div.cuh:56-65, a host MPFR round trip there):
+ * both operands rounded to MP_PRECISION bits, the quotient rounded to nearest at
+ * MP_PRECISION bits, stored as mp_set_mpfr stores it (significand without trailing zeros)
+ * with the CPU interval evaluation of its residues. A zero divisor gives zero (undefined in
+ * the reference). x, y, r: device arrays of count mp_float_t. */
+int mpres_div(mpres_ctx_t *ctx, int count, const void *x, const void *y, void *r, void *stream);
+/* cuda::mp_sqrt / cuda::mp_dsqrt (src/arith/sqrt.cuh:65-89): the square root rounded to
+ * nearest at MP_PRECISION bits, as an mp_float_t or rounded to a double (out: device array).
+ * A negative operand gives NaN (mp_dsqrt) or zero (mp_sqrt). */
+int mpres_sqrt(mpres_ctx_t *ctx, int count, const void *x, void *r, void *stream);
+int mpres_dsqrt(mpres_ctx_t *ctx, int count, const void *x, double *out, void *stream);
+
 /* ---- export (replaces the host-side mp_get_d / mp_get_mpfr, src/arith/assign.cuh:154-211) ----
  * out[i] = x[i] as a double: (-1)^sign X 2^exp with X the RNS integer of the residues (CRT
  * mod M), rounded to 53 bits to nearest, ties to even, then to the double format (subnormal
@@ -215,6 +228,34 @@ int mpres_dot_v2(mpres_ctx_t *ctx, int n, const void *x, const void *y, void *r,
                  size_t ws_bytes, void *stream);
 /* cuda::mp_axpy(n,alpha,x,y,r), src/blas/v2/axpy_v2.cuh:39-52 */
 int mpres_axpy_v2(mpres_ctx_t *ctx, int n, const void *alpha, const void *x, const void *y, void *r, void *stream);
+/* r = (-alpha) x + y: cuda::mp_maxpy, src/blas/v2/maxpy_v2.cuh (order-free) */
+int mpres_maxpy_v2(mpres_ctx_t *ctx, int n, const void *alpha, const void *x, const void *y, void *r, void *stream);
+/* r = x - y: cuda::mp_diff, src/blas/v2/diff_v2.cuh */
+int mpres_diff_v2(mpres_ctx_t *ctx, int n, const void *x, const void *y, void *r, void *stream);
+/* r[i] = x[i] * d[i] (d: device doubles): cuda::mp_prod_d, src/blas/v2/prod_v2.cuh */
+int mpres_prod_d_v2(mpres_ctx_t *ctx, int n, const void *x, const double *d, void *r, void *stream);
+/* *result (host) = ||x||_2: cuda::mp_norm2<blocks, threads>, src/blas/v2/norm2_v2.cuh:107-119
+ * (sum of squares in the order of its two reduction kernels, then mp_dsqrt). Synchronizes
+ * the stream. */
+size_t mpres_norm2_v2_workspace(const mpres_ctx_t *ctx, int n, int blocks, int threads);
+int mpres_norm2_v2(mpres_ctx_t *ctx, int n, const void *x, int blocks, int threads, double *result, void *ws,
+                   size_t ws_bytes, void *stream);
+
+/* ---- solvers ----
+ * A x = b by conjugate gradients: mp_cg_csr<threads, blocks_reduce> (src/sparse/solver/
+ * cg_csr.cuh:52-117), or with precond != NULL (device array of n doubles, the reciprocals of
+ * A's diagonal) mp_pcg_csr with the diagonal preconditioner (pcg_csr.cuh:52-130). Each step
+ * is the library's operation in the reference's order, so every iterate equals the
+ * reference's. x: initial guess on entry (the reference assumes zero), solution on exit.
+ * Stops when ||r_k|| <= tol ||r_0|| or after maxit iterations. resvec: host array of at
+ * least maxit + 1 doubles, receives ||r_k|| / ||r_0|| for k = 0 .. *iters. The stopping test
+ * reads the residual norm on the host, so the stream is synchronized once per iteration.
+ * ws: mpres_cg_workspace() bytes, or NULL for the context's pool. */
+size_t mpres_cg_workspace(const mpres_ctx_t *ctx, int n, int threads, int blocks_reduce);
+int mpres_cg_csr(mpres_ctx_t *ctx, int n, const csr_t *A, const void *b, double tol, int maxit, void *x,
+                 const double *precond, double *resvec, int *iters, int threads, int blocks_reduce, void *ws,
+                 size_t ws_bytes, void *stream);
+
 /* cuda::mp_gemm(transa,transb,...), src/blas/v2/gemm_v2.cuh:55-81.
  * Workspace: the m*n sums, as for mpres_gemm (mpres_gemm_workspace). */
 int mpres_gemm_v2(mpres_ctx_t *ctx, int transa, int transb, int m, int n, int k, const void *alpha, const void *A,

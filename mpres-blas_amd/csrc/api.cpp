@@ -639,6 +639,99 @@ int mpres_dot_v2(mpres_ctx_t *c, int n, const void *x, const void *y, void *r, i
     return MPRES_OK;
 }
 
+// ---------------------------------------------------------------- norm2 (v2)
+size_t mpres_norm2_v2_workspace(const mpres_ctx_t *c, int n, int blocks, int threads) {
+    if (!c) return 0;
+    return mpres_dot_workspace(c, n, blocks, threads) + 16;
+}
+
+// sum of squares in the order of mp_sumsq_kernel1/2 (the v2 dot of x with itself: the squares
+// are nonnegative, and mp_add_abs of nonnegative operands is mp_add), then mp_dsqrt; the
+// double lands in *dres (device) and, after a stream synchronisation, in *result
+static int norm2_into(mpres_ctx_t *c, int n, const void *x, int blocks, int threads, char *w, size_t wb, double *dres,
+                      double *result, hipStream_t st) {
+    char *res;
+    WsUse inner(c, st);
+    SoA none{};
+    int rc = dot_common(c, inner, 1, n, none, 1, none, 1, (const char *)x, (const char *)x, blocks, threads, w, wb, st,
+                        &res);
+    if (rc) return rc;
+    HIPCHK(c->ops[OP_ELEM]->hostop(c->dev, 2, 1, res, nullptr, nullptr, dres, st));
+    HIPCHK(hipMemcpyAsync(result, dres, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return MPRES_OK;
+}
+
+int mpres_norm2_v2(mpres_ctx_t *c, int n, const void *x, int blocks, int threads, double *result, void *ws, size_t wsb,
+                   void *st) {
+    if (!c || !x || !result) return fail(MPRES_ERR_ARG, "null argument");
+    if (n <= 0 || blocks <= 0 || threads <= 0) return fail(MPRES_ERR_ARG, "invalid size or launch shape");
+    const size_t dw = mpres_dot_workspace(c, n, blocks, threads);
+    char *w;
+    WsUse wu(c, strm(st));
+    int rc = wu.get(ws, wsb, dw + 16, &w);
+    if (rc) return rc;
+    return norm2_into(c, n, x, blocks, threads, w, dw, (double *)(w + dw), result, strm(st));
+}
+
+// ---------------------------------------------------------------- CG / PCG (CSR)
+size_t mpres_cg_workspace(const mpres_ctx_t *c, int n, int threads, int blocks_reduce) {
+    if (!c || n <= 0 || threads <= 0 || blocks_reduce <= 0) return 0;
+    return mpsize(c) * (4 * (size_t)n + 5) + mpres_dot_workspace(c, n, blocks_reduce, threads) + 16;
+}
+
+int mpres_cg_csr(mpres_ctx_t *c, int n, const csr_t *A, const void *b, double tol, int maxit, void *x,
+                 const double *precond, double *resvec, int *iters, int threads, int blocks_reduce, void *ws,
+                 size_t wsb, void *st) {
+    if (!c || !A || !b || !x || !resvec || !iters) return fail(MPRES_ERR_ARG, "null argument");
+    if (n <= 0 || maxit < 0 || threads <= 0 || blocks_reduce <= 0) return fail(MPRES_ERR_ARG, "invalid arguments");
+    const hipStream_t s = strm(st);
+    const size_t S = mpsize(c), dw = mpres_dot_workspace(c, n, blocks_reduce, threads);
+    char *w;
+    WsUse wu(c, s);
+    int rc = wu.get(ws, wsb, mpres_cg_workspace(c, n, threads, blocks_reduce), &w);
+    if (rc) return rc;
+    char *r = w, *p = r + S * n, *q = p + S * n, *z = q + S * n, *rho = z + S * n, *rhop = rho + S, *alpha = rhop + S,
+         *beta = alpha + S, *pq = beta + S, *dws = pq + S;
+    double *dres = (double *)(dws + dw);
+    char *zz = precond ? z : r;  // z = M^-1 r (diagonal preconditioner) or r itself
+#define CGCHK(call)                   \
+    do {                              \
+        if (int e_ = (call)) return e_; \
+    } while (0)
+    // r = b - A x (src/sparse/solver/cg_csr.cuh:78-80)
+    CGCHK(mpres_spmv_csr(c, n, A, x, r, st));
+    CGCHK(mpres_diff_v2(c, n, b, r, r, st));
+    double norm0 = 0, normk = 0;
+    CGCHK(norm2_into(c, n, r, blocks_reduce, threads, dws, dw, dres, &norm0, s));
+    const double eps = norm0 * tol;
+    normk = norm0;
+    int k = 0;
+    resvec[0] = normk / norm0;
+    while (normk > eps && k < maxit) {  // :88-114 (pcg_csr.cuh:88-127)
+        if (precond) CGCHK(mpres_prod_d_v2(c, n, r, precond, z, st));
+        HIPCHK(hipMemcpyAsync(rhop, rho, S, hipMemcpyDeviceToDevice, s));
+        CGCHK(mpres_dot_v2(c, n, r, zz, rho, blocks_reduce, threads, dws, dw, st));
+        if (k == 0) {
+            HIPCHK(hipMemcpyAsync(p, zz, S * n, hipMemcpyDeviceToDevice, s));
+        } else {
+            CGCHK(mpres_div(c, 1, rho, rhop, beta, st));
+            CGCHK(mpres_axpy_v2(c, n, beta, p, zz, p, st));
+        }
+        CGCHK(mpres_spmv_csr(c, n, A, p, q, st));
+        CGCHK(mpres_dot_v2(c, n, p, q, pq, blocks_reduce, threads, dws, dw, st));
+        CGCHK(mpres_div(c, 1, rho, pq, alpha, st));
+        CGCHK(mpres_axpy_v2(c, n, alpha, p, x, x, st));
+        CGCHK(mpres_maxpy_v2(c, n, alpha, q, r, r, st));
+        CGCHK(norm2_into(c, n, r, blocks_reduce, threads, dws, dw, dres, &normk, s));
+        resvec[k + 1] = normk / norm0;
+        k++;
+    }
+#undef CGCHK
+    *iters = k;
+    return MPRES_OK;
+}
+
 // ---------------------------------------------------------------- axpy
 int mpres_axpy(mpres_ctx_t *c, int n, const mp_array_t *alpha, const mp_array_t *x, int incx, mp_array_t *y, int incy,
                void *st) {
@@ -653,7 +746,49 @@ int mpres_axpy_v2(mpres_ctx_t *c, int n, const void *alpha, const void *x, const
     if (!c || !alpha || !x || !y || !r) return fail(MPRES_ERR_ARG, "null argument");
     if (n <= 0) return MPRES_OK;
     HIPCHK(c->ops[OP_AXPY]->axpy_v2(c->dev, n, (const char *)alpha, (const char *)x, (const char *)y, (char *)r,
-                                     strm(st)));
+                                     false, strm(st)));
+    return MPRES_OK;
+}
+
+// ---------------------------------------------------------------- v2 helpers of the solvers
+int mpres_maxpy_v2(mpres_ctx_t *c, int n, const void *alpha, const void *x, const void *y, void *r, void *st) {
+    if (!c || !alpha || !x || !y || !r) return fail(MPRES_ERR_ARG, "null argument");
+    if (n <= 0) return MPRES_OK;
+    HIPCHK(c->ops[OP_AXPY]->axpy_v2(c->dev, n, (const char *)alpha, (const char *)x, (const char *)y, (char *)r,
+                                     true, strm(st)));
+    return MPRES_OK;
+}
+
+int mpres_diff_v2(mpres_ctx_t *c, int n, const void *x, const void *y, void *r, void *st) {
+    return mpres_elementwise(c, MPRES_OP_SUB, n, x, y, r, st);
+}
+
+int mpres_prod_d_v2(mpres_ctx_t *c, int n, const void *x, const double *d, void *r, void *st) {
+    return mpres_elementwise(c, MPRES_OP_MUL_D, n, x, d, r, st);
+}
+
+// ---------------------------------------------------------------- the reference's host MPFR operations
+int mpres_div(mpres_ctx_t *c, int count, const void *x, const void *y, void *r, void *st) {
+    if (!c) return fail(MPRES_ERR_ARG, "null context");
+    if (count <= 0) return MPRES_OK;
+    if (!x || !y || !r) return fail(MPRES_ERR_ARG, "null argument");
+    HIPCHK(c->ops[OP_ELEM]->hostop(c->dev, 0, count, (const char *)x, (const char *)y, (char *)r, nullptr, strm(st)));
+    return MPRES_OK;
+}
+
+int mpres_sqrt(mpres_ctx_t *c, int count, const void *x, void *r, void *st) {
+    if (!c) return fail(MPRES_ERR_ARG, "null context");
+    if (count <= 0) return MPRES_OK;
+    if (!x || !r) return fail(MPRES_ERR_ARG, "null argument");
+    HIPCHK(c->ops[OP_ELEM]->hostop(c->dev, 1, count, (const char *)x, nullptr, (char *)r, nullptr, strm(st)));
+    return MPRES_OK;
+}
+
+int mpres_dsqrt(mpres_ctx_t *c, int count, const void *x, double *out, void *st) {
+    if (!c) return fail(MPRES_ERR_ARG, "null context");
+    if (count <= 0) return MPRES_OK;
+    if (!x || !out) return fail(MPRES_ERR_ARG, "null argument");
+    HIPCHK(c->ops[OP_ELEM]->hostop(c->dev, 2, count, (const char *)x, nullptr, nullptr, out, strm(st)));
     return MPRES_OK;
 }
 

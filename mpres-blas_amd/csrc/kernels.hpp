@@ -26,6 +26,7 @@
 #include <type_traits>
 
 #include "group.hpp"
+#include "hostops.hpp"
 #include "ops.hpp"
 
 namespace mph {
@@ -245,6 +246,68 @@ __global__ __launch_bounds__(BLK) void k_export(const DevConsts *__restrict__ C,
     }
 }
 
+// ---------------------------------------------------------------- host MPFR operations
+// mp_div (src/arith/div.cuh:31-65) on count numbers: r[i] = x[i] / y[i] as MPFR computes it
+// at MP_PRECISION bits, with the CPU evaluation (hostops.hpp). A zero divisor gives zero
+// (the reference's result is undefined).
+template <int N, int L>
+__global__ __launch_bounds__(BLK) void k_div(const DevConsts *__restrict__ C, int count, AoSRef<N> x, AoSRef<N> y,
+                                            AoSRef<N> r) {
+    const long long i = (long long)blockIdx.x * BLK + threadIdx.x;
+    if (i >= count) return;
+    int dx[N], dy[N];
+    gather_digits<N>(x, i, dx);
+    gather_digits<N>(y, i, dy);
+    const int sign = *x.sign(i) ^ *y.sign(i);
+    int ea, eb, e = 0;
+    const BigN<N> a = mp_to_big<N>(C, dx, *x.exp(i), ea), b = mp_to_big<N>(C, dy, *y.exp(i), eb);
+    BigN<N> q;
+    big_zero(q);
+    if (!big_is_zero(a) && !big_is_zero(b)) q = big_quotient<N>(a, ea, b, eb, C->mp_precision, e);
+    int dr[N], sr, er;
+    Er lo, up;
+    big_to_mp<N>(C, q, e, sign, dr, sr, er, lo, up);
+#pragma unroll
+    for (int k = 0; k < N; k++) r.digits(i)[k] = dr[k];
+    *r.sign(i) = sr;
+    *r.exp(i) = er;
+    r.eval(i)[0] = ErMem{lo.frac, (long long)lo.exp};
+    r.eval(i)[1] = ErMem{up.frac, (long long)up.exp};
+}
+// mp_sqrt (MODE 0: multiple-precision result) and mp_dsqrt (MODE 1: double), src/arith/
+// sqrt.cuh:33-89. A negative operand gives NaN (MODE 1) or zero (MODE 0); MPFR gives NaN.
+template <int N, int L, int MODE>
+__global__ __launch_bounds__(BLK) void k_sqrt(const DevConsts *__restrict__ C, int count, AoSRef<N> x, AoSRef<N> r,
+                                             double *__restrict__ out) {
+    const long long i = (long long)blockIdx.x * BLK + threadIdx.x;
+    if (i >= count) return;
+    int dx[N];
+    gather_digits<N>(x, i, dx);
+    const int sign = *x.sign(i);
+    int ea, e = 0;
+    const BigN<N> a = mp_to_big<N>(C, dx, *x.exp(i), ea);
+    const bool zero = big_is_zero(a);
+    Big<2 * big_limbs<N>()> s;
+    big_zero(s);
+    if (!zero && !sign) s = big_sqrt_round<N>(a, ea, C->mp_precision, e);
+    if constexpr (MODE == 1) {
+        out[i] = (sign && !zero) ? __builtin_nan("") : big_to_double(s, e, sign);
+    } else {
+        BigN<N> q;
+#pragma unroll
+        for (int k = 0; k < big_limbs<N>(); k++) q.l[k] = s.l[k];  // the root has MP_PRECISION bits
+        int dr[N], sr, er;
+        Er lo, up;
+        big_to_mp<N>(C, q, e, sign && zero, dr, sr, er, lo, up);
+#pragma unroll
+        for (int k = 0; k < N; k++) r.digits(i)[k] = dr[k];
+        *r.sign(i) = sr;
+        *r.exp(i) = er;
+        r.eval(i)[0] = ErMem{lo.frac, (long long)lo.exp};
+        r.eval(i)[1] = ErMem{up.frac, (long long)up.exp};
+    }
+}
+
 // ---------------------------------------------------------------- element-wise
 template <int N, int L, int OP>
 __global__ __launch_bounds__(BLK) void k_elem(const DevConsts *__restrict__ C, int count, AoSRef<N> x, AoSRef<N> y,
@@ -407,7 +470,8 @@ __global__ __launch_bounds__(BLK) void k_axpy(const DevConsts *__restrict__ C, i
     store_soa(g, r, y, iy);
 }
 // v2: r = alpha * x (+) y, src/blas/v2/axpy_v2.cuh:39-52
-template <int N, int L>
+// NEG: mp_maxpy, r = (-alpha) * x (+) y (src/blas/v2/maxpy_v2.cuh)
+template <int N, int L, bool NEG = false>
 __global__ __launch_bounds__(BLK) void k_axpy_v2(const DevConsts *__restrict__ C, int n, AoSRef<N> alpha, AoSRef<N> x,
                                                  AoSRef<N> y, AoSRef<N> r) {
     const long long i = group_id<L>();
@@ -415,6 +479,7 @@ __global__ __launch_bounds__(BLK) void k_axpy_v2(const DevConsts *__restrict__ C
     Grp<N, L> g(C);
     MP<N / L> a, xv, yv, ax, s;
     load_aos(g, a, alpha, 0);
+    if (NEG) a.sign ^= 1;
     load_aos(g, xv, x, i);
     load_aos(g, yv, y, i);
     g.mul(ax, a, xv);

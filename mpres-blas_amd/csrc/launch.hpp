@@ -131,10 +131,27 @@ struct Launch {
         return hipGetLastError();
     }
     static hipError_t axpy_v2(const DevConsts *c, int n, const char *alpha, const char *x, const char *y, char *r,
-                              hipStream_t st) {
+                              bool neg, hipStream_t st) {
         if (n <= 0) return hipSuccess;
-        hipLaunchKernelGGL((k_axpy_v2<N, L>), dim3(grid_for(n, L)), dim3(BLK), 0, st, c, n, AoSRef<N>{(char *)alpha},
-                           AoSRef<N>{(char *)x}, AoSRef<N>{(char *)y}, AoSRef<N>{r});
+        const dim3 gr(grid_for(n, L)), bl(BLK);
+        const AoSRef<N> A{(char *)alpha}, X{(char *)x}, Y{(char *)y}, R{r};
+        if (neg)
+            hipLaunchKernelGGL((k_axpy_v2<N, L, true>), gr, bl, 0, st, c, n, A, X, Y, R);
+        else
+            hipLaunchKernelGGL((k_axpy_v2<N, L, false>), gr, bl, 0, st, c, n, A, X, Y, R);
+        return hipGetLastError();
+    }
+    static hipError_t hostop(const DevConsts *c, int op, int count, const char *x, const char *y, char *r, double *out,
+                             hipStream_t st) {
+        if (count <= 0) return hipSuccess;
+        const dim3 gr(grid_for(count, 1)), bl(BLK);
+        const AoSRef<N> X{(char *)x}, Y{(char *)y}, R{r};
+        if (op == 0)
+            hipLaunchKernelGGL((k_div<N, L>), gr, bl, 0, st, c, count, X, Y, R);
+        else if (op == 1)
+            hipLaunchKernelGGL((k_sqrt<N, L, 0>), gr, bl, 0, st, c, count, X, R, out);
+        else
+            hipLaunchKernelGGL((k_sqrt<N, L, 1>), gr, bl, 0, st, c, count, X, R, out);
         return hipGetLastError();
     }
     static hipError_t scal(const DevConsts *c, int len, SoA s, SoA v, int inc, char *out, bool inplace,
@@ -420,6 +437,7 @@ struct Launch {
         o.soa_to_aos = &soa_to_aos;
         o.axpy = &axpy;
         o.axpy_v2 = &axpy_v2;
+        o.hostop = &hostop;
         o.scal = &scal;
         o.gemv_fold = &gemv_fold;
         o.gemv_final = &gemv_final;

@@ -54,7 +54,9 @@ struct Ops {
     hipError_t (*aos_to_soa)(const DevConsts *, const char *, SoA, int, hipStream_t);
     hipError_t (*soa_to_aos)(const DevConsts *, SoA, char *, int, hipStream_t);
     hipError_t (*axpy)(const DevConsts *, int, SoA, SoA, int, SoA, int, hipStream_t);
-    hipError_t (*axpy_v2)(const DevConsts *, int, const char *, const char *, const char *, char *, hipStream_t);
+    hipError_t (*axpy_v2)(const DevConsts *, int, const char *, const char *, const char *, char *, bool, hipStream_t);
+    // the reference's host MPFR operations: 0 mp_div (x / y), 1 mp_sqrt, 2 mp_dsqrt (into out)
+    hipError_t (*hostop)(const DevConsts *, int, int, const char *, const char *, char *, double *, hipStream_t);
     hipError_t (*scal)(const DevConsts *, int, SoA, SoA, int, char *, bool, hipStream_t);
     hipError_t (*gemv_fold)(const DevConsts *, bool, int, int, int, SoA, int, const char *, int, long long, char *,
                             char *, hipStream_t);

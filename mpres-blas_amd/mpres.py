@@ -98,6 +98,19 @@ def lib():
         L.mpres_elementwise.argtypes = [vp, i32, i32, vp, vp, vp, vp]
         L.mpres_cmp.argtypes = [vp, i32, i32, vp, vp, vp, vp]
         L.mpres_get_d.argtypes = [vp, i32, vp, vp, vp]
+        L.mpres_div.argtypes = [vp, i32, vp, vp, vp, vp]
+        L.mpres_sqrt.argtypes = [vp, i32, vp, vp, vp]
+        L.mpres_dsqrt.argtypes = [vp, i32, vp, vp, vp]
+        L.mpres_maxpy_v2.argtypes = [vp, i32, vp, vp, vp, vp, vp]
+        L.mpres_diff_v2.argtypes = [vp, i32, vp, vp, vp, vp]
+        L.mpres_prod_d_v2.argtypes = [vp, i32, vp, vp, vp, vp]
+        L.mpres_norm2_v2_workspace.restype = sz
+        L.mpres_norm2_v2_workspace.argtypes = [vp, i32, i32, i32]
+        L.mpres_norm2_v2.argtypes = [vp, i32, vp, i32, i32, C.POINTER(C.c_double), vp, sz, vp]
+        L.mpres_cg_workspace.restype = sz
+        L.mpres_cg_workspace.argtypes = [vp, i32, i32, i32]
+        L.mpres_cg_csr.argtypes = [vp, i32, vp, vp, C.c_double, i32, vp, vp, C.POINTER(C.c_double),
+                                   C.POINTER(i32), i32, i32, vp, sz, vp]
         L.mpres_array_get_d.argtypes = [vp, i32, vp, vp, vp]
         L.mpres_export_limbs.argtypes = [vp]
         L.mpres_get_integer.argtypes = [vp, i32, vp, vp, vp, vp, vp]

@@ -12,6 +12,9 @@
 #include <omp.h>
 #endif
 
+#include <gmp.h>
+#include <mpfr.h>
+
 #include "orc_internal.h"
 
 #define ORC_N 8
@@ -238,3 +241,37 @@ void orc_spmv_csr(const orc_ctx *c, int policy, int m, const int *irp, const int
                   void *y, int tpr, const int *rows, int nrows) {
     DISPATCH(c, policy, B_SPMV);
 }
+
+/* ---- v2 helpers of the solvers, mp_add_abs, sum of squares ---- */
+#define B_ADDABS(s)                                                                        \
+    for (int i = 0; i < count; i++)                                                        \
+        G(add_abs, s)(c, &((G(mpf, s) *)r)[i], ((const G(mpf, s) *)x)[i], ((const G(mpf, s) *)y)[i]);
+void orc_add_abs(const orc_ctx *c, int policy, int count, const void *x, const void *y, void *r) {
+    DISPATCH(c, policy, B_ADDABS);
+}
+#define B_MAXPY(s) G(maxpy_v2, s)(c, n, (const G(mpf, s) *)alpha, (const G(mpf, s) *)x, (const G(mpf, s) *)y, (G(mpf, s) *)r);
+void orc_maxpy_v2(const orc_ctx *c, int policy, int n, const void *alpha, const void *x, const void *y, void *r) {
+    DISPATCH(c, policy, B_MAXPY);
+}
+#define B_SUMSQ(s) G(sumsq, s)(c, n, (const G(mpf, s) *)x, blocks, threads, (G(mpf, s) *)r);
+void orc_sumsq(const orc_ctx *c, int policy, int n, const void *x, int blocks, int threads, void *r) {
+    DISPATCH(c, policy, B_SUMSQ);
+}
+
+/* ---- host MPFR operations (policy-independent) ---- */
+#define DISPATCH_E(c, BODY)                                                     \
+    do {                                                                        \
+        if ((c)->n == 8) { BODY(_e8) }                                          \
+        else if ((c)->n == 16) { BODY(_e16) }                                   \
+        else { BODY(_e32) }                                                     \
+    } while (0)
+#define B_DIV(s)                                                                           \
+    for (int i = 0; i < count; i++)                                                        \
+        G(div, s)(c, &((G(mpf, s) *)r)[i], &((const G(mpf, s) *)x)[i], &((const G(mpf, s) *)y)[i]);
+void orc_div(const orc_ctx *c, int count, const void *x, const void *y, void *r) { DISPATCH_E(c, B_DIV); }
+#define B_SQRT(s)                                                                          \
+    for (int i = 0; i < count; i++) G(sqrt, s)(c, &((G(mpf, s) *)r)[i], &((const G(mpf, s) *)x)[i]);
+void orc_sqrt(const orc_ctx *c, int count, const void *x, void *r) { DISPATCH_E(c, B_SQRT); }
+#define B_DSQRT(s)                                                                         \
+    for (int i = 0; i < count; i++) out[i] = G(dsqrt, s)(c, &((const G(mpf, s) *)x)[i]);
+void orc_dsqrt(const orc_ctx *c, int count, const void *x, double *out) { DISPATCH_E(c, B_DSQRT); }

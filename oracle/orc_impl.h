@@ -517,6 +517,41 @@ static void F(add)(const orc_ctx *c, MPF *r, MPF x, MPF y) {
     F(round_if_needed)(c, r);
 }
 
+/* mp_add_abs (|x| + |y|): CPU src/arith/addabs.cuh:31-74; CUDA mp_add_abs_common :81-116
+ * and mp_add_abs :118-127. Both align as mp_add does, add the bounds and the scaled
+ * residues without signs, and round. */
+static void F(add_abs)(const orc_ctx *c, MPF *r, MPF x, MPF y) {
+    int ex = x.exp, ey = y.exp;
+    int dexp = ex - ey;
+    int gamma = dexp * (dexp > 0);
+    int theta = -dexp * (dexp < 0);
+    int nzx = ((y.eval[1].frac == 0) || (theta + y.eval[1].exp) < c->mp_j);
+    int nzy = ((x.eval[1].frac == 0) || (gamma + x.eval[1].exp) < c->mp_j);
+    gamma = gamma * nzy;
+    theta = theta * nzx;
+    ex = (ex - gamma) * nzx;
+    ey = (ey - theta) * nzy;
+    x.eval[0].exp += gamma;
+    x.eval[1].exp += gamma;
+    y.eval[0].exp += theta;
+    y.eval[1].exp += theta;
+    x.eval[0].frac *= nzx;
+    x.eval[1].frac *= nzx;
+    y.eval[0].frac *= nzy;
+    y.eval[1].frac *= nzy;
+    r->eval[0] = F(er_add_rd)(x.eval[0], y.eval[0]);
+    r->eval[1] = F(er_add_ru)(x.eval[1], y.eval[1]);
+    r->sign = 0;
+    r->exp = (ex == 0) ? ey : ex;
+    const int pg = F(prow)(c, gamma), pt = F(prow)(c, theta);
+    for (int i = 0; i < N; i++) {
+        long v = ((long)c->pow2[pg * N + i] * nzx * (long)x.digits[i] +
+                  (long)c->pow2[pt * N + i] * nzy * (long)y.digits[i]) % (long)c->m[i];
+        r->digits[i] = (int)v;
+    }
+    F(round_if_needed)(c, r);
+}
+
 /* mp_cmp (abs_only = 0) and mp_cmp_abs (abs_only = 1): CPU src/arith/cmp.cuh:31-67 and
  * cmpabs.cuh:31-64; CUDA mp_cmp_common cmp.cuh:79-114 and mp_cmp_abs_common cmpabs.cuh:76-109.
  * The operands are aligned to the larger exponent (the smaller one is dropped when the larger's
@@ -765,6 +800,149 @@ static void F(axpy_v2)(const orc_ctx *c, int n, const MPF *alpha, const MPF *x, 
         F(add)(c, &r[i], ax, y[i]);
     }
 }
+
+/* src/blas/v2/maxpy_v2.cuh: r = (-alpha) x + y (mp_mul, then mp_add) */
+static void F(maxpy_v2)(const orc_ctx *c, int n, const MPF *alpha, const MPF *x, const MPF *y, MPF *r) {
+    MPF a = *alpha;
+    a.sign ^= 1;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        MPF ax;
+        F(mul)(c, &ax, a, x[i]);
+        F(add)(c, &r[i], ax, y[i]);
+    }
+}
+
+/* halving tree of mp_sumsq_kernel1/2 with mp_add_abs (src/blas/v2/norm2_v2.cuh:50-58) */
+static void F(tree_abs)(const orc_ctx *c, MPF *s, int bsize) {
+    unsigned p = F(next_pow2)((unsigned)bsize);
+    for (unsigned i = p >> 1; i >= 1; i >>= 1) {
+        for (unsigned tid = 0; tid < i; tid++)
+            if (tid + i < (unsigned)bsize) F(add_abs)(c, &s[tid], s[tid], s[tid + i]);
+    }
+}
+/* sum of squares of mp_norm2<blocks, threads> (src/blas/v2/norm2_v2.cuh:36-120):
+ * mp_sumsq_kernel1 (thread tid of block bid folds x_i * x_i, i = bid*threads + tid, step
+ * blocks*threads, by mp_add_abs from zero; halving tree), then mp_sumsq_kernel2 (one block
+ * of `threads` over the `blocks` block sums) */
+static void F(sumsq)(const orc_ctx *c, int n, const MPF *x, int blocks, int threads, MPF *res) {
+    MPF *part = (MPF *)malloc(sizeof(MPF) * (size_t)blocks);
+    const long G = (long)blocks * threads;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int bid = 0; bid < blocks; bid++) {
+        MPF *s = (MPF *)malloc(sizeof(MPF) * (size_t)threads);
+        for (int tid = 0; tid < threads; tid++) {
+            s[tid] = F(MP_ZERO);
+            for (long i = (long)bid * threads + tid; i < n; i += G) {
+                MPF p;
+                F(mul)(c, &p, x[i], x[i]);
+                F(add_abs)(c, &s[tid], s[tid], p);
+            }
+        }
+        F(tree_abs)(c, s, threads);
+        part[bid] = s[0];
+        free(s);
+    }
+    MPF *s = (MPF *)malloc(sizeof(MPF) * (size_t)threads);
+    for (int tid = 0; tid < threads; tid++) {
+        s[tid] = F(MP_ZERO);
+        for (int i = tid; i < blocks; i += threads) F(add_abs)(c, &s[tid], s[tid], part[i]);
+    }
+    F(tree_abs)(c, s, threads);
+    *res = s[0];
+    free(s);
+    free(part);
+}
+
+#if ORC_POL == 0
+/* ------------------------------------------------------------------ */
+/* Host MPFR operations of the reference (policy-independent: the CUDA  */
+/* mp_div / mp_sqrt / mp_dsqrt copy the operands to the host and run    */
+/* these, src/arith/div.cuh:56-65, src/arith/sqrt.cuh:65-89)            */
+/* ------------------------------------------------------------------ */
+/* mp_get_mpfr, src/arith/assign.cuh:185-211: X = sum digits_i * RNS_ORTHOGONAL_BASE_i mod M,
+ * then (-1)^sign X 2^exp rounded to r's precision to nearest (mpfr_set_str of the binary
+ * string there, mpfr_set_z_2exp here: both round the exact value correctly) */
+static void F(get_mpfr)(const orc_ctx *c, mpfr_t r, const MPF *x) {
+    mpz_t X, M, t;
+    mpz_inits(X, M, t, NULL);
+    mpz_set_ui(M, 1);
+    for (int i = 0; i < N; i++) mpz_mul_ui(M, M, (unsigned long)c->m[i]);
+    for (int i = 0; i < N; i++) {
+        mpz_divexact_ui(t, M, (unsigned long)c->m[i]);  /* M_i */
+        mpz_mul_ui(t, t, (unsigned long)c->w[i]);       /* M_i * w_i */
+        mpz_mul_si(t, t, (long)x->digits[i]);
+        mpz_add(X, X, t);
+    }
+    mpz_mod(X, X, M);
+    mpfr_set_z_2exp(r, X, (long)x->exp, MPFR_RNDN);
+    if (x->sign == 1) mpfr_neg(r, r, MPFR_RNDN);  /* "-" prefix: a zero becomes -0, as the string does */
+    mpz_clears(X, M, t, NULL);
+}
+/* mp_set_mpfr, src/arith/assign.cuh:86-127: the significand string of mpfr_get_str (base 2,
+ * all digits), trailing zeros dropped, residues of its magnitude, the CPU evaluation */
+static void F(set_mpfr)(const orc_ctx *c, MPF *r, mpfr_srcptr x) {
+    mpfr_exp_t e;
+    char *str = mpfr_get_str(NULL, &e, 2, 0, x, MPFR_RNDN);
+    size_t len = strlen(str);
+    size_t num = 0;
+    for (size_t i = len; i-- > 0;)
+        if (str[i] != '0') {
+            num = i;
+            break;
+        }
+    mpz_t mant, rem;
+    mpz_inits(mant, rem, NULL);
+    size_t used = (num != len - 1) ? num + 1 : len;
+    char *sub = (char *)malloc(used + 1);
+    memcpy(sub, str, used);
+    sub[used] = 0;
+    if (mpz_set_str(mant, sub, 2) != 0) mpz_set_ui(mant, 0);  /* "-" alone: mant keeps 0 */
+    r->exp = (int)(e - (long)used);
+    if (str[0] == '-') {
+        r->sign = 1;
+        r->exp++;
+        mpz_neg(mant, mant);
+    } else {
+        r->sign = 0;
+    }
+    for (int i = 0; i < N; i++) r->digits[i] = (int)mpz_fdiv_r_ui(rem, mant, (unsigned long)c->m[i]);
+    F(eval_compute)(c, &r->eval[0], &r->eval[1], r->digits, 0);
+    free(sub);
+    mpfr_free_str(str);
+    mpz_clears(mant, rem, NULL);
+}
+/* mp_div, src/arith/div.cuh:31-44 */
+static void F(div)(const orc_ctx *c, MPF *r, const MPF *x, const MPF *y) {
+    mpfr_t a, b;
+    mpfr_init2(a, c->mp_precision);
+    mpfr_init2(b, c->mp_precision);
+    F(get_mpfr)(c, a, x);
+    F(get_mpfr)(c, b, y);
+    mpfr_div(a, a, b, MPFR_RNDN);
+    F(set_mpfr)(c, r, a);
+    mpfr_clear(a);
+    mpfr_clear(b);
+}
+/* mp_sqrt and mp_dsqrt, src/arith/sqrt.cuh:33-57 */
+static void F(sqrt)(const orc_ctx *c, MPF *r, const MPF *x) {
+    mpfr_t a;
+    mpfr_init2(a, c->mp_precision);
+    F(get_mpfr)(c, a, x);
+    mpfr_sqrt(a, a, MPFR_RNDN);
+    F(set_mpfr)(c, r, a);
+    mpfr_clear(a);
+}
+static double F(dsqrt)(const orc_ctx *c, const MPF *x) {
+    mpfr_t a;
+    mpfr_init2(a, c->mp_precision);
+    F(get_mpfr)(c, a, x);
+    mpfr_sqrt(a, a, MPFR_RNDN);
+    double d = mpfr_get_d(a, MPFR_RNDN);
+    mpfr_clear(a);
+    return d;
+}
+#endif
 
 /* src/blas/gemv.cuh:44-268 */
 static void F(gemv)(const orc_ctx *c, int trans, int m, int n, const MPF *alpha, const MPF *A, int lda, const MPF *x,

@@ -256,6 +256,74 @@ class Oracle:
                            _p(np.ascontiguousarray(beta)), _p(Cm), ldc, _p(e), 0 if e is None else len(e) // 2)
         return Cm
 
+    def add_abs(self, x, y, policy=POLICY_D):
+        return self._bin("orc_add_abs", x, y, policy)
+
+    def maxpy_v2(self, alpha, x, y, policy=POLICY_D):
+        r = self.empty(len(x))
+        self.L.orc_maxpy_v2(self.ctx, policy, len(x), _p(np.ascontiguousarray(alpha)), _p(np.ascontiguousarray(x)),
+                            _p(np.ascontiguousarray(y)), _p(r))
+        return r
+
+    def sumsq(self, x, blocks, threads, policy=POLICY_D):
+        r = self.empty(1)
+        self.L.orc_sumsq(self.ctx, policy, len(x), _p(np.ascontiguousarray(x)), blocks, threads, _p(r))
+        return r
+
+    # the reference's host MPFR operations (the same under both policies)
+    def div(self, x, y):
+        r = self.empty(len(x))
+        self.L.orc_div(self.ctx, len(x), _p(np.ascontiguousarray(x)), _p(np.ascontiguousarray(y)), _p(r))
+        return r
+
+    def sqrt(self, x):
+        r = self.empty(len(x))
+        self.L.orc_sqrt(self.ctx, len(x), _p(np.ascontiguousarray(x)), _p(r))
+        return r
+
+    def dsqrt(self, x):
+        out = np.zeros(len(x), np.float64)
+        self.L.orc_dsqrt(self.ctx, len(x), _p(np.ascontiguousarray(x)), _p(out))
+        return out
+
+    def norm2(self, x, blocks, threads, policy=POLICY_D):
+        """cuda::mp_norm2<blocks, threads> (src/blas/v2/norm2_v2.cuh:107-119)"""
+        return float(self.dsqrt(self.sumsq(x, blocks, threads, policy))[0])
+
+    def cg_csr(self, n, irp, ja, as_, b, tol, maxit, x, threads=32, blocks_reduce=256, policy=POLICY_D,
+               precond=None):
+        """mp_cg_csr<threads, blocks_reduce> (src/sparse/solver/cg_csr.cuh:52-117) or, with precond (the
+        reciprocals of A's diagonal), mp_pcg_csr with the diagonal preconditioner (pcg_csr.cuh:52-130),
+        composed from the oracle's operations in the reference's order. Returns (x, resvec)."""
+        x = np.array(x, copy=True)
+        r = self.spmv(n, irp, ja, as_, x, 1, policy)
+        r = self.sub(b, r, policy)
+        norm0 = self.norm2(r, blocks_reduce, threads, policy)
+        eps = norm0 * tol
+        normk = norm0
+        resvec = [normk / norm0]
+        k = 0
+        rho = self.empty(1)
+        p = None
+        while normk > eps and k < maxit:
+            z = r if precond is None else self.mul_d(r, precond, policy)
+            rhop = rho
+            rho = self.dot(r, z, blocks_reduce, threads, policy)
+            if k == 0:
+                p = np.array(z, copy=True)
+            else:
+                beta = self.div(rho, rhop)
+                p = self.axpy_v2(beta, p, z, policy)
+            q = self.spmv(n, irp, ja, as_, p, 1, policy)
+            pq = self.dot(p, q, blocks_reduce, threads, policy)
+            alpha = self.div(rho, pq)
+            x = self.axpy_v2(alpha, p, x, policy)
+            r = self.maxpy_v2(alpha, q, r, policy)
+            normk = self.norm2(r, blocks_reduce, threads, policy)
+            resvec.append(normk / norm0)
+            k += 1
+        return x, resvec
+
     def spmv(self, m, irp, ja, as_, x, tpr=1, policy=POLICY_D, rows=None):
         irp = np.ascontiguousarray(irp, np.int32)
         ja = np.ascontiguousarray(ja, np.int32)

@@ -290,6 +290,106 @@ def test_get_d_and_integer(name):
         assert X == exact.crt(allr[i]["digits"], m), f"record {i}"
 
 
+@pytest.mark.parametrize("name", ["8_2double", "16_4double", "32_8double"])
+def test_div_sqrt(name):
+    """mpres_div / mpres_sqrt / mpres_dsqrt (the reference's host MPFR operations) against the
+    oracle's MPFR restatement, bit for bit: full-length significands, zeros of both signs,
+    equal operands, powers of two"""
+    c, o = ctx(name), orc(name)
+    a = o.set_d(inputs(o, 91, 300), D)
+    b = o.set_d(inputs(o, 92, 300), D)
+    x = np.concatenate([o.mul(a, b, D), o.add(a, b, D), a, o.set_d([0.0, -0.0, 4.0, 2.0**-30], D)])
+    y = np.concatenate([o.add(b, a, D), o.mul(b, b, D), b[::-1], o.set_d([3.0, -3.0, 4.0, 1.0], D)])
+    y[y["eval"][:, 1]["frac"] == 0] = o.set_d([7.0], D)[0]  # no zero divisors
+    n = len(x)
+    note("div/sqrt", name, n)
+    dx, dy, dr = c.aos(x), c.aos(y), c.aos(count=n)
+    mp.check(c.L.mpres_div(c.h, n, dx.ptr, dy.ptr, dr.ptr, None))
+    same(c.aos_to_host(dr, n), o.div(x, y), f"div {name}")
+    xs = np.array(x, copy=True)
+    xs["sign"] = 0
+    ds, dd = c.aos(xs), mp.DevBuf(nbytes=8 * n)
+    mp.check(c.L.mpres_sqrt(c.h, n, ds.ptr, dr.ptr, None))
+    same(c.aos_to_host(dr, n), o.sqrt(xs), f"sqrt {name}")
+    mp.check(c.L.mpres_dsqrt(c.h, n, ds.ptr, dd.ptr, None))
+    assert dd.to_numpy(np.float64, n).tobytes() == o.dsqrt(xs).tobytes()
+
+
+@pytest.mark.parametrize("name", ["8_2double", "32_8double"])
+def test_solver_helpers(name):
+    """v2 maxpy, diff, prod_d and norm2 (every order of the two reduction kernels used)"""
+    c, o = ctx(name), orc(name)
+    n = 3000
+    x, y = o.set_d(inputs(o, 93, n), D), o.set_d(inputs(o, 94, n), D)
+    alpha = o.set_d([-0.375], D)
+    dvals = gen.uexp(95, n, -10, 10)
+    dx, dy, da, dr = c.aos(x), c.aos(y), c.aos(alpha), c.aos(count=n)
+    note("solver helpers", name)
+    mp.check(c.L.mpres_maxpy_v2(c.h, n, da.ptr, dx.ptr, dy.ptr, dr.ptr, None))
+    same(c.aos_to_host(dr, n), o.maxpy_v2(alpha, x, y, D), f"maxpy {name}")
+    mp.check(c.L.mpres_diff_v2(c.h, n, dx.ptr, dy.ptr, dr.ptr, None))
+    same(c.aos_to_host(dr, n), o.sub(x, y, D), f"diff {name}")
+    dd = mp.DevBuf(host=dvals)
+    mp.check(c.L.mpres_prod_d_v2(c.h, n, dx.ptr, dd.ptr, dr.ptr, None))
+    same(c.aos_to_host(dr, n), o.mul_d(x, dvals, D), f"prod_d {name}")
+    for blocks, threads in ((256, 32), (7, 64), (3, 5)):
+        r = C.c_double()
+        mp.check(c.L.mpres_norm2_v2(c.h, n, dx.ptr, blocks, threads, C.byref(r), None, 0, None))
+        assert r.value == o.norm2(x, blocks, threads, D), f"norm2 {name} {blocks}x{threads}"
+
+
+def _cg_case(name, n, irp, ja, as_, b, tol, maxit, precond, threads, blocks_reduce):
+    c, o = ctx(name), orc(name)
+    x0 = o.set_d(np.zeros(n), D)
+    xw, rw = o.cg_csr(n, irp, ja, as_, b, tol, maxit, x0, threads=threads, blocks_reduce=blocks_reduce,
+                      precond=precond)
+    dirp, dja, das = mp.DevBuf(host=irp), mp.DevBuf(host=ja), mp.DevBuf(host=as_)
+    csr = mp.Csr(das.ptr.value, dja.ptr.value, dirp.ptr.value)
+    db, dx = c.aos(b), c.aos(x0)
+    dm = mp.DevBuf(host=precond) if precond is not None else None
+    res = (C.c_double * (maxit + 1))()
+    it = C.c_int()
+    mp.check(c.L.mpres_cg_csr(c.h, n, C.byref(csr), db.ptr, tol, maxit, dx.ptr, dm.ptr if dm else None, res,
+                              C.byref(it), threads, blocks_reduce, None, 0, None))
+    assert it.value == len(rw) - 1, f"{it.value} iterations, oracle {len(rw) - 1}"
+    assert list(res[:it.value + 1]) == rw
+    same(c.aos_to_host(dx, n), xw, f"cg x {name}")
+    return it.value
+
+
+@pytest.mark.parametrize("name", ["8_2double", "16_4double", "32_8double"])
+@pytest.mark.parametrize("precond", [False, True])
+def test_cg_lf10(name, precond):
+    """mp_cg_csr / mp_pcg_csr (diagonal) on the reference's LF10 matrix: every iterate's
+    relative residual and the solution, bit for bit against the oracle's composition"""
+    n, _, irp, ja, as_ = gen.read_mtx(os.path.join(ROOT, "tests", "golden", "LF10.mtx"))
+    o = orc(name)
+    b = o.spmv(n, irp, ja, as_, o.set_d(gen.uexp(96, n, -2, 2), D), 1, D)
+    diag = np.array([as_[irp[i] + list(ja[irp[i]:irp[i + 1]]).index(i)] for i in range(n)])
+    note("cg lf10", name, precond)
+    its = _cg_case(name, n, irp, ja, as_, b, {"8_2double": 1e-25, "16_4double": 1e-50, "32_8double": 1e-90}[name],
+                   300, 1.0 / diag if precond else None, 32, 256)
+    assert its > 5
+
+
+def test_cg_laplacian():
+    """CG on the 1D Laplacian of size 5000 (several blocks in every reduction), 30 iterations"""
+    n = 5000
+    irp = np.zeros(n + 1, np.int32)
+    ja, as_ = [], []
+    for i in range(n):
+        for j, v in ((i - 1, -1.0), (i, 2.0), (i + 1, -1.0)):
+            if 0 <= j < n:
+                ja.append(j)
+                as_.append(v)
+        irp[i + 1] = len(ja)
+    ja, as_ = np.array(ja, np.int32), np.array(as_, np.float64)
+    o = orc("16_4double")
+    b = o.set_d(gen.uexp(97, n, -3, 3), D)
+    note("cg laplacian", n)
+    _cg_case("16_4double", n, irp, ja, as_, b, 1e-40, 30, None, 64, 128)
+
+
 def test_pool_two_streams():
     """The context's workspace pool shared by calls on two streams: a call on another stream
     waits for the pool's last use, and the pool grows only after that use has ended, so
