@@ -60,6 +60,9 @@ typedef struct {                                                      /* src/typ
     er_float_t *eval;
 } mp_collection_t;
 typedef struct { double *as; int *ja; int *irp; } csr_t;              /* src/types.cuh:109-113 */
+typedef struct { double *as; int *ja; int *jcp; int *perm; } jad_t;   /* src/types.cuh:118-123 */
+typedef struct { double *as; int *ja; } ell_t;                        /* src/types.cuh:128-131 */
+typedef struct { double *as; int *offset; } dia_t;                    /* src/types.cuh:136-139 */
 
 /* The layouts above are the reference's (LP64: long and pointers are 8 bytes). */
 #ifdef __cplusplus
@@ -76,6 +79,9 @@ MPRES_STATIC_ASSERT(sizeof(mp_array_t) == 48 && offsetof(mp_array_t, sign) == 8 
                     "mp_array_t layout");
 MPRES_STATIC_ASSERT(sizeof(mp_collection_t) == 32 && offsetof(mp_collection_t, eval) == 24, "mp_collection_t layout");
 MPRES_STATIC_ASSERT(sizeof(csr_t) == 24 && offsetof(csr_t, ja) == 8 && offsetof(csr_t, irp) == 16, "csr_t layout");
+MPRES_STATIC_ASSERT(sizeof(jad_t) == 32 && offsetof(jad_t, jcp) == 16 && offsetof(jad_t, perm) == 24, "jad_t layout");
+MPRES_STATIC_ASSERT(sizeof(ell_t) == 16 && offsetof(ell_t, ja) == 8, "ell_t layout");
+MPRES_STATIC_ASSERT(sizeof(dia_t) == 16 && offsetof(dia_t, offset) == 8, "dia_t layout");
 
 /* mblas_trans_type, src/blas/mblas_enum.cuh:25-31 */
 #define MPRES_NO_TRANS 111
@@ -241,6 +247,23 @@ size_t mpres_norm2_v2_workspace(const mpres_ctx_t *ctx, int n, int blocks, int t
 int mpres_norm2_v2(mpres_ctx_t *ctx, int n, const void *x, int blocks, int threads, double *result, void *ws,
                    size_t ws_bytes, void *stream);
 
+/* ---- Matrix Market input (host functions, no GPU work): src/sparse/utils/mtx_reader.cuh ----
+ * mpres_mtx_info: sizes, number of entry lines, longest row (maxnzr, mirrored entries of
+ * symmetric files counted), diagonal entries (collect_mtx_stats :52-90); mpres_mtx_nnz: the
+ * stored entries once mirrored. The builders fill caller-allocated host arrays exactly as the
+ * reference's build_csr / build_ell / build_jad / build_dia / build_dense do, including the
+ * within-row order left by its std::sort by row (which fixes the SpMV summation order). */
+int mpres_mtx_info(const char *path, int *m, int *n, int *lines, int *maxnzr, int *nzmd, int *symmetric);
+int mpres_mtx_nnz(int lines, int nzmd, int symmetric);
+int mpres_mtx_ndiag(const char *path, int lines, int symmetric, int *ndiag);
+int mpres_mtx_csr(const char *path, int m, int nnz, int lines, int symmetric, double *as, int *ja, int *irp);
+int mpres_mtx_ell(const char *path, int m, int maxnzr, int lines, int symmetric, double *as, int *ja);
+int mpres_mtx_jad(const char *path, int m, int maxnzr, int nnz, int lines, int symmetric, double *as, int *ja,
+                  int *jcp, int *perm);
+int mpres_mtx_dia(const char *path, int m, int lines, int symmetric, int ndiag, double *as, int *offset);
+int mpres_mtx_dense(const char *path, int m, int n, int lines, int symmetric, double *a);
+int mpres_read_vector(const char *path, int n, double *v);
+
 /* ---- solvers ----
  * A x = b by conjugate gradients: mp_cg_csr<threads, blocks_reduce> (src/sparse/solver/
  * cg_csr.cuh:52-117), or with precond != NULL (device array of n doubles, the reciprocals of
@@ -252,6 +275,26 @@ int mpres_norm2_v2(mpres_ctx_t *ctx, int n, const void *x, int blocks, int threa
  * reads the residual norm on the host, so the stream is synchronized once per iteration.
  * ws: mpres_cg_workspace() bytes, or NULL for the context's pool. */
 size_t mpres_cg_workspace(const mpres_ctx_t *ctx, int n, int threads, int blocks_reduce);
+/* a sparse matrix for mpres_cg: one of the formats above with its sizes */
+#define MPRES_SPARSE_CSR 0
+#define MPRES_SPARSE_ELL 1
+#define MPRES_SPARSE_JAD 2
+#define MPRES_SPARSE_DIA 3
+typedef struct {
+    int format;  /* MPRES_SPARSE_* */
+    int maxnzr;  /* ELL, JAD */
+    int ndiag;   /* DIA (the matrix is n x n) */
+    csr_t csr;
+    ell_t ell;
+    jad_t jad;
+    dia_t dia;
+} mpres_sparse_t;
+/* mp_cg_csr / mp_cg_ell / mp_cg_jad / mp_cg_dia and the mp_pcg_* with the diagonal
+ * preconditioner (src/sparse/solver/{cg,pcg}_{csr,ell,jad,dia}.cuh): the solver above over A's
+ * scalar SpMV kernel */
+int mpres_cg(mpres_ctx_t *ctx, int n, const mpres_sparse_t *A, const void *b, double tol, int maxit, void *x,
+             const double *precond, double *resvec, int *iters, int threads, int blocks_reduce, void *ws,
+             size_t ws_bytes, void *stream);
 int mpres_cg_csr(mpres_ctx_t *ctx, int n, const csr_t *A, const void *b, double tol, int maxit, void *x,
                  const double *precond, double *resvec, int *iters, int threads, int blocks_reduce, void *ws,
                  size_t ws_bytes, void *stream);
@@ -268,6 +311,21 @@ int mpres_spmv_csr(mpres_ctx_t *ctx, int m, const csr_t *csr, const void *x, voi
 /* cuda::mp_spmv_csrv<threads,threadsPerRow>, src/sparse/spmv/spmv_csrv.cuh:48-87; threads_per_row in {1,2,4,8,16,32} */
 int mpres_spmv_csrv(mpres_ctx_t *ctx, int m, const csr_t *csr, int threads_per_row, const void *x, void *y,
                     void *stream);
+/* y = A x with A in ELLPACK form: mp_spmv_ell<threads>, src/sparse/spmv/spmv_ell.cuh:50-66.
+ * ell->as, ell->ja: m x maxnzr column-major (slot c of row r at c*m + r); zero values are
+ * skipped. Order: the slots of each row in turn. */
+int mpres_spmv_ell(mpres_ctx_t *ctx, int m, int maxnzr, const ell_t *ell, const void *x, void *y, void *stream);
+/* y = A x with A in jagged-diagonal form: mp_spmv_jad<threads> (src/sparse/spmv/spmv_jad.cuh:50-66)
+ * and mp_spmv_jadv<threads, threadsPerRow> (spmv_jadv.cuh:50-87: lane-strided diagonals, then the
+ * 16/8/4/2/1 tree); row r's result goes to y[perm[r]]. */
+int mpres_spmv_jad(mpres_ctx_t *ctx, int m, int maxnzr, const jad_t *jad, const void *x, void *y, void *stream);
+int mpres_spmv_jadv(mpres_ctx_t *ctx, int m, int maxnzr, const jad_t *jad, int threadsPerRow, const void *x, void *y,
+                    void *stream);
+/* y = A x with A in diagonal form: mp_spmv_dia<threads>, src/sparse/spmv/spmv_dia.cuh:50-66.
+ * dia->as: m x ndiag column-major (diagonal i of row r at i*m + r), dia->offset: ndiag column
+ * offsets; n: the number of columns. Terms with a column outside [0, n) or a zero value are
+ * skipped. */
+int mpres_spmv_dia(mpres_ctx_t *ctx, int m, int n, int ndiag, const dia_t *dia, const void *x, void *y, void *stream);
 
 /* ---- batched form: many independent dot / gemv problems in one launch ----
  * No counterpart in the reference: a batch stands for a loop of the single calls

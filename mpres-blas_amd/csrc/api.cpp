@@ -680,9 +680,30 @@ size_t mpres_cg_workspace(const mpres_ctx_t *c, int n, int threads, int blocks_r
     return mpsize(c) * (4 * (size_t)n + 5) + mpres_dot_workspace(c, n, blocks_reduce, threads) + 16;
 }
 
+// y = A x in the solver's format (the scalar SpMV kernels: one thread per row in the reference)
+static int solver_spmv(mpres_ctx_t *c, int n, const mpres_sparse_t *A, const void *x, void *y, void *st) {
+    switch (A->format) {
+        case MPRES_SPARSE_CSR: return mpres_spmv_csr(c, n, &A->csr, x, y, st);
+        case MPRES_SPARSE_ELL: return mpres_spmv_ell(c, n, A->maxnzr, &A->ell, x, y, st);
+        case MPRES_SPARSE_JAD: return mpres_spmv_jad(c, n, A->maxnzr, &A->jad, x, y, st);
+        case MPRES_SPARSE_DIA: return mpres_spmv_dia(c, n, n, A->ndiag, &A->dia, x, y, st);
+        default: return fail(MPRES_ERR_ARG, "unknown sparse format %d", A->format);
+    }
+}
+
 int mpres_cg_csr(mpres_ctx_t *c, int n, const csr_t *A, const void *b, double tol, int maxit, void *x,
                  const double *precond, double *resvec, int *iters, int threads, int blocks_reduce, void *ws,
                  size_t wsb, void *st) {
+    if (!A) return fail(MPRES_ERR_ARG, "null argument");
+    mpres_sparse_t S{};
+    S.format = MPRES_SPARSE_CSR;
+    S.csr = *A;
+    return mpres_cg(c, n, &S, b, tol, maxit, x, precond, resvec, iters, threads, blocks_reduce, ws, wsb, st);
+}
+
+int mpres_cg(mpres_ctx_t *c, int n, const mpres_sparse_t *A, const void *b, double tol, int maxit, void *x,
+             const double *precond, double *resvec, int *iters, int threads, int blocks_reduce, void *ws, size_t wsb,
+             void *st) {
     if (!c || !A || !b || !x || !resvec || !iters) return fail(MPRES_ERR_ARG, "null argument");
     if (n <= 0 || maxit < 0 || threads <= 0 || blocks_reduce <= 0) return fail(MPRES_ERR_ARG, "invalid arguments");
     const hipStream_t s = strm(st);
@@ -700,7 +721,7 @@ int mpres_cg_csr(mpres_ctx_t *c, int n, const csr_t *A, const void *b, double to
         if (int e_ = (call)) return e_; \
     } while (0)
     // r = b - A x (src/sparse/solver/cg_csr.cuh:78-80)
-    CGCHK(mpres_spmv_csr(c, n, A, x, r, st));
+    CGCHK(solver_spmv(c, n, A, x, r, st));
     CGCHK(mpres_diff_v2(c, n, b, r, r, st));
     double norm0 = 0, normk = 0;
     CGCHK(norm2_into(c, n, r, blocks_reduce, threads, dws, dw, dres, &norm0, s));
@@ -718,7 +739,7 @@ int mpres_cg_csr(mpres_ctx_t *c, int n, const csr_t *A, const void *b, double to
             CGCHK(mpres_div(c, 1, rho, rhop, beta, st));
             CGCHK(mpres_axpy_v2(c, n, beta, p, zz, p, st));
         }
-        CGCHK(mpres_spmv_csr(c, n, A, p, q, st));
+        CGCHK(solver_spmv(c, n, A, p, q, st));
         CGCHK(mpres_dot_v2(c, n, p, q, pq, blocks_reduce, threads, dws, dw, st));
         CGCHK(mpres_div(c, 1, rho, pq, alpha, st));
         CGCHK(mpres_axpy_v2(c, n, alpha, p, x, x, st));
@@ -926,8 +947,8 @@ int mpres_gemm_v2(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const voi
 }
 
 // ---------------------------------------------------------------- SpMV
-int mpres_spmv_csrv(mpres_ctx_t *c, int m, const csr_t *csr, int tpr, const void *x, void *y, void *st) {
-    if (!c || !csr || !x || !y) return fail(MPRES_ERR_ARG, "null argument");
+// one SpMV in any storage format: the main kernel, then the fixup of the rows it marked
+static int spmv_common(mpres_ctx_t *c, int fmt, int m, const SpMat &A, int tpr, const void *x, void *y, void *st) {
     if (m <= 0) return MPRES_OK;
     if (tpr < 1 || tpr > 32 || (tpr & (tpr - 1))) return fail(MPRES_ERR_ARG, "threadsPerRow %d not in 1..32", tpr);
     const Ops *o = c->ops[OP_SPMV];
@@ -944,14 +965,42 @@ int mpres_spmv_csrv(mpres_ctx_t *c, int m, const csr_t *csr, int tpr, const void
     }
     {
         ProfScope ps(c, OP_SPMV, strm(st));
-        HIPCHK(o->spmv(c->dev, 0, m, csr->irp, csr->ja, csr->as, (const char *)x, (char *)y, tpr, strm(st)));
+        HIPCHK(o->spmv(c->dev, 0, fmt, m, A, (const char *)x, (char *)y, tpr, strm(st)));
     }
-    HIPCHK(o->spmv(c->dev, 1, m, csr->irp, csr->ja, csr->as, (const char *)x, (char *)y, tpr, strm(st)));
+    HIPCHK(o->spmv(c->dev, 1, fmt, m, A, (const char *)x, (char *)y, tpr, strm(st)));
     return MPRES_OK;
+}
+
+int mpres_spmv_csrv(mpres_ctx_t *c, int m, const csr_t *csr, int tpr, const void *x, void *y, void *st) {
+    if (!c || !csr || !x || !y) return fail(MPRES_ERR_ARG, "null argument");
+    return spmv_common(c, SP_CSR, m, SpMat{csr->as, csr->ja, csr->irp, nullptr, nullptr, 0, 0}, tpr, x, y, st);
 }
 
 int mpres_spmv_csr(mpres_ctx_t *c, int m, const csr_t *csr, const void *x, void *y, void *st) {
     return mpres_spmv_csrv(c, m, csr, 1, x, y, st);
+}
+
+int mpres_spmv_ell(mpres_ctx_t *c, int m, int maxnzr, const ell_t *ell, const void *x, void *y, void *st) {
+    if (!c || !ell || !x || !y) return fail(MPRES_ERR_ARG, "null argument");
+    if (maxnzr < 0) return fail(MPRES_ERR_ARG, "maxnzr %d", maxnzr);
+    return spmv_common(c, SP_ELL, m, SpMat{ell->as, ell->ja, nullptr, nullptr, nullptr, maxnzr, 0}, 1, x, y, st);
+}
+
+int mpres_spmv_jadv(mpres_ctx_t *c, int m, int maxnzr, const jad_t *jad, int tpr, const void *x, void *y, void *st) {
+    if (!c || !jad || !x || !y) return fail(MPRES_ERR_ARG, "null argument");
+    if (maxnzr < 0) return fail(MPRES_ERR_ARG, "maxnzr %d", maxnzr);
+    return spmv_common(c, SP_JAD, m, SpMat{jad->as, jad->ja, jad->jcp, jad->perm, nullptr, maxnzr, 0}, tpr, x, y,
+                       st);
+}
+
+int mpres_spmv_jad(mpres_ctx_t *c, int m, int maxnzr, const jad_t *jad, const void *x, void *y, void *st) {
+    return mpres_spmv_jadv(c, m, maxnzr, jad, 1, x, y, st);
+}
+
+int mpres_spmv_dia(mpres_ctx_t *c, int m, int n, int ndiag, const dia_t *dia, const void *x, void *y, void *st) {
+    if (!c || !dia || !x || !y) return fail(MPRES_ERR_ARG, "null argument");
+    if (ndiag < 0 || n < 0) return fail(MPRES_ERR_ARG, "invalid dimensions");
+    return spmv_common(c, SP_DIA, m, SpMat{dia->as, nullptr, nullptr, nullptr, dia->offset, ndiag, n}, 1, x, y, st);
 }
 
 // ---------------------------------------------------------------- profiling

@@ -1181,9 +1181,49 @@ __global__ __launch_bounds__(GV_RT) void k_gemv_resid(const DevConsts *__restric
 // (:66-80) through LDS. One workgroup holds BLK/L groups = (BLK/L)/tpr rows.
 // The fast kernel marks a row whose chain it abandoned with y[row].sign = REDO;
 // the fixup kernel recomputes those rows only.
-template <int N, int L, bool FULL, bool FIX>
-__global__ __launch_bounds__(BLK) void k_spmv(const DevConsts *__restrict__ C, int m, const int *__restrict__ irp,
-                                              const int *__restrict__ ja, const double *__restrict__ as, AoSRef<N> x,
+//
+// The storage formats differ only in which terms a lane visits, in which order, and
+// where the row's result goes (FMT):
+//   SP_CSR  entries irp[row] + lane, + tpr, ... (spmv_csr.cuh:53-62, spmv_csrv.cuh:61-65)
+//   SP_ELL  slots c = lane, + tpr, ... of the column-major m x width arrays, skipping
+//           zero values (spmv_ell.cuh:53-66)
+//   SP_JAD  jagged diagonals j = lane, + tpr, ... while row + jcp[j] < jcp[j + 1];
+//           the result goes to y[perm[row]] (spmv_jad.cuh:53-66, spmv_jadv.cuh:53-62)
+//   SP_DIA  diagonals i = lane, + tpr, ... with 0 <= row + offset[i] < ncols and a
+//           nonzero value (spmv_dia.cuh:53-66)
+template <int FMT>
+__device__ __forceinline__ long long sp_out_row(const SpMat &A, long long row) {
+    if constexpr (FMT == SP_JAD) return A.perm[row];
+    return row;
+}
+// the row's terms of one lane, in the reference's order: f(column, value) until it returns false
+template <int FMT, class F>
+__device__ __forceinline__ void sp_terms(const SpMat &A, int m, long long row, int lane, int tpr, F &&f) {
+    if constexpr (FMT == SP_CSR) {
+        const int re = A.ptr[row + 1];
+        for (int i = A.ptr[row] + lane; i < re; i += tpr)
+            if (!f(A.ja[i], A.as[i])) return;
+    } else if constexpr (FMT == SP_ELL) {
+        for (int c = lane; c < A.width; c += tpr) {
+            const long long q = (long long)c * m + row;
+            const double v = A.as[q];
+            if (v != 0 && !f(A.ja[q], v)) return;
+        }
+    } else if constexpr (FMT == SP_JAD) {
+        for (int j = lane; j < A.width; j += tpr) {
+            const long long q = row + A.ptr[j];
+            if (q < A.ptr[j + 1] && !f(A.ja[q], A.as[q])) return;
+        }
+    } else {
+        for (int i = lane; i < A.width; i += tpr) {
+            const long long col = row + A.offset[i];
+            const double v = A.as[(long long)m * i + row];
+            if (col >= 0 && col < A.ncols && v != 0 && !f((int)col, v)) return;
+        }
+    }
+}
+template <int N, int L, bool FULL, bool FIX, int FMT = SP_CSR>
+__global__ __launch_bounds__(BLK) void k_spmv(const DevConsts *__restrict__ C, int m, SpMat A, AoSRef<N> x,
                                               AoSRef<N> y, int tpr) {
     constexpr int R = N / L;
     constexpr int W = BLK / L;
@@ -1196,15 +1236,17 @@ __global__ __launch_bounds__(BLK) void k_spmv(const DevConsts *__restrict__ C, i
     const long long row = v / tpr;
     const int lane = (int)(v % tpr);
     bool live = row < m;
-    if (FIX && live) live = *y.sign(row) == REDO;
+    const long long orow = live ? sp_out_row<FMT>(A, row) : 0;
+    if (FIX && live) live = *y.sign(orow) == REDO;
     Grp<N, L> g(C);
     MP<R> acc, p;
     g.zero(acc);
     bool ok = true;
-    if (live) {
-        const int re = irp[row + 1];
-        for (int i = irp[row] + lane; i < re && ok; i += tpr) ok = g.template fma_d_step<FULL>(acc, x, ja[i], as[i]);
-    }
+    if (live)
+        sp_terms<FMT>(A, m, row, lane, tpr, [&](int col, double v) {
+            ok = g.template fma_d_step<FULL>(acc, x, col, v);
+            return ok;
+        });
     if (tpr > 1) {
         for (int h = tpr >> 1; h >= 1; h >>= 1) {
             // publish partials of lanes h..2h-1, then lanes < h add them
@@ -1237,9 +1279,9 @@ __global__ __launch_bounds__(BLK) void k_spmv(const DevConsts *__restrict__ C, i
     }
     if (live && lane == 0) {
         if (ok)
-            store_aos(g, acc, y, row);
+            store_aos(g, acc, y, orow);
         else if (g.lane() == 0)
-            *y.sign(row) = REDO;
+            *y.sign(orow) = REDO;
     }
 }
 

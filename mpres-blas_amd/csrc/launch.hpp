@@ -402,17 +402,27 @@ struct Launch {
                            EL, BL, EH + m, BH + m, EL + m, BL + m, alpha, beta, Cm, ldc);
         return hipGetLastError();
     }
-    static hipError_t spmv(const DevConsts *c, int pass, int m, const int *irp, const int *ja, const double *as,
-                           const char *x, char *y, int tpr, hipStream_t st) {
-        if (m <= 0) return hipSuccess;
-        if (tpr > BLK / L) return hipErrorInvalidValue;
+    template <int FMT>
+    static void spmv_fmt(const DevConsts *c, int pass, int m, const SpMat &A, const char *x, char *y, int tpr,
+                         hipStream_t st) {
         const dim3 gr(grid_for((long long)m * tpr, L)), bl(BLK);
         if (pass == 0)
-            hipLaunchKernelGGL((k_spmv<N, L, !SPLIT, false>), gr, bl, 0, st, c, m, irp, ja, as, AoSRef<N>{(char *)x},
+            hipLaunchKernelGGL((k_spmv<N, L, !SPLIT, false, FMT>), gr, bl, 0, st, c, m, A, AoSRef<N>{(char *)x},
                                AoSRef<N>{y}, tpr);
         else if constexpr (SPLIT)
-            hipLaunchKernelGGL((k_spmv<N, L, true, true>), gr, bl, 0, st, c, m, irp, ja, as, AoSRef<N>{(char *)x},
+            hipLaunchKernelGGL((k_spmv<N, L, true, true, FMT>), gr, bl, 0, st, c, m, A, AoSRef<N>{(char *)x},
                                AoSRef<N>{y}, tpr);
+    }
+    static hipError_t spmv(const DevConsts *c, int pass, int fmt, int m, SpMat A, const char *x, char *y, int tpr,
+                           hipStream_t st) {
+        if (m <= 0) return hipSuccess;
+        if (tpr > BLK / L) return hipErrorInvalidValue;
+        switch (fmt) {
+            case SP_CSR: spmv_fmt<SP_CSR>(c, pass, m, A, x, y, tpr, st); break;
+            case SP_ELL: spmv_fmt<SP_ELL>(c, pass, m, A, x, y, tpr, st); break;
+            case SP_JAD: spmv_fmt<SP_JAD>(c, pass, m, A, x, y, tpr, st); break;
+            default: spmv_fmt<SP_DIA>(c, pass, m, A, x, y, tpr, st); break;
+        }
         return hipGetLastError();
     }
     // ---- batched form (kernels_batch.hpp): clear the queue counter, then the persistent kernel ----

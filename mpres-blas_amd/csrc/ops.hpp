@@ -14,6 +14,18 @@ struct ErMem {  // er_float_t in memory: {double frac; long exp}
 };
 
 // mp_array_t / mp_collection_t (structure of arrays), src/types.cuh:85-104
+// a sparse matrix in one of the storage formats of the SpMV kernels (kernels.hpp)
+enum { SP_CSR = 0, SP_ELL = 1, SP_JAD = 2, SP_DIA = 3 };
+struct SpMat {
+    const double *as;
+    const int *ja;      // CSR, ELL, JAD: column indices
+    const int *ptr;     // CSR: irp; JAD: jcp
+    const int *perm;    // JAD: row permutation
+    const int *offset;  // DIA: diagonal offsets
+    int width;          // ELL, JAD: maxnzr; DIA: ndiag
+    int ncols;          // DIA: columns of the matrix
+};
+
 struct SoA {
     int *d;
     int *s;
@@ -70,8 +82,8 @@ struct Ops {
     hipError_t (*gemm_mfma)(const DevConsts *, int, int, int, int, int, int, int, SoA, SoA, int, SoA, int, SoA, SoA,
                             int, char *, hipStream_t);
     size_t (*gemm_mfma_ws)(int, int, int, int);
-    hipError_t (*spmv)(const DevConsts *, int, int, const int *, const int *, const double *, const char *, char *,
-                       int, hipStream_t);
+    // SpMV over a storage format (SP_CSR, SP_ELL, SP_JAD, SP_DIA of kernels.hpp): pass 0 main, 1 fixup
+    hipError_t (*spmv)(const DevConsts *, int, int, int, SpMat, const char *, char *, int, hipStream_t);
     // batched form: reset the work queue, then the persistent kernel over `count` items on `grid` workgroups
     hipError_t (*batch)(const DevConsts *, const BatchItem *, int, int *, char *, int, hipStream_t);
 };

@@ -32,9 +32,28 @@ class MpArray(C.Structure):  # mp_array_t, src/types.cuh:85-92
                 ("buf", C.c_void_p), ("len", C.c_void_p)]
 
 
+class Jad(C.Structure):  # jad_t, src/types.cuh:118-123
+    _fields_ = [("as_", C.c_void_p), ("ja", C.c_void_p), ("jcp", C.c_void_p), ("perm", C.c_void_p)]
+
+
+class Ell(C.Structure):  # ell_t, src/types.cuh:128-131
+    _fields_ = [("as_", C.c_void_p), ("ja", C.c_void_p)]
+
+
+class Dia(C.Structure):  # dia_t, src/types.cuh:136-139
+    _fields_ = [("as_", C.c_void_p), ("offset", C.c_void_p)]
+
+
 class Csr(C.Structure):  # csr_t, src/types.cuh:109-113
     _fields_ = [("as_", C.c_void_p), ("ja", C.c_void_p), ("irp", C.c_void_p)]
 
+
+class Sparse(C.Structure):  # mpres_sparse_t: a matrix of one format for mpres_cg
+    _fields_ = [("format", C.c_int), ("maxnzr", C.c_int), ("ndiag", C.c_int), ("csr", Csr), ("ell", Ell),
+                ("jad", Jad), ("dia", Dia)]
+
+
+SPARSE_CSR, SPARSE_ELL, SPARSE_JAD, SPARSE_DIA = 0, 1, 2, 3
 
 BATCH_DOT, BATCH_GEMV = 0, 1
 
@@ -107,6 +126,22 @@ def lib():
         L.mpres_norm2_v2_workspace.restype = sz
         L.mpres_norm2_v2_workspace.argtypes = [vp, i32, i32, i32]
         L.mpres_norm2_v2.argtypes = [vp, i32, vp, i32, i32, C.POINTER(C.c_double), vp, sz, vp]
+        L.mpres_spmv_ell.argtypes = [vp, i32, i32, vp, vp, vp, vp]
+        L.mpres_spmv_jad.argtypes = [vp, i32, i32, vp, vp, vp, vp]
+        L.mpres_spmv_jadv.argtypes = [vp, i32, i32, vp, i32, vp, vp, vp]
+        L.mpres_spmv_dia.argtypes = [vp, i32, i32, i32, vp, vp, vp, vp]
+        L.mpres_cg.argtypes = [vp, i32, vp, vp, C.c_double, i32, vp, vp, C.POINTER(C.c_double), C.POINTER(i32), i32,
+                               i32, vp, sz, vp]
+        ip = C.POINTER(i32)
+        L.mpres_mtx_info.argtypes = [C.c_char_p] + [ip] * 6
+        L.mpres_mtx_nnz.argtypes = [i32, i32, i32]
+        L.mpres_mtx_ndiag.argtypes = [C.c_char_p, i32, i32, ip]
+        L.mpres_mtx_csr.argtypes = [C.c_char_p, i32, i32, i32, i32, vp, vp, vp]
+        L.mpres_mtx_ell.argtypes = [C.c_char_p, i32, i32, i32, i32, vp, vp]
+        L.mpres_mtx_jad.argtypes = [C.c_char_p, i32, i32, i32, i32, i32, vp, vp, vp, vp]
+        L.mpres_mtx_dia.argtypes = [C.c_char_p, i32, i32, i32, i32, vp, vp]
+        L.mpres_mtx_dense.argtypes = [C.c_char_p, i32, i32, i32, i32, vp]
+        L.mpres_read_vector.argtypes = [C.c_char_p, i32, vp]
         L.mpres_cg_workspace.restype = sz
         L.mpres_cg_workspace.argtypes = [vp, i32, i32, i32]
         L.mpres_cg_csr.argtypes = [vp, i32, vp, vp, C.c_double, i32, vp, vp, C.POINTER(C.c_double),

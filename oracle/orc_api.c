@@ -275,3 +275,10 @@ void orc_sqrt(const orc_ctx *c, int count, const void *x, void *r) { DISPATCH_E(
 #define B_DSQRT(s)                                                                         \
     for (int i = 0; i < count; i++) out[i] = G(dsqrt, s)(c, &((const G(mpf, s) *)x)[i]);
 void orc_dsqrt(const orc_ctx *c, int count, const void *x, double *out) { DISPATCH_E(c, B_DSQRT); }
+
+/* ELL / JAD / DIA SpMV */
+#define B_SPFMT(s) G(spmv_fmt, s)(c, fmt, m, w, ncols, as, ja, jcp, perm, offset, (const G(mpf, s) *)x, (G(mpf, s) *)y, tpr);
+void orc_spmv_fmt(const orc_ctx *c, int policy, int fmt, int m, int w, int ncols, const double *as, const int *ja,
+                  const int *jcp, const int *perm, const int *offset, const void *x, void *y, int tpr) {
+    DISPATCH(c, policy, B_SPFMT);
+}

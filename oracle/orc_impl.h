@@ -1051,6 +1051,47 @@ static void F(spmv)(const orc_ctx *c, int m, const int *irp, const int *ja, cons
     }
 }
 
+/* the ELLPACK, jagged-diagonal and diagonal kernels (src/sparse/spmv/spmv_ell.cuh:50-66,
+ * spmv_jad.cuh:50-66, spmv_jadv.cuh:50-87, spmv_dia.cuh:50-66): per row, the lane-strided
+ * terms in the stored order from zero, then the 16/8/4/2/1 tree (tpr > 1: jadv only).
+ * fmt 1 ELL (as, ja: m x w column-major; zero values skipped), 2 JAD (jcp, perm; the row's
+ * result goes to y[perm[row]]), 3 DIA (offset; w diagonals, ncols columns) */
+static void F(spmv_fmt)(const orc_ctx *c, int fmt, int m, int w, int ncols, const double *as, const int *ja,
+                        const int *jcp, const int *perm, const int *offset, const MPF *x, MPF *y, int tpr) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int row = 0; row < m; row++) {
+        MPF s[32], p;
+        for (int l = 0; l < tpr; l++) {
+            s[l] = F(MP_ZERO);
+            for (int t = l; t < w; t += tpr) {
+                int col;
+                double v;
+                if (fmt == 1) {
+                    col = ja[(long)t * m + row];
+                    v = as[(long)t * m + row];
+                    if (v == 0) continue;
+                } else if (fmt == 2) {
+                    long q = (long)row + jcp[t];
+                    if (q >= jcp[t + 1]) continue;
+                    col = ja[q];
+                    v = as[q];
+                } else {
+                    long cc = (long)row + offset[t];
+                    v = as[(long)m * t + row];
+                    if (cc < 0 || cc >= ncols || v == 0) continue;
+                    col = (int)cc;
+                }
+                F(mul_d)(c, &p, x[col], v);
+                F(add)(c, &s[l], s[l], p);
+            }
+        }
+        for (int h = 16; h >= 1; h >>= 1)
+            if (tpr >= 2 * h)
+                for (int l = 0; l < h; l++) F(add)(c, &s[l], s[l], s[l + h]);
+        y[fmt == 2 ? perm[row] : row] = s[0];
+    }
+}
+
 #undef MPF
 #undef N
 #undef F

@@ -290,13 +290,34 @@ class Oracle:
         """cuda::mp_norm2<blocks, threads> (src/blas/v2/norm2_v2.cuh:107-119)"""
         return float(self.dsqrt(self.sumsq(x, blocks, threads, policy))[0])
 
+    def spmv_ell(self, m, maxnzr, as_, ja, x, policy=POLICY_D):
+        return self._spmv_fmt(1, m, maxnzr, 0, as_, ja, None, None, None, x, 1, policy)
+
+    def spmv_jad(self, m, maxnzr, as_, ja, jcp, perm, x, tpr=1, policy=POLICY_D):
+        return self._spmv_fmt(2, m, maxnzr, 0, as_, ja, jcp, perm, None, x, tpr, policy)
+
+    def spmv_dia(self, m, n, ndiag, as_, offset, x, policy=POLICY_D):
+        return self._spmv_fmt(3, m, ndiag, n, as_, None, None, None, offset, x, 1, policy)
+
+    def _spmv_fmt(self, fmt, m, w, ncols, as_, ja, jcp, perm, offset, x, tpr, policy):
+        y = self.empty(m)
+        arr = [None if a is None else np.ascontiguousarray(a, t) for a, t in
+               ((as_, np.float64), (ja, np.int32), (jcp, np.int32), (perm, np.int32), (offset, np.int32))]
+        self.L.orc_spmv_fmt(self.ctx, policy, fmt, m, w, ncols, *[_p(a) for a in arr], _p(np.ascontiguousarray(x)),
+                            _p(y), tpr)
+        return y
+
     def cg_csr(self, n, irp, ja, as_, b, tol, maxit, x, threads=32, blocks_reduce=256, policy=POLICY_D,
-               precond=None):
+               precond=None, spmv=None):
         """mp_cg_csr<threads, blocks_reduce> (src/sparse/solver/cg_csr.cuh:52-117) or, with precond (the
         reciprocals of A's diagonal), mp_pcg_csr with the diagonal preconditioner (pcg_csr.cuh:52-130),
-        composed from the oracle's operations in the reference's order. Returns (x, resvec)."""
+        composed from the oracle's operations in the reference's order; spmv (v -> A v) replaces the
+        CSR product for the solvers of the other formats (cg_ell.cuh, cg_jad.cuh, cg_dia.cuh).
+        Returns (x, resvec)."""
         x = np.array(x, copy=True)
-        r = self.spmv(n, irp, ja, as_, x, 1, policy)
+        if spmv is None:
+            spmv = lambda v: self.spmv(n, irp, ja, as_, v, 1, policy)  # noqa: E731
+        r = spmv(x)
         r = self.sub(b, r, policy)
         norm0 = self.norm2(r, blocks_reduce, threads, policy)
         eps = norm0 * tol
@@ -314,7 +335,7 @@ class Oracle:
             else:
                 beta = self.div(rho, rhop)
                 p = self.axpy_v2(beta, p, z, policy)
-            q = self.spmv(n, irp, ja, as_, p, 1, policy)
+            q = spmv(p)
             pq = self.dot(p, q, blocks_reduce, threads, policy)
             alpha = self.div(rho, pq)
             x = self.axpy_v2(alpha, p, x, policy)

@@ -390,6 +390,128 @@ def test_cg_laplacian():
     _cg_case("16_4double", n, irp, ja, as_, b, 1e-40, 30, None, 64, 128)
 
 
+def _mtx_all(path):
+    """the library's Matrix Market reader: CSR, ELL, JAD and DIA host arrays of one file"""
+    L = mp.lib()
+    P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    v = [C.c_int() for _ in range(6)]
+    mp.check(L.mpres_mtx_info(path.encode(), *[C.byref(q) for q in v]))
+    m, n, lines, maxnzr, nzmd, sym = (q.value for q in v)
+    nnz = L.mpres_mtx_nnz(lines, nzmd, sym)
+    csr = (np.zeros(nnz), np.zeros(nnz, np.int32), np.zeros(m + 1, np.int32))
+    mp.check(L.mpres_mtx_csr(path.encode(), m, nnz, lines, sym, *[P(a) for a in csr]))
+    ell = (np.zeros(m * maxnzr), np.zeros(m * maxnzr, np.int32))
+    mp.check(L.mpres_mtx_ell(path.encode(), m, maxnzr, lines, sym, *[P(a) for a in ell]))
+    jad = (np.zeros(nnz), np.zeros(nnz, np.int32), np.zeros(maxnzr + 1, np.int32), np.zeros(m, np.int32))
+    mp.check(L.mpres_mtx_jad(path.encode(), m, maxnzr, nnz, lines, sym, *[P(a) for a in jad]))
+    nd = C.c_int()
+    mp.check(L.mpres_mtx_ndiag(path.encode(), lines, sym, C.byref(nd)))
+    dia = (np.zeros(m * nd.value), np.zeros(nd.value, np.int32))
+    mp.check(L.mpres_mtx_dia(path.encode(), m, lines, sym, nd.value, *[P(a) for a in dia]))
+    return dict(m=m, n=n, maxnzr=maxnzr, ndiag=nd.value, csr=csr, ell=ell, jad=jad, dia=dia)
+
+
+class _Dev:
+    """device copies of a format's host arrays and the struct that points at them"""
+
+    def __init__(self, kind, arrays):
+        self.bufs = [mp.DevBuf(host=a) for a in arrays]
+        ptrs = [b.ptr.value for b in self.bufs]
+        self.s = {"csr": mp.Csr, "ell": mp.Ell, "jad": mp.Jad, "dia": mp.Dia}[kind](*ptrs)
+
+
+def _spmv_all(c, o, S, x, tprs=(2, 8, 32)):
+    """every format's GPU SpMV against the oracle on the same arrays; returns the device structs"""
+    m, w, nd = S["m"], S["maxnzr"], S["ndiag"]
+    dx, dy = c.aos(x), c.aos(count=m)
+    dev = {k: _Dev(k, S[k]) for k in ("csr", "ell", "jad", "dia")}
+    mp.check(c.L.mpres_spmv_ell(c.h, m, w, C.byref(dev["ell"].s), dx.ptr, dy.ptr, None))
+    same(c.aos_to_host(dy, m), o.spmv_ell(m, w, *S["ell"], x, D), "ell")
+    mp.check(c.L.mpres_spmv_jad(c.h, m, w, C.byref(dev["jad"].s), dx.ptr, dy.ptr, None))
+    same(c.aos_to_host(dy, m), o.spmv_jad(m, w, *S["jad"], x, 1, D), "jad")
+    for tpr in tprs:
+        mp.check(c.L.mpres_spmv_jadv(c.h, m, w, C.byref(dev["jad"].s), tpr, dx.ptr, dy.ptr, None))
+        same(c.aos_to_host(dy, m), o.spmv_jad(m, w, *S["jad"], x, tpr, D), f"jadv tpr={tpr}")
+    if nd:
+        mp.check(c.L.mpres_spmv_dia(c.h, m, S["n"], nd, C.byref(dev["dia"].s), dx.ptr, dy.ptr, None))
+        same(c.aos_to_host(dy, m), o.spmv_dia(m, S["n"], nd, *S["dia"], x, D), "dia")
+    return dev
+
+
+@pytest.mark.parametrize("name", ["8_2double", "16_4double", "32_8double"])
+def test_spmv_formats_lf10(name):
+    """ELL / JAD / JADV / DIA on the reference's LF10 file through the library's reader"""
+    c, o = ctx(name), orc(name)
+    S = _mtx_all(os.path.join(ROOT, "tests", "golden", "LF10.mtx"))
+    note("spmv formats lf10", name)
+    _spmv_all(c, o, S, o.set_d(gen.uexp(101, S["m"], -20, 20), D))
+
+
+@pytest.mark.parametrize("name", ["8_2double", "16_4double", "32_8double"])
+def test_spmv_formats_large(name):
+    """20000 power-law rows in ELL and JAD (rows with exactly cancelling pairs take the fixup
+    path), and a banded 20000-row matrix in DIA"""
+    c, o = ctx(name), orc(name)
+    m = 20000
+    irp, ja, as_ = gen.powerlaw_csr(102, m, kmax=64)
+    xd = gen.uexp(103, m, -30, 30)
+    for i in range(0, m, 97):  # a pair of equal x entries with opposite coefficients in the row
+        if irp[i + 1] - irp[i] >= 2:
+            a, b = ja[irp[i]], ja[irp[i] + 1]
+            xd[b] = xd[a]
+            as_[irp[i] + 1] = -as_[irp[i]]
+    x = o.set_d(xd, D)
+    lens = np.diff(irp)
+    w = int(lens.max())
+    eas, eja = np.zeros(m * w), np.full(m * w, -1, np.int32)
+    for i in range(m):
+        eas[np.arange(lens[i]) * m + i] = as_[irp[i]:irp[i + 1]]
+        eja[np.arange(lens[i]) * m + i] = ja[irp[i]:irp[i + 1]]
+    perm = np.argsort(-lens, kind="stable").astype(np.int32)
+    jcp = np.concatenate([[0], np.cumsum([(lens > j).sum() for j in range(w)])]).astype(np.int32)
+    jas = np.concatenate([as_[irp[perm[:jcp[j + 1] - jcp[j]]] + j] for j in range(w)])
+    jja = np.concatenate([ja[irp[perm[:jcp[j + 1] - jcp[j]]] + j] for j in range(w)]).astype(np.int32)
+    offs = np.arange(-8, 9, dtype=np.int32)
+    das = gen.uexp(104, m * len(offs), 0, 0) * (np.arange(m * len(offs)) % 5 != 0)  # some zeros
+    S = dict(m=m, n=m, maxnzr=w, ndiag=len(offs), csr=(as_, ja, irp), ell=(eas, eja), jad=(jas, jja, jcp, perm),
+             dia=(das, offs))
+    note("spmv formats large", name, m)
+    _spmv_all(c, o, S, x)
+
+
+@pytest.mark.parametrize("name", ["8_2double", "32_8double"])
+@pytest.mark.parametrize("fmt", ["ell", "jad", "dia"])
+def test_cg_formats_lf10(name, fmt):
+    """mp_cg_ell / mp_cg_jad / mp_cg_dia (and the diagonal PCG) on LF10 through the reader,
+    every iterate bit-exact against the oracle's composition over the same format"""
+    c, o = ctx(name), orc(name)
+    S = _mtx_all(os.path.join(ROOT, "tests", "golden", "LF10.mtx"))
+    n = S["m"]
+    irp, ja, as_ = S["csr"][2], S["csr"][1], S["csr"][0]
+    b = o.spmv(n, irp, ja, as_, o.set_d(gen.uexp(105, n, -2, 2), D), 1, D)
+    diag = np.array([as_[irp[i] + list(ja[irp[i]:irp[i + 1]]).index(i)] for i in range(n)])
+    spmv = {"ell": lambda v: o.spmv_ell(n, S["maxnzr"], *S["ell"], v, D),
+            "jad": lambda v: o.spmv_jad(n, S["maxnzr"], *S["jad"], v, 1, D),
+            "dia": lambda v: o.spmv_dia(n, n, S["ndiag"], *S["dia"], v, D)}[fmt]
+    dev = _Dev(fmt, S[fmt])
+    A = mp.Sparse()
+    A.format = {"ell": mp.SPARSE_ELL, "jad": mp.SPARSE_JAD, "dia": mp.SPARSE_DIA}[fmt]
+    A.maxnzr, A.ndiag = S["maxnzr"], S["ndiag"]
+    setattr(A, fmt, dev.s)
+    tol = 1e-25 if name == "8_2double" else 1e-90
+    for precond in (None, 1.0 / diag):
+        note("cg formats", name, fmt, precond is not None)
+        x0 = o.set_d(np.zeros(n), D)
+        xw, rw = o.cg_csr(n, None, None, None, b, tol, 300, x0, precond=precond, spmv=spmv)
+        db, dxx = c.aos(b), c.aos(x0)
+        dm = mp.DevBuf(host=precond) if precond is not None else None
+        res, it = (C.c_double * 301)(), C.c_int()
+        mp.check(c.L.mpres_cg(c.h, n, C.byref(A), db.ptr, tol, 300, dxx.ptr, dm.ptr if dm else None, res,
+                              C.byref(it), 32, 256, None, 0, None))
+        assert it.value == len(rw) - 1 and list(res[:it.value + 1]) == rw
+        same(c.aos_to_host(dxx, n), xw, f"cg {fmt} {name}")
+
+
 def test_pool_two_streams():
     """The context's workspace pool shared by calls on two streams: a call on another stream
     waits for the pool's last use, and the pool grows only after that use has ended, so
