@@ -87,6 +87,13 @@ MPRES_STATIC_ASSERT(sizeof(dia_t) == 16 && offsetof(dia_t, offset) == 8, "dia_t 
 #define MPRES_NO_TRANS 111
 #define MPRES_TRANS 112
 #define MPRES_CONJ_TRANS 113
+/* mblas_uplo_type, mblas_side_type, mblas_norm_type: src/blas/mblas_enum.cuh:31-44 */
+#define MPRES_UPPER 121
+#define MPRES_LOWER 122
+#define MPRES_LEFT 141
+#define MPRES_RIGHT 142
+#define MPRES_ONE_NORM 171
+#define MPRES_INF_NORM 175
 
 typedef struct mpres_ctx mpres_ctx_t;
 
@@ -227,6 +234,64 @@ int mpres_gemm_mfma(mpres_ctx_t *ctx, int transa, int transb, int m, int n, int 
                     const mp_array_t *A, int lda, const mp_array_t *B, int ldb, const mp_array_t *beta, mp_array_t *C,
                     int ldc, void *ws, size_t ws_bytes, void *stream);
 
+/* ---- the rest of the v1 BLAS. Each reference element kernel triple (esi + digits + round)
+ * is one pass here, run in the reference's order with the reference's caller buffers, so the
+ * results and every intermediate the reference leaves in a buffer are the reference's. The
+ * reference's silent returns on invalid increments or leading dimensions are MPRES_ERR_ARG. */
+/* x = alpha x: cuda::mp_scal, src/blas/scal.cuh:43-61 (returns if n <= 0 or incx <= 0) */
+int mpres_scal(mpres_ctx_t *ctx, int n, const mp_array_t *alpha, mp_array_t *x, int incx, void *stream);
+/* plane rotation: cuda::mp_rot, src/blas/rot.cuh:49-100. buffer1 = s x, buffer2 = s y, then
+ * x = c x and y = c y over the first n elements with increment 1 (the reference calls mp_scal
+ * with increment 1 whatever incx and incy are), then x = x + buffer2, y = y - buffer1 with
+ * incx, incy. buffer1, buffer2: n numbers each. */
+int mpres_rot(mpres_ctx_t *ctx, int n, mp_array_t *x, int incx, mp_array_t *y, int incy, const mp_array_t *c,
+              const mp_array_t *s, mp_array_t *buffer1, mp_array_t *buffer2, void *stream);
+/* w = alpha x + beta y: cuda::mp_waxpby, src/blas/waxpby.cuh:51-89 (buffer: n numbers) */
+int mpres_waxpby(mpres_ctx_t *ctx, int n, const mp_array_t *alpha, const mp_array_t *x, int incx,
+                 const mp_array_t *beta, const mp_array_t *y, int incy, mp_array_t *w, int incw, mp_array_t *buffer,
+                 void *stream);
+/* w = w - alpha v, r = u . w: cuda::mp_axpy_dot, src/blas/axpydot.cuh:54-86. buffer: n numbers,
+ * holds alpha v on return (the reference's dot then overwrites it with u .* w); the dot is
+ * mpres_dot with (grid3, block3) and its workspace. */
+int mpres_axpy_dot(mpres_ctx_t *ctx, int n, const mp_array_t *alpha, mp_array_t *w, int incw, const mp_array_t *v,
+                   int incv, const mp_array_t *u, int incu, mp_array_t *r, mp_array_t *buffer, int grid3, int block3,
+                   void *ws, size_t ws_bytes, void *stream);
+/* r = sum |x_i|: cuda::mp_asum<gridDim1, blockDim1>, src/blas/asum.cuh:41-70 (the reduction
+ * kernels src/mpreduct.cuh:120-192; the sum order is fixed by (grid1, block1)) */
+size_t mpres_asum_workspace(const mpres_ctx_t *ctx, int n, int grid1, int block1);
+int mpres_asum(mpres_ctx_t *ctx, int n, const mp_array_t *x, int incx, mp_array_t *r, int grid1, int block1, void *ws,
+               size_t ws_bytes, void *stream);
+/* r = ||x||_1 (mpres_asum) or max |x_i| (MPRES_INF_NORM): cuda::mp_norm<gridDim1, blockDim1>,
+ * src/blas/norm.cuh:43-79 (src/mpreduct.cuh:199-286). Workspace: mpres_asum_workspace. */
+int mpres_norm(mpres_ctx_t *ctx, int norm, int n, const mp_array_t *x, int incx, mp_array_t *r, int grid1, int block1,
+               void *ws, size_t ws_bytes, void *stream);
+/* C = alpha A + beta B: cuda::mp_ge_add, src/blas/geadd.cuh:58-114 (buffer: m n numbers, ld m) */
+int mpres_ge_add(mpres_ctx_t *ctx, int m, int n, const mp_array_t *alpha, const mp_array_t *A, int lda,
+                 const mp_array_t *beta, const mp_array_t *B, int ldb, mp_array_t *C, int ldc, mp_array_t *buffer,
+                 void *stream);
+/* B = alpha A + beta B: cuda::mp_ge_acc, src/blas/geacc.cuh:57-113 (buffer: m n numbers) */
+int mpres_ge_acc(mpres_ctx_t *ctx, int m, int n, const mp_array_t *alpha, const mp_array_t *A, int lda,
+                 const mp_array_t *beta, mp_array_t *B, int ldb, mp_array_t *buffer, void *stream);
+/* A = D A (MPRES_LEFT) or A D (MPRES_RIGHT), D diagonal stored as a vector:
+ * cuda::mp_ge_diag_scale, src/blas/gediagscale.cuh:54-91 */
+int mpres_ge_diag_scale(mpres_ctx_t *ctx, int side, int m, int n, const mp_array_t *D, int incd, mp_array_t *A,
+                        int lda, void *stream);
+/* A = DL A DR: cuda::mp_ge_lr_scale, src/blas/gelrscale.cuh:56-93 (left scaling first) */
+int mpres_ge_lr_scale(mpres_ctx_t *ctx, int m, int n, const mp_array_t *DL, int incdl, const mp_array_t *DR,
+                      int incdr, mp_array_t *A, int lda, void *stream);
+/* r = ||A||_1 (largest column sum of |a|) or ||A||_inf (largest row sum): cuda::mp_ge_norm
+ * <gridDim1, blockDim1>, src/blas/genorm.cuh:142-173. The column (row) sums land in buffer
+ * (n or m numbers) as in the reference; block1 fixes their sum order, (grid1, block1) the
+ * maximum's. */
+size_t mpres_ge_norm_workspace(const mpres_ctx_t *ctx, int norm, int m, int n, int grid1, int block1);
+int mpres_ge_norm(mpres_ctx_t *ctx, int norm, int m, int n, const mp_array_t *A, int lda, mp_array_t *r,
+                  mp_array_t *buffer, int grid1, int block1, void *ws, size_t ws_bytes, void *stream);
+/* A = A + alpha x y^T: cuda::mp_ger, src/blas/ger.cuh:155-214. buffer1 = alpha y (n numbers),
+ * buffer2 = x buffer1^T (m n numbers, ld m, its len must be m n as the reference's), A += buffer2. */
+int mpres_ger(mpres_ctx_t *ctx, int m, int n, const mp_array_t *alpha, const mp_array_t *x, int incx,
+              const mp_array_t *y, int incy, mp_array_t *A, int lda, mp_array_t *buffer1, mp_array_t *buffer2,
+              void *stream);
+
 /* ---- BLAS, v2 interface over mp_float_t arrays (src/blas/v2) ---- */
 /* cuda::mp_dot<blocks,threads>(n,x,y,r), src/blas/v2/dot_v2.cuh:121-129 */
 size_t mpres_dot_v2_workspace(const mpres_ctx_t *ctx, int n, int blocks, int threads);
@@ -240,6 +305,30 @@ int mpres_maxpy_v2(mpres_ctx_t *ctx, int n, const void *alpha, const void *x, co
 int mpres_diff_v2(mpres_ctx_t *ctx, int n, const void *x, const void *y, void *r, void *stream);
 /* r[i] = x[i] * d[i] (d: device doubles): cuda::mp_prod_d, src/blas/v2/prod_v2.cuh */
 int mpres_prod_d_v2(mpres_ctx_t *ctx, int n, const void *x, const double *d, void *r, void *stream);
+/* r[i] = alpha x[i]: cuda::mp_scal (v2), src/blas/v2/scal_v2.cuh:37-48 */
+int mpres_scal_v2(mpres_ctx_t *ctx, int n, const void *alpha, const void *x, void *r, void *stream);
+/* r[i] = x[i] y[i]: cuda::mp_prod, src/blas/v2/prod_v2.cuh:37-43 */
+int mpres_prod_v2(mpres_ctx_t *ctx, int n, const void *x, const void *y, void *r, void *stream);
+/* The v2 matrix kernels below stride row and column together, so the reference touches every
+ * element only when its launch covers the matrix; these functions always cover it. */
+/* A = A + (alpha x) y^T: cuda::mp_ger (v2), src/blas/v2/ger_v2.cuh:49-68 */
+int mpres_ger_v2(mpres_ctx_t *ctx, int m, int n, const void *alpha, const void *x, int incx, const void *y, int incy,
+                 void *A, int lda, void *stream);
+/* A = A + (x x^T) alpha on the uplo triangle: cuda::mp_syr, src/blas/v2/syr_v2.cuh:58-76 */
+int mpres_syr_v2(mpres_ctx_t *ctx, int uplo, int n, const void *alpha, const void *x, int incx, void *A, int lda,
+                 void *stream);
+/* A = A + ((x y^T) alpha + (y x^T) alpha) on the uplo triangle: cuda::mp_syr2,
+ * src/blas/v2/syr2_v2.cuh:59-83 */
+int mpres_syr2_v2(mpres_ctx_t *ctx, int uplo, int n, const void *alpha, const void *x, int incx, const void *y,
+                  int incy, void *A, int lda, void *stream);
+/* y = beta y + alpha (A x), A symmetric from its uplo triangle: cuda::mp_symv<threads>,
+ * src/blas/v2/symv_v2.cuh:57-95 (each row summed in column order) */
+int mpres_symv_v2(mpres_ctx_t *ctx, int uplo, int n, const void *alpha, const void *A, int lda, const void *x,
+                  int incx, const void *beta, void *y, int incy, void *stream);
+/* C = (op(A) op(A)^T) alpha + C beta on the uplo triangle: cuda::mp_syrk,
+ * src/blas/v2/syrk_v2.cuh:61-85 (only MPRES_TRANS reads A transposed, as the reference) */
+int mpres_syrk_v2(mpres_ctx_t *ctx, int uplo, int trans, int n, int k, const void *alpha, const void *A, int lda,
+                  const void *beta, void *C, int ldc, void *stream);
 /* *result (host) = ||x||_2: cuda::mp_norm2<blocks, threads>, src/blas/v2/norm2_v2.cuh:107-119
  * (sum of squares in the order of its two reduction kernels, then mp_dsqrt). Synchronizes
  * the stream. */

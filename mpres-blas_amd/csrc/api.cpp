@@ -1168,6 +1168,312 @@ int mpres_batch_destroy(mpres_batch_t *b) {
     return MPRES_OK;
 }
 
+// ---------------------------------------------------------------- the rest of the v1 BLAS
+// Each reference element kernel triple (esi, digits, round) is one k_v1map pass here; the
+// passes run in the reference's order, so every intermediate the reference writes to a
+// caller buffer is written there too.
+static int v1map(mpres_ctx_t *c, int op, int sf, int m, int n, const mp_array_t *S, int incs, const mp_array_t *X,
+                 int ldx, int incx, const mp_array_t *Y, int ldy, int incy, mp_array_t *W, int ldw, int incw,
+                 hipStream_t st) {
+    HIPCHK(c->ops[OP_ELEM]->v1map(c->dev, op, sf, m, n, S ? soa_of(S) : SoA{}, incs, soa_of(X), ldx, incx,
+                                  Y ? soa_of(Y) : SoA{}, ldy, incy, soa_of(W), ldw, incw, st));
+    return MPRES_OK;
+}
+#define V1(...)                            \
+    do {                                   \
+        const int rc_ = v1map(__VA_ARGS__); \
+        if (rc_) return rc_;               \
+    } while (0)
+
+int mpres_scal(mpres_ctx_t *c, int n, const mp_array_t *alpha, mp_array_t *x, int incx, void *st) {
+    if (!c || !alpha || !x) return fail(MPRES_ERR_ARG, "null argument");
+    if (n <= 0 || incx <= 0) return MPRES_OK;  // src/blas/scal.cuh:46-48
+    V1(c, V1_MUL, SF_SCALAR, n, 1, alpha, 1, x, 0, incx, nullptr, 0, 0, x, 0, incx, strm(st));
+    return MPRES_OK;
+}
+
+int mpres_rot(mpres_ctx_t *c, int n, mp_array_t *x, int incx, mp_array_t *y, int incy, const mp_array_t *cc,
+              const mp_array_t *s, mp_array_t *buffer1, mp_array_t *buffer2, void *st) {
+    if (!c || !x || !y || !cc || !s || !buffer1 || !buffer2) return fail(MPRES_ERR_ARG, "null argument");
+    if (n <= 0) return MPRES_OK;  // src/blas/rot.cuh:52-54
+    const hipStream_t q = strm(st);
+    V1(c, V1_MUL, SF_SCALAR, n, 1, s, 1, x, 0, incx, nullptr, 0, 0, buffer1, 0, 1, q);  // buffer1 = s x
+    V1(c, V1_MUL, SF_SCALAR, n, 1, s, 1, y, 0, incy, nullptr, 0, 0, buffer2, 0, 1, q);  // buffer2 = s y
+    // x = c x and y = c y with increment 1 whatever incx and incy are (src/blas/rot.cuh:78-81 call mp_scal so)
+    V1(c, V1_MUL, SF_SCALAR, n, 1, cc, 1, x, 0, 1, nullptr, 0, 0, x, 0, 1, q);
+    V1(c, V1_MUL, SF_SCALAR, n, 1, cc, 1, y, 0, 1, nullptr, 0, 0, y, 0, 1, q);
+    V1(c, V1_ADD, 0, n, 1, nullptr, 0, x, 0, incx, buffer2, 0, 1, x, 0, incx, q);  // x = x + buffer2
+    V1(c, V1_SUB, 0, n, 1, nullptr, 0, y, 0, incy, buffer1, 0, 1, y, 0, incy, q);  // y = y - buffer1
+    return MPRES_OK;
+}
+
+int mpres_waxpby(mpres_ctx_t *c, int n, const mp_array_t *alpha, const mp_array_t *x, int incx,
+                 const mp_array_t *beta, const mp_array_t *y, int incy, mp_array_t *w, int incw, mp_array_t *buffer,
+                 void *st) {
+    if (!c || !alpha || !x || !beta || !y || !w || !buffer) return fail(MPRES_ERR_ARG, "null argument");
+    if (n <= 0) return MPRES_OK;  // src/blas/waxpby.cuh:54-56
+    const hipStream_t q = strm(st);
+    V1(c, V1_MUL, SF_SCALAR, n, 1, alpha, 1, x, 0, incx, nullptr, 0, 0, buffer, 0, 1, q);  // buffer = alpha x
+    V1(c, V1_MUL, SF_SCALAR, n, 1, beta, 1, y, 0, incy, nullptr, 0, 0, w, 0, incw, q);     // w = beta y
+    V1(c, V1_ADD, 0, n, 1, nullptr, 0, w, 0, incw, buffer, 0, 1, w, 0, incw, q);           // w = w + buffer
+    return MPRES_OK;
+}
+
+int mpres_axpy_dot(mpres_ctx_t *c, int n, const mp_array_t *alpha, mp_array_t *w, int incw, const mp_array_t *v,
+                   int incv, const mp_array_t *u, int incu, mp_array_t *r, mp_array_t *buffer, int g3, int b3,
+                   void *ws, size_t wsb, void *st) {
+    if (!c || !alpha || !w || !v || !u || !r || !buffer) return fail(MPRES_ERR_ARG, "null argument");
+    if (n <= 0) return MPRES_OK;  // src/blas/axpydot.cuh:57-59
+    if (g3 <= 0 || b3 <= 0) return fail(MPRES_ERR_ARG, "invalid reduction shape %d x %d", g3, b3);
+    const hipStream_t q = strm(st);
+    V1(c, V1_MUL, SF_SCALAR, n, 1, alpha, 1, v, 0, incv, nullptr, 0, 0, buffer, 0, 1, q);  // buffer = alpha v
+    V1(c, V1_SUB, 0, n, 1, nullptr, 0, w, 0, incw, buffer, 0, 1, w, 0, incw, q);           // w = w - buffer
+    return mpres_dot(c, n, u, incu, w, incw, r, g3, b3, ws, wsb, st);                       // r = u . w
+}
+
+// ---- sum and maximum of absolute values: the fold over grid1 x block1 virtual threads,
+// the tree per block, then one block of block1 over the grid1 block results
+size_t mpres_asum_workspace(const mpres_ctx_t *c, int n, int g1, int b1) { return mpres_dot_workspace(c, n, g1, b1); }
+
+// maximum |x_i| of an SoA vector into *res (AoS, sign 0); w holds (g1 b1 + g1 + b1 + 1) numbers
+static int maxabs_into(mpres_ctx_t *c, int n, SoA x, int incx, int g1, int b1, char *w, hipStream_t st, char **res) {
+    const size_t S = mpsize(c);
+    const long long G = (long long)g1 * b1;
+    char *p1 = w, *p2 = p1 + S * G, *p3 = p2 + S * g1, *p4 = p3 + S * b1;
+    const Ops *o = c->ops[OP_DOT];
+    const unsigned P = next_pow2((unsigned)b1);
+    // src/mpreduct.cuh:199-238 over x, then :246-286 over the block results
+    HIPCHK(o->maxabs(c->dev, 0, n, G, x, incx, nullptr, p1, b1, P, g1, p2, st));
+    HIPCHK(o->maxabs(c->dev, 1, g1, b1, SoA{}, 1, p2, p3, b1, P, 1, p4, st));
+    *res = p4;
+    return MPRES_OK;
+}
+
+int mpres_asum(mpres_ctx_t *c, int n, const mp_array_t *x, int incx, mp_array_t *r, int g1, int b1, void *ws,
+               size_t wsb, void *st) {
+    if (!c || !x || !r) return fail(MPRES_ERR_ARG, "null argument");
+    if (n <= 0 || incx <= 0) return MPRES_OK;  // src/blas/asum.cuh:42-44
+    if (g1 <= 0 || b1 <= 0) return fail(MPRES_ERR_ARG, "invalid reduction shape %d x %d", g1, b1);
+    const size_t S = mpsize(c);
+    char *w;
+    WsUse wu(c, strm(st));
+    int rc = wu.get(ws, wsb, mpres_asum_workspace(c, n, g1, b1), &w);
+    if (rc) return rc;
+    const long long G = (long long)g1 * b1;
+    char *p1 = w, *p2 = p1 + S * G, *p3 = p2 + S * g1, *p4 = p3 + S * b1;
+    const Ops *o = c->ops[OP_DOT];
+    const unsigned P = next_pow2((unsigned)b1);
+    // src/mpreduct.cuh:120-153: |x_i| summed by mp_add_abs, which on the nonnegative terms and
+    // partial sums is mp_add; :161-192 over the block results
+    HIPCHK(o->fold_abs(c->dev, n, G, soa_of(x), incx, p1, strm(st)));
+    HIPCHK(o->tree(c->dev, p1, b1, P, g1, p2, strm(st)));
+    SoA none{};
+    HIPCHK(o->fold(c->dev, 2, 0, g1, b1, none, 1, none, 1, p2, nullptr, p3, strm(st)));
+    HIPCHK(o->tree(c->dev, p3, b1, P, 1, p4, strm(st)));
+    HIPCHK(o->aos_to_soa(c->dev, p4, soa_of(r), 1, strm(st)));
+    return MPRES_OK;
+}
+
+int mpres_norm(mpres_ctx_t *c, int norm, int n, const mp_array_t *x, int incx, mp_array_t *r, int g1, int b1,
+               void *ws, size_t wsb, void *st) {
+    if (norm == MPRES_ONE_NORM) return mpres_asum(c, n, x, incx, r, g1, b1, ws, wsb, st);  // src/blas/norm.cuh:44-47
+    if (!c || !x || !r) return fail(MPRES_ERR_ARG, "null argument");
+    if (norm != MPRES_INF_NORM) return fail(MPRES_ERR_ARG, "invalid norm %d", norm);
+    if (n <= 0 || incx <= 0) return MPRES_OK;  // :49-51
+    if (g1 <= 0 || b1 <= 0) return fail(MPRES_ERR_ARG, "invalid reduction shape %d x %d", g1, b1);
+    char *w, *res;
+    WsUse wu(c, strm(st));
+    int rc = wu.get(ws, wsb, mpres_asum_workspace(c, n, g1, b1), &w);
+    if (rc) return rc;
+    rc = maxabs_into(c, n, soa_of(x), incx, g1, b1, w, strm(st), &res);
+    if (rc) return rc;
+    HIPCHK(c->ops[OP_DOT]->aos_to_soa(c->dev, res, soa_of(r), 1, strm(st)));
+    return MPRES_OK;
+}
+
+// ---- general matrices (column-major, SoA)
+int mpres_ge_add(mpres_ctx_t *c, int m, int n, const mp_array_t *alpha, const mp_array_t *A, int lda,
+                 const mp_array_t *beta, const mp_array_t *B, int ldb, mp_array_t *C, int ldc, mp_array_t *buffer,
+                 void *st) {
+    if (!c || !alpha || !A || !beta || !B || !C || !buffer) return fail(MPRES_ERR_ARG, "null argument");
+    if (m <= 0 || n <= 0) return MPRES_OK;  // src/blas/geadd.cuh:61-63
+    if (lda < std::max(1, m) || ldb < std::max(1, m) || ldc < std::max(1, m))
+        return fail(MPRES_ERR_ARG, "invalid leading dimension");  // :65-67
+    const hipStream_t q = strm(st);
+    V1(c, V1_MUL, SF_SCALAR, m, n, alpha, 1, A, lda, 1, nullptr, 0, 0, buffer, m, 1, q);  // buffer = alpha A
+    V1(c, V1_MUL, SF_SCALAR, m, n, beta, 1, B, ldb, 1, nullptr, 0, 0, C, ldc, 1, q);      // C = beta B
+    V1(c, V1_ADD, 0, m, n, nullptr, 0, C, ldc, 1, buffer, m, 1, C, ldc, 1, q);            // C = C + buffer
+    return MPRES_OK;
+}
+
+int mpres_ge_acc(mpres_ctx_t *c, int m, int n, const mp_array_t *alpha, const mp_array_t *A, int lda,
+                 const mp_array_t *beta, mp_array_t *B, int ldb, mp_array_t *buffer, void *st) {
+    if (!c || !alpha || !A || !beta || !B || !buffer) return fail(MPRES_ERR_ARG, "null argument");
+    if (m <= 0 || n <= 0) return MPRES_OK;  // src/blas/geacc.cuh:60-62
+    if (lda < std::max(1, m) || ldb < std::max(1, m))
+        return fail(MPRES_ERR_ARG, "invalid leading dimension");  // :64-66
+    const hipStream_t q = strm(st);
+    V1(c, V1_MUL, SF_SCALAR, m, n, alpha, 1, A, lda, 1, nullptr, 0, 0, buffer, m, 1, q);  // buffer = alpha A
+    V1(c, V1_MUL, SF_SCALAR, m, n, beta, 1, B, ldb, 1, nullptr, 0, 0, B, ldb, 1, q);      // B = beta B
+    V1(c, V1_ADD, 0, m, n, nullptr, 0, B, ldb, 1, buffer, m, 1, B, ldb, 1, q);            // B = B + buffer
+    return MPRES_OK;
+}
+
+int mpres_ge_diag_scale(mpres_ctx_t *c, int side, int m, int n, const mp_array_t *D, int incd, mp_array_t *A, int lda,
+                        void *st) {
+    if (!c || !D || !A) return fail(MPRES_ERR_ARG, "null argument");
+    if (m <= 0 || n <= 0) return MPRES_OK;  // src/blas/gediagscale.cuh:57-59
+    if (incd == 0 || lda < std::max(1, m)) return fail(MPRES_ERR_ARG, "invalid increment or leading dimension");
+    if (side != MPRES_LEFT && side != MPRES_RIGHT) return fail(MPRES_ERR_ARG, "invalid side %d", side);
+    // right: column j times D_j (A = A D, :75-81); left: row i times D_i (A = D A, :83-89)
+    V1(c, V1_MUL, side == MPRES_RIGHT ? SF_COL : SF_ROW, m, n, D, incd, A, lda, 1, nullptr, 0, 0, A, lda, 1, strm(st));
+    return MPRES_OK;
+}
+
+int mpres_ge_lr_scale(mpres_ctx_t *c, int m, int n, const mp_array_t *DL, int incdl, const mp_array_t *DR, int incdr,
+                      mp_array_t *A, int lda, void *st) {
+    if (!c || !DL || !DR || !A) return fail(MPRES_ERR_ARG, "null argument");
+    if (m <= 0 || n <= 0) return MPRES_OK;  // src/blas/gelrscale.cuh:59-61
+    if (incdl == 0 || incdr == 0 || lda < std::max(1, m))
+        return fail(MPRES_ERR_ARG, "invalid increment or leading dimension");  // :63-65
+    // A = DL A, then A = A DR (:78-92)
+    V1(c, V1_MUL, SF_ROW, m, n, DL, incdl, A, lda, 1, nullptr, 0, 0, A, lda, 1, strm(st));
+    V1(c, V1_MUL, SF_COL, m, n, DR, incdr, A, lda, 1, nullptr, 0, 0, A, lda, 1, strm(st));
+    return MPRES_OK;
+}
+
+size_t mpres_ge_norm_workspace(const mpres_ctx_t *c, int norm, int m, int n, int g1, int b1) {
+    if (!c || g1 <= 0 || b1 <= 0) return 0;
+    const size_t outs = (size_t)std::max(norm == MPRES_ONE_NORM ? n : m, 0);
+    return mpsize(c) * (outs * b1 + outs) + mpres_dot_workspace(c, 0, g1, b1);
+}
+
+int mpres_ge_norm(mpres_ctx_t *c, int norm, int m, int n, const mp_array_t *A, int lda, mp_array_t *r,
+                  mp_array_t *buffer, int g1, int b1, void *ws, size_t wsb, void *st) {
+    if (!c || !A || !r || !buffer) return fail(MPRES_ERR_ARG, "null argument");
+    if (norm != MPRES_ONE_NORM && norm != MPRES_INF_NORM) return fail(MPRES_ERR_ARG, "invalid norm %d", norm);
+    if (m <= 0 || n <= 0) return MPRES_OK;  // src/blas/genorm.cuh:144-146
+    if (lda < std::max(1, m)) return fail(MPRES_ERR_ARG, "invalid leading dimension");  // :148-150
+    if (g1 <= 0 || b1 <= 0) return fail(MPRES_ERR_ARG, "invalid reduction shape %d x %d", g1, b1);
+    const bool cols = norm == MPRES_ONE_NORM;
+    const int outs = cols ? n : m;
+    const size_t S = mpsize(c);
+    char *w, *res;
+    WsUse wu(c, strm(st));
+    int rc = wu.get(ws, wsb, mpres_ge_norm_workspace(c, norm, m, n, g1, b1), &w);
+    if (rc) return rc;
+    char *part = w, *sums = part + S * (size_t)outs * b1, *rest = sums + S * (size_t)outs;
+    const Ops *o = c->ops[OP_DOT];
+    // column (one-norm) or row (infinity-norm) sums of |a| into buffer (:39-139, :154-163),
+    // then the maximum of buffer as mp_norm's infinity norm with (grid1, block1)
+    HIPCHK(o->matsum(c->dev, cols, m, n, soa_of(A), lda, b1, part, strm(st)));
+    HIPCHK(o->tree(c->dev, part, b1, next_pow2((unsigned)b1), outs, sums, strm(st)));
+    HIPCHK(o->aos_to_soa(c->dev, sums, soa_of(buffer), outs, strm(st)));
+    rc = maxabs_into(c, outs, soa_of(buffer), 1, g1, b1, rest, strm(st), &res);
+    if (rc) return rc;
+    HIPCHK(o->aos_to_soa(c->dev, res, soa_of(r), 1, strm(st)));
+    return MPRES_OK;
+}
+
+int mpres_ger(mpres_ctx_t *c, int m, int n, const mp_array_t *alpha, const mp_array_t *x, int incx,
+              const mp_array_t *y, int incy, mp_array_t *A, int lda, mp_array_t *buffer1, mp_array_t *buffer2,
+              void *st) {
+    if (!c || !alpha || !x || !y || !A || !buffer1 || !buffer2) return fail(MPRES_ERR_ARG, "null argument");
+    if (m < 0 || n < 0 || lda < std::max(1, m) || incx == 0 || incy == 0)
+        return fail(MPRES_ERR_ARG, "invalid size, increment or leading dimension");  // src/blas/ger.cuh:158-163
+    if (m == 0 || n == 0) return MPRES_OK;                                            // :165-167
+    const hipStream_t q = strm(st);
+    V1(c, V1_MUL, SF_SCALAR, n, 1, alpha, 1, y, 0, incy, nullptr, 0, 0, buffer1, 0, 1, q);  // buffer1 = alpha y
+    // buffer2 = x buffer1^T, m x n with leading dimension m (:190-199)
+    V1(c, V1_MUL, SF_COL, m, n, buffer1, 1, x, 0, incx, nullptr, 0, 0, buffer2, m, 1, q);
+    V1(c, V1_ADD, 0, m, n, nullptr, 0, A, lda, 1, buffer2, m, 1, A, lda, 1, q);  // A = A + buffer2
+    return MPRES_OK;
+}
+#undef V1
+
+// ---------------------------------------------------------------- the rest of the v2 BLAS
+int mpres_scal_v2(mpres_ctx_t *c, int n, const void *alpha, const void *x, void *r, void *st) {
+    if (!c || !alpha || !x || !r) return fail(MPRES_ERR_ARG, "null argument");
+    if (n <= 0) return MPRES_OK;
+    HIPCHK(c->ops[OP_ELEM]->v2map(c->dev, V2_SCAL, n, 1, 0, (const char *)alpha, (const char *)x, 1, nullptr, 1,
+                                  nullptr, 0, (char *)r, strm(st)));
+    return MPRES_OK;
+}
+
+int mpres_prod_v2(mpres_ctx_t *c, int n, const void *x, const void *y, void *r, void *st) {
+    return mpres_elementwise(c, MPRES_OP_MUL, n, x, y, r, st);
+}
+
+int mpres_ger_v2(mpres_ctx_t *c, int m, int n, const void *alpha, const void *x, int incx, const void *y, int incy,
+                 void *A, int lda, void *st) {
+    if (!c || !alpha || !x || !y || !A) return fail(MPRES_ERR_ARG, "null argument");
+    if (m <= 0 || n <= 0) return MPRES_OK;
+    if (incx == 0 || incy == 0 || lda < std::max(1, m)) return fail(MPRES_ERR_ARG, "invalid increment or leading dimension");
+    HIPCHK(c->ops[OP_ELEM]->v2map(c->dev, V2_GER, m, n, 0, (const char *)alpha, (const char *)x, incx,
+                                  (const char *)y, incy, (char *)A, lda, nullptr, strm(st)));
+    return MPRES_OK;
+}
+
+static int uplo_flag(int uplo, int *upper) {
+    if (uplo != MPRES_UPPER && uplo != MPRES_LOWER) return fail(MPRES_ERR_ARG, "invalid uplo %d", uplo);
+    *upper = uplo == MPRES_UPPER;
+    return MPRES_OK;
+}
+
+int mpres_syr_v2(mpres_ctx_t *c, int uplo, int n, const void *alpha, const void *x, int incx, void *A, int lda,
+                 void *st) {
+    if (!c || !alpha || !x || !A) return fail(MPRES_ERR_ARG, "null argument");
+    int up, rc = uplo_flag(uplo, &up);
+    if (rc) return rc;
+    if (n <= 0) return MPRES_OK;
+    if (incx == 0 || lda < std::max(1, n)) return fail(MPRES_ERR_ARG, "invalid increment or leading dimension");
+    HIPCHK(c->ops[OP_ELEM]->v2map(c->dev, V2_SYR, n, n, up, (const char *)alpha, (const char *)x, incx, nullptr, 1,
+                                  (char *)A, lda, nullptr, strm(st)));
+    return MPRES_OK;
+}
+
+int mpres_syr2_v2(mpres_ctx_t *c, int uplo, int n, const void *alpha, const void *x, int incx, const void *y, int incy,
+                  void *A, int lda, void *st) {
+    if (!c || !alpha || !x || !y || !A) return fail(MPRES_ERR_ARG, "null argument");
+    int up, rc = uplo_flag(uplo, &up);
+    if (rc) return rc;
+    if (n <= 0) return MPRES_OK;
+    if (incx == 0 || incy == 0 || lda < std::max(1, n))
+        return fail(MPRES_ERR_ARG, "invalid increment or leading dimension");
+    HIPCHK(c->ops[OP_ELEM]->v2map(c->dev, V2_SYR2, n, n, up, (const char *)alpha, (const char *)x, incx,
+                                  (const char *)y, incy, (char *)A, lda, nullptr, strm(st)));
+    return MPRES_OK;
+}
+
+int mpres_symv_v2(mpres_ctx_t *c, int uplo, int n, const void *alpha, const void *A, int lda, const void *x, int incx,
+                  const void *beta, void *y, int incy, void *st) {
+    if (!c || !alpha || !A || !x || !beta || !y) return fail(MPRES_ERR_ARG, "null argument");
+    int up, rc = uplo_flag(uplo, &up);
+    if (rc) return rc;
+    if (n <= 0) return MPRES_OK;
+    if (incx == 0 || incy == 0 || lda < std::max(1, n))
+        return fail(MPRES_ERR_ARG, "invalid increment or leading dimension");
+    HIPCHK(c->ops[OP_ELEM]->symv_v2(c->dev, up, n, (const char *)alpha, (const char *)A, lda, (const char *)x, incx,
+                                    (const char *)beta, (char *)y, incy, strm(st)));
+    return MPRES_OK;
+}
+
+int mpres_syrk_v2(mpres_ctx_t *c, int uplo, int trans, int n, int k, const void *alpha, const void *A, int lda,
+                  const void *beta, void *C, int ldc, void *st) {
+    if (!c || !alpha || !A || !beta || !C) return fail(MPRES_ERR_ARG, "null argument");
+    int up, rc = uplo_flag(uplo, &up);
+    if (rc) return rc;
+    if (trans != MPRES_NO_TRANS && trans != MPRES_TRANS && trans != MPRES_CONJ_TRANS)
+        return fail(MPRES_ERR_ARG, "invalid trans %d", trans);
+    if (n <= 0 || k < 0) return MPRES_OK;
+    if (ldc < std::max(1, n) || lda < std::max(1, trans == MPRES_TRANS ? k : n))
+        return fail(MPRES_ERR_ARG, "invalid leading dimension");
+    // only mblas_trans reads A transposed (src/blas/v2/syrk_v2.cuh:69-73)
+    HIPCHK(c->ops[OP_ELEM]->syrk_v2(c->dev, up, trans == MPRES_TRANS, n, k, (const char *)alpha, (const char *)A, lda,
+                                    (const char *)beta, (char *)C, ldc, strm(st)));
+    return MPRES_OK;
+}
+
 // ---------------------------------------------------------------- tuning
 int mpres_lanes_supported(const mpres_ctx_t *c, int lanes) {
     return (c && find_ops(c->host.n, lanes) != nullptr) ? 1 : 0;

@@ -367,6 +367,8 @@ __global__ __launch_bounds__(BLK) void k_cmp(const DevConsts *__restrict__ C, in
 // MODE 0: terms are x_i * y_i with SoA x, y and BLAS strides (v1 dot)
 // MODE 1: terms are x_i * y_i with AoS x, y (v2 dot)
 // MODE 2: terms are AoS partials t_i (always FULL, never FIX)
+// MODE 3: terms are |x_i| with SoA x and increment incx > 0 (mp_array_reduce_sum_abs_kernel1,
+//         src/mpreduct.cuh:120-153: mp_add_abs of nonnegative operands is mp_add; always FULL)
 template <int N, int L, int MODE, bool FULL, bool FIX>
 __global__ __launch_bounds__(BLK) void k_fold(const DevConsts *__restrict__ C, long long cnt, long long nvt, SoA xs,
                                               int incx, SoA ys, int incy, AoSRef<N> xa, AoSRef<N> ya, AoSRef<N> out) {
@@ -384,6 +386,10 @@ __global__ __launch_bounds__(BLK) void k_fold(const DevConsts *__restrict__ C, l
             ok = g.template fma_step<FULL>(acc, xs, bidx(i, cnt, incx), ys, bidx(i, cnt, incy));
         } else if (MODE == 1) {
             ok = g.template fma_step<FULL>(acc, xa, i, ya, i);
+        } else if (MODE == 3) {
+            load_soa(g, p, xs, i * incx);
+            p.sign = 0;
+            g.add(acc, acc, p);
         } else {
             load_aos(g, p, xa, i);
             g.add(acc, acc, p);

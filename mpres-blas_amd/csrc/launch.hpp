@@ -12,6 +12,7 @@
 #include "kernels.hpp"
 #include "kernels_mfma.hpp"
 #include "kernels_batch.hpp"
+#include "kernels_blas.hpp"
 
 namespace mph {
 
@@ -433,6 +434,93 @@ struct Launch {
         hipLaunchKernelGGL((k_batch<N, L>), dim3(grid), dim3(BLK), 0, st, c, items, count, queue, ws);
         return hipGetLastError();
     }
+    // ---- the rest of the BLAS surface (kernels_blas.hpp) ----
+    static hipError_t v1map(const DevConsts *c, int op, int sf, int m, int n, SoA S, int incs, SoA X, int ldx,
+                            int incx, SoA Y, int ldy, int incy, SoA W, int ldw, int incw, hipStream_t st) {
+        if ((long long)m * n <= 0) return hipSuccess;
+        const dim3 gr(grid_for((long long)m * n, L)), bl(BLK);
+#define MPH_V1(OP, SF)                                                                                            \
+    hipLaunchKernelGGL((k_v1map<N, L, OP, SF>), gr, bl, 0, st, c, m, n, S, incs, X, ldx, incx, Y, ldy, incy, W, ldw, \
+                       incw)
+        if (op == V1_ADD)
+            MPH_V1(V1_ADD, SF_SCALAR);
+        else if (op == V1_SUB)
+            MPH_V1(V1_SUB, SF_SCALAR);
+        else if (sf == SF_ROW)
+            MPH_V1(V1_MUL, SF_ROW);
+        else if (sf == SF_COL)
+            MPH_V1(V1_MUL, SF_COL);
+        else
+            MPH_V1(V1_MUL, SF_SCALAR);
+#undef MPH_V1
+        return hipGetLastError();
+    }
+    // sum of |x_i| (pass 0: the strided fold; the tree and second pass are fold / tree)
+    static hipError_t fold_abs(const DevConsts *c, long long cnt, long long nvt, SoA x, int incx, char *out,
+                               hipStream_t st) {
+        if (nvt <= 0) return hipSuccess;
+        SoA none{};
+        hipLaunchKernelGGL((k_fold<N, L, 3, true, false>), dim3(grid_for(nvt, L)), dim3(BLK), 0, st, c, cnt, nvt, x,
+                           incx, none, 1, AoSRef<N>{nullptr}, AoSRef<N>{nullptr}, AoSRef<N>{out});
+        return hipGetLastError();
+    }
+    // maximum |x_i|: mode 0 the SoA vector (increment incx), 1 AoS partials; then the tree
+    static hipError_t maxabs(const DevConsts *c, int mode, long long cnt, long long nvt, SoA x, int incx,
+                             const char *xa, char *part, int b, unsigned p2, int blocks, char *out, hipStream_t st) {
+        if (nvt <= 0) return hipSuccess;
+        const dim3 gr(grid_for(nvt, L)), bl(BLK);
+        if (mode == 0)
+            hipLaunchKernelGGL((k_maxabs_fold<N, L, 0>), gr, bl, 0, st, c, cnt, nvt, x, incx, AoSRef<N>{(char *)xa},
+                               AoSRef<N>{part});
+        else
+            hipLaunchKernelGGL((k_maxabs_fold<N, L, 1>), gr, bl, 0, st, c, cnt, nvt, x, incx, AoSRef<N>{(char *)xa},
+                               AoSRef<N>{part});
+        hipLaunchKernelGGL((k_maxabs_tree<N, L>), dim3(blocks), bl, 0, st, c, AoSRef<N>{part}, b, p2, AoSRef<N>{out});
+        return hipGetLastError();
+    }
+    // per-row (cols false) or per-column sums of |A| over b1 virtual threads each
+    static hipError_t matsum(const DevConsts *c, bool cols, int m, int n, SoA A, int lda, int b1, char *part,
+                             hipStream_t st) {
+        const long long nv = (long long)(cols ? n : m) * b1;
+        if (nv <= 0) return hipSuccess;
+        const dim3 gr(grid_for(nv, L)), bl(BLK);
+        if (cols)
+            hipLaunchKernelGGL((k_matsum_fold<N, L, true>), gr, bl, 0, st, c, m, n, A, lda, b1, AoSRef<N>{part});
+        else
+            hipLaunchKernelGGL((k_matsum_fold<N, L, false>), gr, bl, 0, st, c, m, n, A, lda, b1, AoSRef<N>{part});
+        return hipGetLastError();
+    }
+    static hipError_t v2map(const DevConsts *c, int op, int m, int n, int upper, const char *alpha, const char *x,
+                            int incx, const char *y, int incy, char *A, int lda, char *r, hipStream_t st) {
+        if ((long long)m * n <= 0) return hipSuccess;
+        const dim3 gr(grid_for((long long)m * n, L)), bl(BLK);
+        const AoSRef<N> al{(char *)alpha}, X{(char *)x}, Y{(char *)y}, AA{A}, R{r};
+#define MPH_V2(OP) hipLaunchKernelGGL((k_v2map<N, L, OP>), gr, bl, 0, st, c, m, n, upper, al, X, incx, Y, incy, AA, lda, R)
+        switch (op) {
+            case V2_SCAL: MPH_V2(V2_SCAL); break;
+            case V2_GER: MPH_V2(V2_GER); break;
+            case V2_SYR: MPH_V2(V2_SYR); break;
+            default: MPH_V2(V2_SYR2); break;
+        }
+#undef MPH_V2
+        return hipGetLastError();
+    }
+    static hipError_t symv_v2(const DevConsts *c, int upper, int n, const char *alpha, const char *A, int lda,
+                              const char *x, int incx, const char *beta, char *y, int incy, hipStream_t st) {
+        if (n <= 0) return hipSuccess;
+        hipLaunchKernelGGL((k_symv_v2<N, L>), dim3(grid_for(n, L)), dim3(BLK), 0, st, c, upper, n,
+                           AoSRef<N>{(char *)alpha}, AoSRef<N>{(char *)A}, lda, AoSRef<N>{(char *)x}, incx,
+                           AoSRef<N>{(char *)beta}, AoSRef<N>{y}, incy);
+        return hipGetLastError();
+    }
+    static hipError_t syrk_v2(const DevConsts *c, int upper, int trans, int n, int k, const char *alpha, const char *A,
+                              int lda, const char *beta, char *Cm, int ldc, hipStream_t st) {
+        if (n <= 0) return hipSuccess;
+        hipLaunchKernelGGL((k_syrk_v2<N, L>), dim3(grid_for((long long)n * n, L)), dim3(BLK), 0, st, c, upper, trans,
+                           n, k, AoSRef<N>{(char *)alpha}, AoSRef<N>{(char *)A}, lda, AoSRef<N>{(char *)beta},
+                           AoSRef<N>{Cm}, ldc);
+        return hipGetLastError();
+    }
     static Ops table() {
         Ops o;
         o.N = N;
@@ -448,6 +536,13 @@ struct Launch {
         o.axpy = &axpy;
         o.axpy_v2 = &axpy_v2;
         o.hostop = &hostop;
+        o.v1map = &v1map;
+        o.fold_abs = &fold_abs;
+        o.maxabs = &maxabs;
+        o.matsum = &matsum;
+        o.v2map = &v2map;
+        o.symv_v2 = &symv_v2;
+        o.syrk_v2 = &syrk_v2;
         o.scal = &scal;
         o.gemv_fold = &gemv_fold;
         o.gemv_final = &gemv_final;

@@ -14,6 +14,15 @@ struct ErMem {  // er_float_t in memory: {double frac; long exp}
 };
 
 // mp_array_t / mp_collection_t (structure of arrays), src/types.cuh:85-104
+struct SoA {
+    int *d;
+    int *s;
+    int *e;
+    ErMem *ev;
+    const int *lenp;  // device pointer to the actual length (mp_array_t.len)
+    int len;          // filled from *lenp inside the kernels: eval upper bounds start at ev + len
+};
+
 // a sparse matrix in one of the storage formats of the SpMV kernels (kernels.hpp)
 enum { SP_CSR = 0, SP_ELL = 1, SP_JAD = 2, SP_DIA = 3 };
 struct SpMat {
@@ -26,14 +35,11 @@ struct SpMat {
     int ncols;          // DIA: columns of the matrix
 };
 
-struct SoA {
-    int *d;
-    int *s;
-    int *e;
-    ErMem *ev;
-    const int *lenp;  // device pointer to the actual length (mp_array_t.len)
-    int len;          // filled from *lenp inside the kernels: eval upper bounds start at ev + len
-};
+// element maps of the rest of the BLAS (kernels_blas.hpp): the v1 operation and where its
+// scalar factor comes from, and the v2 element operations
+enum { V1_MUL = 0, V1_ADD = 1, V1_SUB = 2 };
+enum { SF_SCALAR = 0, SF_ROW = 1, SF_COL = 2 };
+enum { V2_SCAL = 0, V2_GER = 1, V2_SYR = 2, V2_SYR2 = 3 };
 
 // One unit of work of the batched form (mpres_batch_run), built on the host and
 // read by the persistent kernel: a whole dot, or a run of output rows of a gemv.
@@ -84,6 +90,19 @@ struct Ops {
     size_t (*gemm_mfma_ws)(int, int, int, int);
     // SpMV over a storage format (SP_CSR, SP_ELL, SP_JAD, SP_DIA of kernels.hpp): pass 0 main, 1 fixup
     hipError_t (*spmv)(const DevConsts *, int, int, int, SpMat, const char *, char *, int, hipStream_t);
+    // the rest of the BLAS surface (kernels_blas.hpp)
+    hipError_t (*v1map)(const DevConsts *, int, int, int, int, SoA, int, SoA, int, int, SoA, int, int, SoA, int, int,
+                        hipStream_t);
+    hipError_t (*fold_abs)(const DevConsts *, long long, long long, SoA, int, char *, hipStream_t);
+    hipError_t (*maxabs)(const DevConsts *, int, long long, long long, SoA, int, const char *, char *, int, unsigned,
+                         int, char *, hipStream_t);
+    hipError_t (*matsum)(const DevConsts *, bool, int, int, SoA, int, int, char *, hipStream_t);
+    hipError_t (*v2map)(const DevConsts *, int, int, int, int, const char *, const char *, int, const char *, int,
+                        char *, int, char *, hipStream_t);
+    hipError_t (*symv_v2)(const DevConsts *, int, int, const char *, const char *, int, const char *, int,
+                          const char *, char *, int, hipStream_t);
+    hipError_t (*syrk_v2)(const DevConsts *, int, int, int, int, const char *, const char *, int, const char *, char *,
+                          int, hipStream_t);
     // batched form: reset the work queue, then the persistent kernel over `count` items on `grid` workgroups
     hipError_t (*batch)(const DevConsts *, const BatchItem *, int, int *, char *, int, hipStream_t);
 };

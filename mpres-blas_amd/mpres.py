@@ -54,6 +54,10 @@ class Sparse(C.Structure):  # mpres_sparse_t: a matrix of one format for mpres_c
 
 
 SPARSE_CSR, SPARSE_ELL, SPARSE_JAD, SPARSE_DIA = 0, 1, 2, 3
+# mblas_enum.cuh values (src/blas/mblas_enum.cuh:31-44)
+UPPER, LOWER = 121, 122
+LEFT, RIGHT = 141, 142
+ONE_NORM, INF_NORM = 171, 175
 
 BATCH_DOT, BATCH_GEMV = 0, 1
 
@@ -177,6 +181,31 @@ def lib():
         L.mpres_batch_run.argtypes = [vp, vp]
         L.mpres_batch_info.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(i32), vp]
         L.mpres_batch_destroy.argtypes = [vp]
+        # the rest of the v1 BLAS
+        L.mpres_scal.argtypes = [vp, i32, vp, vp, i32, vp]
+        L.mpres_rot.argtypes = [vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, vp]
+        L.mpres_waxpby.argtypes = [vp, i32, vp, vp, i32, vp, vp, i32, vp, i32, vp, vp]
+        L.mpres_axpy_dot.argtypes = [vp, i32, vp, vp, i32, vp, i32, vp, i32, vp, vp, i32, i32, vp, sz, vp]
+        L.mpres_asum_workspace.restype = sz
+        L.mpres_asum_workspace.argtypes = [vp, i32, i32, i32]
+        L.mpres_asum.argtypes = [vp, i32, vp, i32, vp, i32, i32, vp, sz, vp]
+        L.mpres_norm.argtypes = [vp, i32, i32, vp, i32, vp, i32, i32, vp, sz, vp]
+        L.mpres_ge_add.argtypes = [vp, i32, i32, vp, vp, i32, vp, vp, i32, vp, i32, vp, vp]
+        L.mpres_ge_acc.argtypes = [vp, i32, i32, vp, vp, i32, vp, vp, i32, vp, vp]
+        L.mpres_ge_diag_scale.argtypes = [vp, i32, i32, i32, vp, i32, vp, i32, vp]
+        L.mpres_ge_lr_scale.argtypes = [vp, i32, i32, vp, i32, vp, i32, vp, i32, vp]
+        L.mpres_ge_norm_workspace.restype = sz
+        L.mpres_ge_norm_workspace.argtypes = [vp, i32, i32, i32, i32, i32]
+        L.mpres_ge_norm.argtypes = [vp, i32, i32, i32, vp, i32, vp, vp, i32, i32, vp, sz, vp]
+        L.mpres_ger.argtypes = [vp, i32, i32, vp, vp, i32, vp, i32, vp, i32, vp, vp, vp]
+        # the rest of the v2 BLAS
+        L.mpres_scal_v2.argtypes = [vp, i32, vp, vp, vp, vp]
+        L.mpres_prod_v2.argtypes = [vp, i32, vp, vp, vp, vp]
+        L.mpres_ger_v2.argtypes = [vp, i32, i32, vp, vp, i32, vp, i32, vp, i32, vp]
+        L.mpres_syr_v2.argtypes = [vp, i32, i32, vp, vp, i32, vp, i32, vp]
+        L.mpres_syr2_v2.argtypes = [vp, i32, i32, vp, vp, i32, vp, i32, vp, i32, vp]
+        L.mpres_symv_v2.argtypes = [vp, i32, i32, vp, vp, i32, vp, i32, vp, vp, i32, vp]
+        L.mpres_syrk_v2.argtypes = [vp, i32, i32, i32, i32, vp, vp, i32, vp, vp, i32, vp]
         L.mpres_profile.argtypes = [vp, i32]
         L.mpres_profile_read.argtypes = [vp, i32, C.POINTER(C.c_double), C.POINTER(C.c_long), i32]
         L.mpres_lanes_supported.argtypes = [vp, i32]

@@ -355,6 +355,277 @@ class Oracle:
                             _p(rows_a), 0 if rows is None else len(rows_a))
         return y
 
+    # ---- the rest of the BLAS (src/blas, src/blas/v2): compositions of the element operations
+    # above in the reference's kernel order. Numbers are AoS records here whatever layout the
+    # library keeps them in; x[bidx] is element i of a vector with increment inc.
+    def _bcast(self, a, count):
+        return np.repeat(np.ascontiguousarray(a)[:1], count)
+
+    def _vsub(self, x, y):
+        """mp_vector_sub_esi_kernel + mp_vector_add_digits_kernel (src/mpvector.cuh:487-560):
+        the v1 add with y's sign inverted"""
+        y = np.array(y, copy=True)
+        y["sign"] ^= 1
+        return self.vadd(x, y)
+
+    def scal(self, n, alpha, x, incx):
+        """x = alpha x: cuda::mp_scal (src/blas/scal.cuh:43-61)"""
+        x = np.array(x, copy=True)
+        if n <= 0 or incx <= 0:
+            return x
+        i = bidx(np.arange(n), n, incx)
+        x[i] = self.mul(x[i], self._bcast(alpha, n))
+        return x
+
+    def rot(self, n, x, incx, y, incy, c, s):
+        """cuda::mp_rot (src/blas/rot.cuh:49-100): returns (x, y, buffer1, buffer2); the c scaling
+        runs with increment 1 (:78-81)"""
+        x, y = np.array(x, copy=True), np.array(y, copy=True)
+        ix, iy = bidx(np.arange(n), n, incx), bidx(np.arange(n), n, incy)
+        S, Cc = self._bcast(s, n), self._bcast(c, n)
+        b1 = self.mul(x[ix], S)
+        b2 = self.mul(y[iy], S)
+        x[:n] = self.mul(x[:n], Cc)
+        y[:n] = self.mul(y[:n], Cc)
+        x[ix] = self.vadd(x[ix], b2)
+        y[iy] = self._vsub(y[iy], b1)
+        return x, y, b1, b2
+
+    def waxpby(self, n, alpha, x, incx, beta, y, incy, w, incw):
+        """w = alpha x + beta y: cuda::mp_waxpby (src/blas/waxpby.cuh:51-89); returns (w, buffer)"""
+        w = np.array(w, copy=True)
+        ar = np.arange(n)
+        buf = self.mul(x[bidx(ar, n, incx)], self._bcast(alpha, n))
+        iw = bidx(ar, n, incw)
+        w[iw] = self.mul(y[bidx(ar, n, incy)], self._bcast(beta, n))
+        w[iw] = self.vadd(w[iw], buf)
+        return w, buf
+
+    def axpy_dot(self, n, alpha, w, incw, v, incv, u, incu, g3, b3):
+        """w = w - alpha v, r = u . w: cuda::mp_axpy_dot (src/blas/axpydot.cuh:54-86); returns (w, r)"""
+        w = np.array(w, copy=True)
+        ar = np.arange(n)
+        buf = self.mul(v[bidx(ar, n, incv)], self._bcast(alpha, n))
+        iw = bidx(ar, n, incw)
+        w[iw] = self._vsub(w[iw], buf)
+        return w, self.dot(u, w, g3, b3, POLICY_D, incu, incw, n)
+
+    def _fold_tree(self, terms_at, cnt, g, b, step):
+        """the reduction kernels' shape (src/mpreduct.cuh:120-286): virtual thread v of g*b folds
+        terms v, v + g b, ... from zero with step(acc, term); then, per block of b, the halving tree
+        from the least power of two >= b. Returns the g block results."""
+        G = g * b
+        acc = self.empty(G)
+        for k in range((cnt + G - 1) // G):
+            v = np.arange(G)
+            idx = v + k * G
+            m = idx < cnt
+            acc[m] = step(acc[m], terms_at(idx[m]))
+        acc = acc.reshape(g, b)
+        h = next_pow2(b) >> 1
+        while h >= 1:
+            t = np.arange(min(h, b))
+            t = t[t + h < b]
+            if len(t):
+                lhs = acc[:, t].reshape(-1)
+                rhs = acc[:, t + h].reshape(-1)
+                acc[:, t] = step(lhs, rhs).reshape(g, len(t))
+            h >>= 1
+        return np.ascontiguousarray(acc[:, 0])
+
+    def _max_step(self, acc, t):
+        """keep t where |t| > |acc| (mp_cmp_abs == 1)"""
+        r = np.array(acc, copy=True)
+        sel = self.cmp(t, acc, abs_only=True) == 1
+        r[sel] = t[sel]
+        return r
+
+    def asum(self, n, x, incx, g1, b1):
+        """r = sum |x_i|: cuda::mp_asum<g1, b1> (src/blas/asum.cuh:41-70), with
+        mp_array_reduce_sum_abs_kernel1/2 (src/mpreduct.cuh:120-192); mp_add_abs throughout"""
+        add = lambda a, t: self.add_abs(a, t)  # noqa: E731
+        p = self._fold_tree(lambda i: x[i * incx], n, g1, b1, add)
+        r = self._fold_tree(lambda i: p[i], g1, 1, b1, add)
+        r["sign"] = 0
+        return r
+
+    def norm_inf(self, n, x, incx, g1, b1):
+        """r = max |x_i|: cuda::mp_norm<g1, b1>(mblas_inf_norm) (src/blas/norm.cuh:43-79), with
+        mp_array_reduce_max_abs_kernel1/2 (src/mpreduct.cuh:199-286)"""
+        p = self._fold_tree(lambda i: x[i * incx], n, g1, b1, self._max_step)
+        p["sign"] = 0
+        r = self._fold_tree(lambda i: p[i], g1, 1, b1, self._max_step)
+        r["sign"] = 0
+        return r
+
+    def _ge_map(self, m, n, A, lda, f):
+        """indices of the leading m x n part of a column-major array with leading dimension lda"""
+        i, j = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
+        return (i + j * lda).reshape(-1, order="F")
+
+    def ge_add(self, m, n, alpha, A, lda, beta, B, ldb, Cm, ldc):
+        """C = alpha A + beta B: cuda::mp_ge_add (src/blas/geadd.cuh:58-114); returns (C, buffer)"""
+        Cm = np.array(Cm, copy=True)
+        ia, ib, ic = (self._ge_map(m, n, None, ld, None) for ld in (lda, ldb, ldc))
+        buf = self.mul(A[ia], self._bcast(alpha, m * n))
+        Cm[ic] = self.mul(B[ib], self._bcast(beta, m * n))
+        Cm[ic] = self.vadd(Cm[ic], buf)
+        return Cm, buf
+
+    def ge_acc(self, m, n, alpha, A, lda, beta, B, ldb):
+        """B = alpha A + beta B: cuda::mp_ge_acc (src/blas/geacc.cuh:57-113); returns (B, buffer)"""
+        B = np.array(B, copy=True)
+        ia, ib = (self._ge_map(m, n, None, ld, None) for ld in (lda, ldb))
+        buf = self.mul(A[ia], self._bcast(alpha, m * n))
+        B[ib] = self.mul(B[ib], self._bcast(beta, m * n))
+        B[ib] = self.vadd(B[ib], buf)
+        return B, buf
+
+    def ge_diag_scale(self, side_right, m, n, D, incd, A, lda):
+        """A = A D (right) or D A (left): cuda::mp_ge_diag_scale (src/blas/gediagscale.cuh:54-91),
+        the factor of element (i, j) D[bidx(j, n, incd)] or D[bidx(i, m, incd)]
+        (mp_mat2vec_right/left_scal_esi_kernel, src/mpmatrix.cuh:259-404)"""
+        A = np.array(A, copy=True)
+        ia = self._ge_map(m, n, None, lda, None)
+        i, j = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
+        i, j = i.reshape(-1, order="F"), j.reshape(-1, order="F")
+        d = D[bidx(j, n, incd)] if side_right else D[bidx(i, m, incd)]
+        A[ia] = self.mul(A[ia], d)
+        return A
+
+    def ge_lr_scale(self, m, n, DL, incdl, DR, incdr, A, lda):
+        """A = DL A DR: cuda::mp_ge_lr_scale (src/blas/gelrscale.cuh:56-93), left first"""
+        A = self.ge_diag_scale(False, m, n, DL, incdl, A, lda)
+        return self.ge_diag_scale(True, m, n, DR, incdr, A, lda)
+
+    def ge_norm(self, one_norm, m, n, A, lda, g1, b1):
+        """cuda::mp_ge_norm<g1, b1> (src/blas/genorm.cuh:142-173): per column (one-norm) or row
+        (infinity-norm) the sum of |a| by b1 threads striding over it from zero and the halving
+        tree (matrix_col/row_sum_abs_kernel, :38-139), written with sign 0; then mp_norm's
+        infinity norm of those sums. Returns (r, sums)."""
+        outs, ln = (n, m) if one_norm else (m, n)
+        add = lambda a, t: self.add_abs(a, t)  # noqa: E731
+        acc = self.empty(outs * b1).reshape(outs, b1)
+        o = np.arange(outs)
+        for k in range((ln + b1 - 1) // b1):
+            for t in range(b1):
+                i = t + k * b1
+                if i >= ln:
+                    continue
+                idx = (i + o * lda) if one_norm else (o + i * lda)
+                acc[:, t] = add(acc[:, t], A[idx])
+        h = next_pow2(b1) >> 1
+        while h >= 1:
+            for t in range(min(h, b1)):
+                if t + h < b1:
+                    acc[:, t] = add(acc[:, t], acc[:, t + h])
+            h >>= 1
+        sums = np.ascontiguousarray(acc[:, 0])
+        sums["sign"] = 0
+        return self.norm_inf(outs, sums, 1, g1, b1), sums
+
+    def ger(self, m, n, alpha, x, incx, y, incy, A, lda):
+        """A = A + x (alpha y)^T: cuda::mp_ger (src/blas/ger.cuh:155-214); returns (A, buffer1,
+        buffer2)"""
+        A = np.array(A, copy=True)
+        b1 = self.mul(y[bidx(np.arange(n), n, incy)], self._bcast(alpha, n))
+        i, j = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
+        i, j = i.reshape(-1, order="F"), j.reshape(-1, order="F")
+        b2 = self.mul(x[bidx(i, m, incx)], b1[j])  # mp_cvec2rvec_mul_esi/digits_kernel (:39-150)
+        ia = i + j * lda
+        A[ia] = self.vadd(A[ia], b2)
+        return A, b1, b2
+
+    # v2 (mp_float_t arrays)
+    def scal_v2(self, alpha, x):
+        """cuda::mp_scal (src/blas/v2/scal_v2.cuh:37-48): r = alpha x"""
+        return self.mul(self._bcast(alpha, len(x)), x)
+
+    def ger_v2(self, m, n, alpha, x, incx, y, incy, A, lda):
+        """cuda::mp_ger (src/blas/v2/ger_v2.cuh:49-68): A += (alpha x_i) y_j"""
+        A = np.array(A, copy=True)
+        i, j = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
+        i, j = i.reshape(-1, order="F"), j.reshape(-1, order="F")
+        t = self.mul(self._bcast(alpha, m * n), x[bidx(i, m, incx)])
+        t = self.mul(t, y[bidx(j, n, incy)])
+        ia = i + j * lda
+        A[ia] = self.add(A[ia], t)
+        return A
+
+    def _tri(self, upper, n):
+        i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+        i, j = i.reshape(-1, order="F"), j.reshape(-1, order="F")
+        keep = (i <= j) if upper else (j <= i)
+        return i[keep], j[keep]
+
+    def syr_v2(self, upper, n, alpha, x, incx, A, lda):
+        """cuda::mp_syr (src/blas/v2/syr_v2.cuh:58-76): A += (x_i x_j) alpha on the triangle"""
+        A = np.array(A, copy=True)
+        i, j = self._tri(upper, n)
+        t = self.mul(x[bidx(i, n, incx)], x[bidx(j, n, incx)])
+        t = self.mul(t, self._bcast(alpha, len(i)))
+        ia = i + j * lda
+        A[ia] = self.add(A[ia], t)
+        return A
+
+    def syr2_v2(self, upper, n, alpha, x, incx, y, incy, A, lda):
+        """cuda::mp_syr2 (src/blas/v2/syr2_v2.cuh:59-83): A += (x_i y_j) alpha + (y_i x_j) alpha"""
+        A = np.array(A, copy=True)
+        i, j = self._tri(upper, n)
+        a = self._bcast(alpha, len(i))
+        t1 = self.mul(self.mul(x[bidx(i, n, incx)], y[bidx(j, n, incy)]), a)
+        t2 = self.mul(self.mul(y[bidx(i, n, incy)], x[bidx(j, n, incx)]), a)
+        ia = i + j * lda
+        A[ia] = self.add(A[ia], self.add(t1, t2))
+        return A
+
+    def symv_v2(self, upper, n, alpha, A, lda, x, incx, beta, y, incy):
+        """cuda::mp_symv (src/blas/v2/symv_v2.cuh:57-95): per row, sum over columns in order of
+        x_col A(row, col) from the stored triangle, then y = (beta y) + (alpha sum)"""
+        y = np.array(y, copy=True)
+        rows = np.arange(n)
+        s = self.empty(n)
+        for col in range(n):
+            lower_part = rows <= col
+            if not upper:
+                lower_part = ~lower_part
+            ia = np.where(lower_part, rows + col * lda, col + rows * lda)
+            s = self.add(s, self.mul(self._bcast(x[bidx(col, n, incx):], n), A[ia]))
+        iy = bidx(rows, n, incy)
+        yb = self.mul(self._bcast(beta, n), y[iy])
+        s = self.mul(self._bcast(alpha, n), s)
+        y[iy] = self.add(yb, s)
+        return y
+
+    def syrk_v2(self, upper, trans, n, k, alpha, A, lda, beta, Cm, ldc):
+        """cuda::mp_syrk (src/blas/v2/syrk_v2.cuh:61-85): C(row, col) = (sum_i op(A)(row, i)
+        op(A)(col, i)) alpha + C(row, col) beta on the triangle; only trans reads A transposed"""
+        Cm = np.array(Cm, copy=True)
+        r, c = self._tri(upper, n)
+        s = self.empty(len(r))
+        for i in range(k):
+            ia = (i + lda * r) if trans else (r + lda * i)
+            ib = (i + lda * c) if trans else (c + lda * i)
+            s = self.add(s, self.mul(A[ia], A[ib]))
+        s = self.mul(s, self._bcast(alpha, len(r)))
+        ic = r + c * ldc
+        t = self.mul(Cm[ic], self._bcast(beta, len(r)))
+        Cm[ic] = self.add(s, t)
+        return Cm
+
+
+def bidx(i, n, inc):
+    """index of element i of a vector of n with increment inc (negative: from the end)"""
+    i = np.asarray(i, np.int64)
+    return i * inc if inc > 0 else (-n + i + 1) * inc
+
+
+def next_pow2(v):
+    p = 1
+    while p < v:
+        p <<= 1
+    return p
+
 
 def set_threads(n):
     lib().orc_set_threads(int(n))
