@@ -9,14 +9,16 @@
  * tests/test_oracle_*.py):
  *   constants            - exact big-integer / rational recomputation in Python
  *   dadd/dmul/ddiv D     - exact rational rounding (fractions) on random and edge inputs
- *   dadd/dmul/ddiv E     - enclosure (result <= / >= exact) and the 2-ulp distance bound
+ *   dadd/dmul/ddiv E     - bit for bit against the definition RN(c) -/+ (phi|c| + eta) with phi, eta
+ *                          derived from src/bitwise.cuh:51-61, on edge inputs; enclosure
  *   psum                 - tree shape: equals the balanced pairwise tree computed in
  *                          Python with the pinned directed adds
  *   mod ops, mrc, mrd    - brute force with Python integers (CRT)
  *   rank                 - exact rank floor(sum s_i/m_i) with Python fractions
  *   scale2pow            - closed form floor(X / 2^D) by CRT
- *   eval (full/fast)     - enclosure low <= X/M <= upp with exact rationals, the width
- *                          bound of the refinement, the zero case, and the reference's
+ *   eval (full/fast)     - bits recomposed from exact directed rounding and the CRT on inputs
+ *                          forcing every branch; fast == full without ambiguity; the width
+ *                          bound of the sums; enclosure low <= X/M <= upp; the reference's
  *                          walk-down sweep from M-1 (tests/arith/test_rns_eval_verify.cu)
  *   set_d                - closed form: X = trimmed significand, exp, sign
  *   mul / mul_d          - closed form X*Y (no wrap) and exponent / sign rules
