@@ -1,6 +1,6 @@
-"""The library's reduction kernels, compiled for the CPU with g++ against a
-host stand-in of the HIP header (tests/host/emu) and run one lane per number,
-against the oracle. This checks the kernels' own logic - the fast kernels'
+"""The library's reduction kernels, compiled for the CPU with g++ (and with clang,
+locals initialised to a non-zero pattern) against a host stand-in of the HIP header
+(tests/host/emu) and run one lane per number, against the oracle. This checks the kernels' own logic - the fast kernels'
 abandoning and marking of rare chains, the fixup kernels, the gemm epilogues -
 on the CPU, at 8, 16 and 32 moduli, including chains that need rounding and
 exact cancellations."""
@@ -19,10 +19,19 @@ D = po.POLICY_D
 SETS = {"8_2double": (-20, 20), "16_4double": (-150, 150), "32_8double": (-400, 400)}
 
 
-@pytest.fixture(scope="module")
-def emu(tmp_path_factory):
+CLANG = "/opt/rocm/llvm/bin/clang++"
+BUILDS = {
+    "g++": ["g++", "-O1"],
+    # every local variable starts as a non-zero byte pattern: a read of an uninitialised
+    # local (which the GPU would see as a stale register) changes the results
+    "clang-pattern": [CLANG, "-O2", "-ftrivial-auto-var-init=pattern", "-Wno-unknown-warning-option"],
+}
+
+
+@pytest.fixture(scope="module", params=[b for b in BUILDS if b == "g++" or os.path.exists(CLANG)])
+def emu(request, tmp_path_factory):
     out = tmp_path_factory.mktemp("emu") / "libkemu.so"
-    cmd = ["g++", "-std=c++17", "-O1", "-fPIC", "-shared", "-ffp-contract=off", "-pthread",
+    cmd = BUILDS[request.param] + ["-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-pthread",
            "-I", os.path.join(ROOT, "tests", "host", "emu"),
            os.path.join(ROOT, "tests", "host", "kernels_emu.cpp"),
            os.path.join(ROOT, "mpres-blas_amd", "csrc", "consts.cpp"), "-o", str(out)]
