@@ -110,6 +110,8 @@ struct mpres_ctx {
 namespace {
 
 size_t mpsize(const mpres_ctx *c) { return (size_t)(4 * c->host.n + 40); }
+// the split-form SpMV's workspace: counts, 3 x SP_NB class counters, the row order and two row lists
+size_t spmv_ws_bytes(int m) { return 4 * (2 + 3 * (size_t)SP_NB + 3 * (size_t)m); }
 
 SoA soa_of(const mp_array_t *a) { return soa_view(a); }
 
@@ -677,16 +679,33 @@ int mpres_norm2_v2(mpres_ctx_t *c, int n, const void *x, int blocks, int threads
 // ---------------------------------------------------------------- CG / PCG (CSR)
 size_t mpres_cg_workspace(const mpres_ctx_t *c, int n, int threads, int blocks_reduce) {
     if (!c || n <= 0 || threads <= 0 || blocks_reduce <= 0) return 0;
-    return mpsize(c) * (4 * (size_t)n + 5) + mpres_dot_workspace(c, n, blocks_reduce, threads) + 16;
+    return mpsize(c) * (4 * (size_t)n + 5) + mpres_dot_workspace(c, n, blocks_reduce, threads) + 16 +
+           spmv_ws_bytes(n);
 }
 
-// y = A x in the solver's format (the scalar SpMV kernels: one thread per row in the reference)
-static int solver_spmv(mpres_ctx_t *c, int n, const mpres_sparse_t *A, const void *x, void *y, void *st) {
+static int spmv_common(mpres_ctx_t *c, int fmt, int m, const SpMat &A, int tpr, const void *x, void *y, void *st,
+                       int *list);
+
+// y = A x in the solver's format (the scalar SpMV kernels: one thread per row in the reference);
+// list: the solver workspace's room for the CSR kernels' long-row list
+static int solver_spmv(mpres_ctx_t *c, int n, const mpres_sparse_t *A, const void *x, void *y, void *st, int *list) {
     switch (A->format) {
-        case MPRES_SPARSE_CSR: return mpres_spmv_csr(c, n, &A->csr, x, y, st);
-        case MPRES_SPARSE_ELL: return mpres_spmv_ell(c, n, A->maxnzr, &A->ell, x, y, st);
-        case MPRES_SPARSE_JAD: return mpres_spmv_jad(c, n, A->maxnzr, &A->jad, x, y, st);
-        case MPRES_SPARSE_DIA: return mpres_spmv_dia(c, n, n, A->ndiag, &A->dia, x, y, st);
+        case MPRES_SPARSE_CSR:
+            if (!A->csr.as || !A->csr.ja || !A->csr.irp) return fail(MPRES_ERR_ARG, "null argument");
+            return spmv_common(c, SP_CSR, n, SpMat{A->csr.as, A->csr.ja, A->csr.irp, nullptr, nullptr, 0, 0}, 1, x,
+                               y, st, list);
+        case MPRES_SPARSE_ELL:
+            if (A->maxnzr < 0) return fail(MPRES_ERR_ARG, "maxnzr %d", A->maxnzr);
+            return spmv_common(c, SP_ELL, n, SpMat{A->ell.as, A->ell.ja, nullptr, nullptr, nullptr, A->maxnzr, 0}, 1,
+                               x, y, st, list);
+        case MPRES_SPARSE_JAD:
+            if (A->maxnzr < 0) return fail(MPRES_ERR_ARG, "maxnzr %d", A->maxnzr);
+            return spmv_common(c, SP_JAD, n, SpMat{A->jad.as, A->jad.ja, A->jad.jcp, A->jad.perm, nullptr, A->maxnzr, 0},
+                               1, x, y, st, list);
+        case MPRES_SPARSE_DIA:
+            if (A->ndiag < 0) return fail(MPRES_ERR_ARG, "ndiag %d", A->ndiag);
+            return spmv_common(c, SP_DIA, n, SpMat{A->dia.as, nullptr, nullptr, nullptr, A->dia.offset, A->ndiag, n}, 1,
+                               x, y, st, list);
         default: return fail(MPRES_ERR_ARG, "unknown sparse format %d", A->format);
     }
 }
@@ -715,13 +734,14 @@ int mpres_cg(mpres_ctx_t *c, int n, const mpres_sparse_t *A, const void *b, doub
     char *r = w, *p = r + S * n, *q = p + S * n, *z = q + S * n, *rho = z + S * n, *rhop = rho + S, *alpha = rhop + S,
          *beta = alpha + S, *pq = beta + S, *dws = pq + S;
     double *dres = (double *)(dws + dw);
+    int *list = (int *)(dws + dw + 16);
     char *zz = precond ? z : r;  // z = M^-1 r (diagonal preconditioner) or r itself
 #define CGCHK(call)                   \
     do {                              \
         if (int e_ = (call)) return e_; \
     } while (0)
     // r = b - A x (src/sparse/solver/cg_csr.cuh:78-80)
-    CGCHK(solver_spmv(c, n, A, x, r, st));
+    CGCHK(solver_spmv(c, n, A, x, r, st, list));
     CGCHK(mpres_diff_v2(c, n, b, r, r, st));
     double norm0 = 0, normk = 0;
     CGCHK(norm2_into(c, n, r, blocks_reduce, threads, dws, dw, dres, &norm0, s));
@@ -739,7 +759,7 @@ int mpres_cg(mpres_ctx_t *c, int n, const mpres_sparse_t *A, const void *b, doub
             CGCHK(mpres_div(c, 1, rho, rhop, beta, st));
             CGCHK(mpres_axpy_v2(c, n, beta, p, zz, p, st));
         }
-        CGCHK(solver_spmv(c, n, A, p, q, st));
+        CGCHK(solver_spmv(c, n, A, p, q, st, list));
         CGCHK(mpres_dot_v2(c, n, p, q, pq, blocks_reduce, threads, dws, dw, st));
         CGCHK(mpres_div(c, 1, rho, pq, alpha, st));
         CGCHK(mpres_axpy_v2(c, n, alpha, p, x, x, st));
@@ -948,7 +968,9 @@ int mpres_gemm_v2(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const voi
 
 // ---------------------------------------------------------------- SpMV
 // one SpMV in any storage format: the main kernel, then the fixup of the rows it marked
-static int spmv_common(mpres_ctx_t *c, int fmt, int m, const SpMat &A, int tpr, const void *x, void *y, void *st) {
+// list: spmv_ws_bytes(m) for the split form's row order and lists (NULL: from the context's pool)
+static int spmv_common(mpres_ctx_t *c, int fmt, int m, const SpMat &A, int tpr, const void *x, void *y, void *st,
+                       int *list) {
     if (m <= 0) return MPRES_OK;
     if (tpr < 1 || tpr > 32 || (tpr & (tpr - 1))) return fail(MPRES_ERR_ARG, "threadsPerRow %d not in 1..32", tpr);
     const Ops *o = c->ops[OP_SPMV];
@@ -963,17 +985,24 @@ static int spmv_common(mpres_ctx_t *c, int fmt, int m, const SpMat &A, int tpr, 
         }
         if (tpr > 256 / o->L) return fail(MPRES_ERR_UNSUPPORTED, "threadsPerRow %d with %d lanes", tpr, o->L);
     }
+    // the split form's row order and row lists
+    char *w = (char *)list;
+    WsUse wu(c, strm(st));
+    if (!w) {
+        const int rc = wu.get(nullptr, 0, spmv_ws_bytes(m), &w);
+        if (rc) return rc;
+    }
     {
         ProfScope ps(c, OP_SPMV, strm(st));
-        HIPCHK(o->spmv(c->dev, 0, fmt, m, A, (const char *)x, (char *)y, tpr, strm(st)));
+        HIPCHK(o->spmv(c->dev, 0, fmt, m, A, (const char *)x, (char *)y, tpr, (int *)w, strm(st)));
     }
-    HIPCHK(o->spmv(c->dev, 1, fmt, m, A, (const char *)x, (char *)y, tpr, strm(st)));
+    HIPCHK(o->spmv(c->dev, 1, fmt, m, A, (const char *)x, (char *)y, tpr, (int *)w, strm(st)));
     return MPRES_OK;
 }
 
 int mpres_spmv_csrv(mpres_ctx_t *c, int m, const csr_t *csr, int tpr, const void *x, void *y, void *st) {
     if (!c || !csr || !x || !y) return fail(MPRES_ERR_ARG, "null argument");
-    return spmv_common(c, SP_CSR, m, SpMat{csr->as, csr->ja, csr->irp, nullptr, nullptr, 0, 0}, tpr, x, y, st);
+    return spmv_common(c, SP_CSR, m, SpMat{csr->as, csr->ja, csr->irp, nullptr, nullptr, 0, 0}, tpr, x, y, st, nullptr);
 }
 
 int mpres_spmv_csr(mpres_ctx_t *c, int m, const csr_t *csr, const void *x, void *y, void *st) {
@@ -983,14 +1012,14 @@ int mpres_spmv_csr(mpres_ctx_t *c, int m, const csr_t *csr, const void *x, void 
 int mpres_spmv_ell(mpres_ctx_t *c, int m, int maxnzr, const ell_t *ell, const void *x, void *y, void *st) {
     if (!c || !ell || !x || !y) return fail(MPRES_ERR_ARG, "null argument");
     if (maxnzr < 0) return fail(MPRES_ERR_ARG, "maxnzr %d", maxnzr);
-    return spmv_common(c, SP_ELL, m, SpMat{ell->as, ell->ja, nullptr, nullptr, nullptr, maxnzr, 0}, 1, x, y, st);
+    return spmv_common(c, SP_ELL, m, SpMat{ell->as, ell->ja, nullptr, nullptr, nullptr, maxnzr, 0}, 1, x, y, st, nullptr);
 }
 
 int mpres_spmv_jadv(mpres_ctx_t *c, int m, int maxnzr, const jad_t *jad, int tpr, const void *x, void *y, void *st) {
     if (!c || !jad || !x || !y) return fail(MPRES_ERR_ARG, "null argument");
     if (maxnzr < 0) return fail(MPRES_ERR_ARG, "maxnzr %d", maxnzr);
     return spmv_common(c, SP_JAD, m, SpMat{jad->as, jad->ja, jad->jcp, jad->perm, nullptr, maxnzr, 0}, tpr, x, y,
-                       st);
+                       st, nullptr);
 }
 
 int mpres_spmv_jad(mpres_ctx_t *c, int m, int maxnzr, const jad_t *jad, const void *x, void *y, void *st) {
@@ -1000,7 +1029,7 @@ int mpres_spmv_jad(mpres_ctx_t *c, int m, int maxnzr, const jad_t *jad, const vo
 int mpres_spmv_dia(mpres_ctx_t *c, int m, int n, int ndiag, const dia_t *dia, const void *x, void *y, void *st) {
     if (!c || !dia || !x || !y) return fail(MPRES_ERR_ARG, "null argument");
     if (ndiag < 0 || n < 0) return fail(MPRES_ERR_ARG, "invalid dimensions");
-    return spmv_common(c, SP_DIA, m, SpMat{dia->as, nullptr, nullptr, nullptr, dia->offset, ndiag, n}, 1, x, y, st);
+    return spmv_common(c, SP_DIA, m, SpMat{dia->as, nullptr, nullptr, nullptr, dia->offset, ndiag, n}, 1, x, y, st, nullptr);
 }
 
 // ---------------------------------------------------------------- profiling

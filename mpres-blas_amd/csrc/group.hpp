@@ -134,15 +134,25 @@ __device__ __forceinline__ void sc_mul(const ScC &K, Sc &r, int xs, int xe, Er x
     er_mdk_pair(x0, y0, x1, y1, K.uuf, K.uue, K.uu_rh, K.uu_rl, K.ulf, K.ule, K.ul_rh, K.ul_rl, r.ev0, r.ev1);
 }
 
-// acc = acc + x*y on the scalar fields, for chains whose every step is exact:
-// false when the product or the sum needs rounding, the sum's interval
-// straddles zero, or the alignment drops a nonzero operand. The decisions are
-// those of the lane-group step (Grp::step_tail).
-__device__ __forceinline__ bool sc_fma_step(const ScC &C, Sc &acc, int xs, int xe, Er x0, Er x1, int ys, int ye,
-                                            Er y0, Er y1) {
-    Sc p;
-    sc_mul(C, p, xs, xe, x0, x1, ys, ye, y0, y1);
-    if (sc_needs_round(C, p.ev1)) return false;
+// scalar part of cuda::mp_mul_d (src/arith/muld.cuh:59-67): y's significand (53-bit
+// integer, 0 for y = 0) scales the bounds and its exponent adds to x's
+__device__ __forceinline__ void sc_muld(Sc &r, int xs, int xe, Er x0, Er x1, double y) {
+    const uint64_t u = d2u(y);
+    const bool nz = y != 0;
+    const double sig = nz ? (double)((u & ((((uint64_t)1) << 52) - 1)) | (((uint64_t)1) << 52)) : 0.0;
+    r.exp = xe + ((int)((u >> 52) & 2047) - (nz ? 1075 : 0));
+    r.sign = xs ^ (int)(u >> 63);
+    double lo, hi;
+    mul_rd_ru(x0.frac, sig, x1.frac, sig, lo, hi);
+    r.ev0 = er_adjust(lo, x0.exp);
+    r.ev1 = er_adjust(hi, x1.exp);
+}
+
+// acc = acc + p on the scalar fields, for chains whose every step is exact:
+// false when the sum needs rounding, the sum's interval straddles zero, or the
+// alignment drops a nonzero operand. The decisions are those of the lane-group
+// step (Grp::step_tail) and add (Grp::add_try).
+__device__ __forceinline__ bool sc_add_step(const ScC &C, Sc &acc, const Sc &p) {
     const Align a = align_sc(C.mp_j, acc.sign, acc.exp, acc.ev0, acc.ev1, p.sign, p.exp, p.ev0, p.ev1);
     if ((!a.nzx && acc.ev1.frac != 0) || (!a.nzy && p.ev1.frac != 0)) return false;
     Er e0, e1;
@@ -159,6 +169,25 @@ __device__ __forceinline__ bool sc_fma_step(const ScC &C, Sc &acc, int xs, int x
         acc.ev1 = e1;
     }
     return !sc_needs_round(C, acc.ev1);
+}
+// acc = acc + x*y (y a double, mp_mul_d then mp_add) on the scalar fields, exact steps only
+__device__ __forceinline__ bool sc_muld_step(const ScC &C, Sc &acc, int xs, int xe, Er x0, Er x1, double y) {
+    Sc p;
+    sc_muld(p, xs, xe, x0, x1, y);
+    if (sc_needs_round(C, p.ev1)) return false;
+    return sc_add_step(C, acc, p);
+}
+
+// acc = acc + x*y on the scalar fields, for chains whose every step is exact:
+// false when the product or the sum needs rounding, the sum's interval
+// straddles zero, or the alignment drops a nonzero operand. The decisions are
+// those of the lane-group step (Grp::step_tail).
+__device__ __forceinline__ bool sc_fma_step(const ScC &C, Sc &acc, int xs, int xe, Er x0, Er x1, int ys, int ye,
+                                            Er y0, Er y1) {
+    Sc p;
+    sc_mul(C, p, xs, xe, x0, x1, ys, ye, y0, y1);
+    if (sc_needs_round(C, p.ev1)) return false;
+    return sc_add_step(C, acc, p);
 }
 
 template <int N, int L>

@@ -1228,20 +1228,13 @@ __device__ __forceinline__ void sp_terms(const SpMat &A, int m, long long row, i
         }
     }
 }
-template <int N, int L, bool FULL, bool FIX, int FMT = SP_CSR>
-__global__ __launch_bounds__(BLK) void k_spmv(const DevConsts *__restrict__ C, int m, SpMat A, AoSRef<N> x,
-                                              AoSRef<N> y, int tpr) {
+// one row of k_spmv (its tree synchronises the workgroup: every thread calls it)
+template <int N, int L, bool FULL, bool FIX, int FMT, int W>
+__device__ __forceinline__ void spmv_row(const DevConsts *__restrict__ C, int m, const SpMat &A, AoSRef<N> x,
+                                         AoSRef<N> y, int tpr, bool live, long long row, int lane, int gl,
+                                         int (&sd)[W][N], int (&ss)[W], int (&se)[W], double (&sf0)[W],
+                                         double (&sf1)[W], int (&sx0)[W], int (&sx1)[W]) {
     constexpr int R = N / L;
-    constexpr int W = BLK / L;
-    __shared__ int sd[W][N];
-    __shared__ int ss[W], se[W];
-    __shared__ double sf0[W], sf1[W];
-    __shared__ int sx0[W], sx1[W];
-    const int gl = threadIdx.x / L;
-    const long long v = group_id<L>();
-    const long long row = v / tpr;
-    const int lane = (int)(v % tpr);
-    bool live = row < m;
     const long long orow = live ? sp_out_row<FMT>(A, row) : 0;
     if (FIX && live) live = *y.sign(orow) == REDO;
     Grp<N, L> g(C);
@@ -1291,4 +1284,400 @@ __global__ __launch_bounds__(BLK) void k_spmv(const DevConsts *__restrict__ C, i
     }
 }
 
+// k_spmv over rows 0 .. m-1, or over a row list (the fixup only: *rcnt rows in rlist, as
+// marked by the split form's chain kernel): the workgroups then take list entries blockIdx.x * RPB, + gridDim.x * RPB, ... (RPB
+// rows per workgroup), so any grid covers the list.
+template <int N, int L, bool FULL, bool FIX, int FMT = SP_CSR>
+__global__ __launch_bounds__(BLK) void k_spmv(const DevConsts *__restrict__ C, int m, SpMat A, AoSRef<N> x,
+                                              AoSRef<N> y, int tpr, const int *rcnt = nullptr,
+                                              const int *rlist = nullptr) {
+    constexpr int W = BLK / L;
+    __shared__ int sd[W][N];
+    __shared__ int ss[W], se[W];
+    __shared__ double sf0[W], sf1[W];
+    __shared__ int sx0[W], sx1[W];
+    const int gl = threadIdx.x / L;
+    const int RPB = W / tpr;
+    const int lane = gl % tpr;
+    const long long count = rlist ? (long long)*rcnt : (long long)m;
+    const long long stride = rlist ? (long long)gridDim.x * RPB : count;
+    for (long long base = (long long)blockIdx.x * RPB; base < count; base += stride) {
+        const long long idx = base + gl / tpr;
+        spmv_row<N, L, FULL, FIX, FMT>(C, m, A, x, y, tpr, idx < count, rlist ? (idx < count ? rlist[idx] : 0) : idx,
+                                       lane, gl, sd, ss, se, sf0, sf1, sx0, sx1);
+    }
+}
+
+// ---------------------------------------------------------------- SpMV, split form
+// At 16 and 32 moduli a row's exact chains (every product and sum exact, no sign
+// ambiguity, no nonzero operand dropped) end in residues |sum_i s_i X_i sig_i 2^(e_i - E)|
+// mod m, where term i is x[col] times the double value (significand sig_i, exponent
+// e_i of the product), s_i the product's sign and E the row result's exponent: the
+// chains and the tree only align and add, and alignment multiplies by powers of two.
+// So the row is formed by separate kernels: k_spmv_chain runs the reference's chains
+// and tree on the scalar fields alone (one thread per chain, where the fused kernel
+// repeats the scalar work on every lane of a number) and writes y's sign, exponent and
+// evaluation, or REDO; k_spmv_resid sums the residues of the row's terms against E, in
+// any order (the sum is exact), CSR rows longer than SP_LONG terms by k_spmv_resid_long
+// with a workgroup per row. REDO rows go to the fused kernel's fixup pass.
+constexpr int SGR = 4;        // residues per thread in the residue kernels
+constexpr int SP_LONG = 64;   // CSR rows with more terms take the per-workgroup residue kernel
+constexpr int SPL_T = 256;    // threads of k_spmv_resid_long
+constexpr int SP_PF = 4;      // terms of a chain whose x fields are in flight ahead of the step
+#ifndef MPH_SP_TS
+#define MPH_SP_TS 4
+#endif
+constexpr int SP_TS = MPH_SP_TS;  // term slots per row in k_spmv_resid (1 in the one-lane CPU emulation)
+
+// sign, exponent and evaluation of x[col] (40 contiguous bytes of the record)
+struct XScal {
+    int2 se;
+    ErMem l, u;
+};
+template <int N>
+__device__ __forceinline__ XScal load_xscal(const AoSRef<N> &x, int col) {
+    const char *p = x.base + (size_t)col * AoSRef<N>::STRIDE + 4 * N;
+    return XScal{*(const int2 *)p, *(const ErMem *)(p + 8), *(const ErMem *)(p + 24)};
+}
+
+// perm (CSR): rows by decreasing length class (k_sp_bin_*), so that the chains of one
+// wavefront have similar lengths; rows marked REDO are appended to rlist (count *rcnt)
+template <int N, int FMT>
+__global__ __launch_bounds__(BLK) void k_spmv_chain(const DevConsts *__restrict__ C, int m, SpMat A, AoSRef<N> x,
+                                                    AoSRef<N> y, int tpr, const int *__restrict__ perm, int *rcnt,
+                                                    int *rlist) {
+    const long long g = (long long)blockIdx.x * BLK + threadIdx.x;
+    const long long idx = g / tpr;
+    const int lane = (int)(g % tpr);
+    const bool live = idx < m;
+    const long long row = live ? (perm ? perm[idx] : idx) : 0;
+    const ScC K = sc_consts(C);
+    Sc acc{0, 0, Er{0.0, 0}, Er{0.0, 0}};
+    bool ok = true;
+    auto step = [&](const XScal &o, double v) {
+        return sc_muld_step(K, acc, o.se.x, o.se.y, Er{o.l.frac, (int)o.l.exp}, Er{o.u.frac, (int)o.u.exp}, v);
+    };
+    if constexpr (FMT == SP_CSR) {
+        // terms i, i + tpr, ... in blocks of SP_PF: x's fields of a block are gathered one
+        // block ahead, its columns and values two blocks ahead, into fixed registers (a
+        // ring that is never shifted, so a step waits only for its own operands)
+        if (live) {
+            const int re = A.ptr[row + 1];
+            int i = A.ptr[row] + lane;
+            auto col = [&](int j) { return j < re ? A.ja[j] : 0; };
+            auto val = [&](int j) { return j < re ? A.as[j] : 0.0; };
+            XScal b[SP_PF];
+            double vb[SP_PF], vn[SP_PF];
+            int cn[SP_PF];
+#pragma unroll
+            for (int k = 0; k < SP_PF; k++) {
+                b[k] = load_xscal(x, col(i + k * tpr));
+                vb[k] = val(i + k * tpr);
+                cn[k] = col(i + (SP_PF + k) * tpr);
+                vn[k] = val(i + (SP_PF + k) * tpr);
+            }
+            for (; i < re && ok; i += SP_PF * tpr) {
+#pragma unroll
+                for (int k = 0; k < SP_PF; k++) {
+                    if (ok && i + k * tpr < re) ok = step(b[k], vb[k]);
+                    b[k] = load_xscal(x, cn[k]);
+                    vb[k] = vn[k];
+                    cn[k] = col(i + (2 * SP_PF + k) * tpr);
+                    vn[k] = val(i + (2 * SP_PF + k) * tpr);
+                }
+            }
+        }
+    } else if (live) {
+        sp_terms<FMT>(A, m, row, lane, tpr, [&](int col, double v) {
+            ok = step(load_xscal(x, col), v);
+            return ok;
+        });
+    }
+    // the 16/8/4/2/1 tree (spmv_csrv.cuh:66-80): lane l < h adds the partial of lane l + h;
+    // a row's tpr lanes are adjacent in one wavefront, and every lane takes part in the shuffles
+    for (int h = tpr >> 1; h >= 1; h >>= 1) {
+        Sc p;
+        p.sign = __shfl_down(acc.sign, h, tpr);
+        p.exp = __shfl_down(acc.exp, h, tpr);
+        p.ev0 = Er{__shfl_down(acc.ev0.frac, h, tpr), __shfl_down(acc.ev0.exp, h, tpr)};
+        p.ev1 = Er{__shfl_down(acc.ev1.frac, h, tpr), __shfl_down(acc.ev1.exp, h, tpr)};
+        const int pok = __shfl_down((int)ok, h, tpr);
+        if (lane < h) {
+            ok = ok && pok;
+            if (ok) ok = sc_add_step(K, acc, p);
+        }
+    }
+    if (!live || lane != 0) return;
+    const long long orow = sp_out_row<FMT>(A, row);
+    if (!ok) {
+        *y.sign(orow) = REDO;
+        rlist[atomicAdd(rcnt, 1)] = (int)row;
+        return;
+    }
+    *y.sign(orow) = acc.sign;
+    *y.exp(orow) = acc.exp;
+    ErMem *ev = y.eval(orow);
+    ev[0] = ErMem{acc.ev0.frac, (long long)acc.ev0.exp};
+    ev[1] = ErMem{acc.ev1.frac, (long long)acc.ev1.exp};
+}
+
+// the operands a residue thread reads of x[col]: residues g0 .. g0 + SGR - 1, sign, exponent
+struct XRes {
+    int2 se, a, b;  // sign and exponent of x; residues g0 .. g0 + 3
+};
+template <int N>
+__device__ __forceinline__ XRes load_xres(const AoSRef<N> &x, int col, int g0) {
+    const int *d = x.digits(col) + g0;
+    return XRes{*(const int2 *)x.sign(col), *(const int2 *)d, *(const int2 *)(d + 2)};
+}
+// one term's residues from its operands and the power 2^d mod m_r of its alignment: acc_r +=
+// s (x_r f_r mod m_r) (sig mod m_r) as an exact remainder
+__device__ __forceinline__ void sp_resid_add(const double *md, const double *mv, double *acc, const XRes &o, double v,
+                                             const int *f) {
+    if (v == 0) return;  // a zero product adds nothing
+    const uint64_t u = d2u(v);
+    const double sig = (double)((u & ((((uint64_t)1) << 52) - 1)) | (((uint64_t)1) << 52));
+    const double sg = (o.se.x ^ (int)(u >> 63)) ? -1.0 : 1.0;
+    const int xd[SGR] = {o.a.x, o.a.y, o.b.x, o.b.y};
+#pragma unroll
+    for (int r = 0; r < SGR; r++) {
+        const double a = (double)mulmod(xd[r], f[r], md[r], mv[r]), b = (double)modd(sig, md[r], mv[r]);
+        const double pr = a * b, pl = fma(a, b, -pr);
+        const double q = floor(pr * mv[r]);
+        acc[r] = fma(sg, fma(-q, md[r], pr) + pl, acc[r]);
+    }
+}
+// d = e - E of a term: 0 <= d <= log2 M for every nonzero term of an exact chain (its aligned
+// value is below M); a zero x may fall outside, adds nothing, and is clamped into the table
+__device__ __forceinline__ int sp_resid_shift(const XRes &o, double v, int E, int top) {
+    const int d = o.se.y + ((int)((d2u(v) >> 52) & 2047) - 1075) - E;
+    return d < 0 ? 0 : (d > top ? top : d);
+}
+// terms i0, i0 + ts, ... below re of a CSR row (rs: a valid index of the row) added to acc,
+// in blocks of PF: a block's table entries are requested before the next block's operands,
+// which are gathered while the block is added
+template <int N, int PF>
+__device__ __forceinline__ void sp_resid_slot(const DevConsts *__restrict__ C, const SpMat &A, const AoSRef<N> &x,
+                                              int g0, const double *md, const double *mv, double *acc, int i0, int re,
+                                              int rs, int ts, int E) {
+    const int top = C->log2m;
+    const int *__restrict__ p2 = C->pow2 + g0;
+    auto ld = [&](int j) { return load_xres(x, A.ja[j < re ? j : rs], g0); };
+    auto lv = [&](int j) { return j < re ? A.as[j] : 0.0; };
+    XRes b[PF];
+    double vb[PF];
+#pragma unroll
+    for (int k = 0; k < PF; k++) {
+        b[k] = ld(i0 + k * ts);
+        vb[k] = lv(i0 + k * ts);
+    }
+    for (int i = i0; i < re; i += PF * ts) {
+        int f[PF][SGR];
+#pragma unroll
+        for (int k = 0; k < PF; k++) {
+            const int2 *t = (const int2 *)(p2 + (size_t)sp_resid_shift(b[k], vb[k], E, top) * N);
+            const int2 t0 = t[0], t1 = t[1];
+            f[k][0] = t0.x;
+            f[k][1] = t0.y;
+            f[k][2] = t1.x;
+            f[k][3] = t1.y;
+        }
+        XRes nb[PF];
+        double nv[PF];
+#pragma unroll
+        for (int k = 0; k < PF; k++) {
+            nb[k] = ld(i + (PF + k) * ts);
+            nv[k] = lv(i + (PF + k) * ts);
+        }
+#pragma unroll
+        for (int k = 0; k < PF; k++)
+            if (i + k * ts < re) sp_resid_add(md, mv, acc, b[k], vb[k], f[k]);
+#pragma unroll
+        for (int k = 0; k < PF; k++) {
+            b[k] = nb[k];
+            vb[k] = nv[k];
+        }
+    }
+}
+
+// Residues of the rows k_spmv_chain finished: SGR residues per thread, N / SGR residue
+// groups by SP_TS term slots per row (slot t takes terms t, t + SP_TS, ...; the slots'
+// sums meet through shuffles), rows in perm's order (CSR). With a long-row list (CSR), a
+// row of more than SP_LONG terms is appended to it (count *lcnt) and left to
+// k_spmv_resid_long.
+template <int N, int FMT>
+__global__ __launch_bounds__(BLK) void k_spmv_resid(const DevConsts *__restrict__ C, int m, SpMat A, AoSRef<N> x,
+                                                    AoSRef<N> y, const int *__restrict__ perm, int *lcnt,
+                                                    int *llist) {
+    static_assert(N % SGR == 0 && SGR == 4, "residue groups");
+    constexpr int G = N / SGR, TPR = G * SP_TS;
+    static_assert(64 % TPR == 0, "a row's threads in one wavefront");
+    const long long g = (long long)blockIdx.x * BLK + threadIdx.x;
+    const long long idx = g / TPR;
+    const int q = (int)(g % G), sl = (int)((g / G) % SP_TS), g0 = q * SGR;
+    bool live = idx < m;
+    const long long row = live ? (perm ? perm[idx] : idx) : 0;
+    const long long orow = live ? sp_out_row<FMT>(A, row) : 0;
+    const int ysign = live ? *y.sign(orow) : REDO;
+    live = live && ysign != REDO;
+    double md[SGR], mv[SGR], acc[SGR];
+#pragma unroll
+    for (int r = 0; r < SGR; r++) {
+        md[r] = C->md[g0 + r];
+        mv[r] = C->minv[g0 + r];
+        acc[r] = 0.0;
+    }
+    if constexpr (FMT == SP_CSR) {
+        const int rs = live ? A.ptr[row] : 0, re = live ? A.ptr[row + 1] : 0;
+        if (llist && re - rs > SP_LONG) {
+            if (g0 == 0 && sl == 0) llist[atomicAdd(lcnt, 1)] = (int)row;
+            live = false;
+        }
+        // at most SP_LONG terms (fewer than 2^20: no intermediate reduction); the next
+        // two terms' operands are in flight while one is added
+        if (live && rs + sl < re) {  // a slot with terms: rs is then a valid index
+            const int E = *y.exp(orow);
+            sp_resid_slot<N, 2>(C, A, x, g0, md, mv, acc, rs + sl, re, rs, SP_TS, E);
+        }
+    } else if (live) {
+        const int E = *y.exp(orow);
+        int since = 0, t = 0;
+        sp_terms<FMT>(A, m, row, 0, 1, [&](int col, double v) {
+            if (t++ % SP_TS != sl) return true;
+            const XRes o = load_xres(x, col, g0);
+            const int2 *t = (const int2 *)(C->pow2 + (size_t)sp_resid_shift(o, v, E, C->log2m) * N + g0);
+            const int f[SGR] = {t[0].x, t[0].y, t[1].x, t[1].y};
+            sp_resid_add(md, mv, acc, o, v, f);
+            if (++since == GV_REDUCE) {
+                since = 0;
+#pragma unroll
+                for (int r = 0; r < SGR; r++) acc[r] = fmod_exact(acc[r], md[r], mv[r]);
+            }
+            return true;
+        });
+    }
+    // the slots' sums (reduced below m, so SP_TS of them add exactly) through shuffles
+    double w[SGR];
+#pragma unroll
+    for (int r = 0; r < SGR; r++) w[r] = fmod_exact(acc[r], md[r], mv[r]);
+#pragma unroll
+    for (int h = G; h < TPR; h <<= 1)
+#pragma unroll
+        for (int r = 0; r < SGR; r++) w[r] += __shfl_xor(w[r], h, 64);
+    if (!live || sl != 0) return;
+    int *dig = y.digits(orow) + g0;
+#pragma unroll
+    for (int r = 0; r < SGR; r++) {
+        const int v = (int)fmod_exact(w[r], md[r], mv[r]);
+        dig[r] = (ysign && v) ? C->m[g0 + r] - v : v;
+    }
+}
+
+// Residues of the long CSR rows listed by k_spmv_resid: one workgroup per row, SPL_T / (N /
+// SGR) term slots of N / SGR threads; the slots' reduced sums meet in LDS. Workgroups
+// take rows blockIdx.x, + gridDim.x, ... of the list, so any grid covers it.
+template <int N>
+__global__ __launch_bounds__(SPL_T) void k_spmv_resid_long(const DevConsts *__restrict__ C, SpMat A, AoSRef<N> x,
+                                                           AoSRef<N> y, const int *__restrict__ lcnt,
+                                                           const int *__restrict__ llist) {
+    constexpr int G = N / SGR, TS = SPL_T / G;
+    __shared__ int part[TS][N + 1];
+    const int q = threadIdx.x % G, sl = threadIdx.x / G, g0 = q * SGR;
+    double md[SGR], mv[SGR];
+#pragma unroll
+    for (int r = 0; r < SGR; r++) {
+        md[r] = C->md[g0 + r];
+        mv[r] = C->minv[g0 + r];
+    }
+    const int count = *lcnt;
+    for (int k = blockIdx.x; k < count; k += gridDim.x) {
+        const int row = llist[k];
+        const int E = *y.exp(row);
+        const int rs = A.ptr[row], re = A.ptr[row + 1];
+        double acc[SGR] = {0.0, 0.0, 0.0, 0.0};
+        // a slot's terms in blocks of at most GV_REDUCE, reduced between blocks
+        for (int i = rs + sl; i < re; i += GV_REDUCE * TS) {
+            const long long e = (long long)i + (long long)GV_REDUCE * TS;
+            sp_resid_slot<N, 4>(C, A, x, g0, md, mv, acc, i, e < re ? (int)e : re, rs, TS, E);
+#pragma unroll
+            for (int r = 0; r < SGR; r++) acc[r] = fmod_exact(acc[r], md[r], mv[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < SGR; r++) part[sl][g0 + r] = (int)fmod_exact(acc[r], md[r], mv[r]);
+        __syncthreads();
+        if (threadIdx.x < N) {
+            const int r = threadIdx.x;
+            double s = 0.0;  // TS sums below 2^31: exact
+            for (int t = 0; t < TS; t++) s += (double)part[t][r];
+            const int w = (int)fmod_exact(s, C->md[r], C->minv[r]);
+            y.digits(row)[r] = (*y.sign(row) && w) ? C->m[r] - w : w;
+        }
+        __syncthreads();
+    }
+}
+
+// ---- rows of a CSR matrix grouped by length class, longest first: class c holds rows of
+// 2^(SP_NB - 2 - c) .. 2^(SP_NB - 1 - c) - 1 terms (class 0: all longer, class SP_NB - 1:
+// empty rows). A workgroup counts its rows per class in LDS and reserves its ranges
+// with one atomic per class; order within a class is arbitrary (each row's own terms
+// keep the reference's order).
+__device__ __forceinline__ int sp_class(int len) {
+    const int b = len > 0 ? 32 - __clz(len) : 0;  // bit length
+    const int c = SP_NB - 1 - b;
+    return c < 0 ? 0 : c;
+}
+// hist[SP_NB]: rows per class. A workgroup covers SP_RPT * BLK rows (thread t: rows
+// t, t + BLK, ...), so that few workgroups contend for the global counters.
+constexpr int SP_RPT = 16;
+template <int N, int L>
+__global__ __launch_bounds__(BLK) void k_sp_bin_count(int m, const int *__restrict__ irp, int *hist) {
+    __shared__ int h[SP_NB];
+    if (threadIdx.x < SP_NB) h[threadIdx.x] = 0;
+    __syncthreads();
+    const long long r0 = (long long)blockIdx.x * BLK * SP_RPT + threadIdx.x;
+    for (int k = 0; k < SP_RPT; k++) {
+        const long long row = r0 + (long long)k * BLK;
+        if (row < m) atomicAdd(&h[sp_class(irp[row + 1] - irp[row])], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < SP_NB && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+// off[c] = rows of the classes before c (one thread)
+template <int N, int L>
+__global__ void k_sp_bin_scan(const int *hist, int *off) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int a = 0;
+    for (int c = 0; c < SP_NB; c++) {
+        off[c] = a;
+        a += hist[c];
+    }
+}
+// perm[off[c] + cursor] = row; cur[SP_NB] starts at zero
+template <int N, int L>
+__global__ __launch_bounds__(BLK) void k_sp_bin_scatter(int m, const int *__restrict__ irp, const int *off, int *cur,
+                                                        int *perm) {
+    __shared__ int h[SP_NB], base[SP_NB];
+    if (threadIdx.x < SP_NB) h[threadIdx.x] = 0;
+    __syncthreads();
+    const long long r0 = (long long)blockIdx.x * BLK * SP_RPT + threadIdx.x;
+    int cls[SP_RPT], rank[SP_RPT];
+#pragma unroll
+    for (int k = 0; k < SP_RPT; k++) {
+        const long long row = r0 + (long long)k * BLK;
+        cls[k] = 0;
+        rank[k] = 0;
+        if (row < m) {
+            cls[k] = sp_class(irp[row + 1] - irp[row]);
+            rank[k] = atomicAdd(&h[cls[k]], 1);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < SP_NB && h[threadIdx.x]) base[threadIdx.x] = off[threadIdx.x] + atomicAdd(&cur[threadIdx.x], h[threadIdx.x]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SP_RPT; k++) {
+        const long long row = r0 + (long long)k * BLK;
+        if (row < m) perm[base[cls[k]] + rank[k]] = (int)row;
+    }
+}
 }  // namespace mph

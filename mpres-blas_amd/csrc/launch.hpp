@@ -403,26 +403,54 @@ struct Launch {
                            EL, BL, EH + m, BH + m, EL + m, BL + m, alpha, beta, Cm, ldc);
         return hipGetLastError();
     }
+    // pass 0: the row results, or REDO marks (16 and 32 moduli: the split form: CSR rows
+    // grouped by length, scalar chains, residues of short and of long rows); pass 1: the
+    // marked rows recomputed by the fused kernel. ws: spmv_ws_ints(m) ints (split form):
+    // long-row and marked-row counts, class counts, offsets and cursors, then the row
+    // order, the long-row list and the marked-row list.
     template <int FMT>
     static void spmv_fmt(const DevConsts *c, int pass, int m, const SpMat &A, const char *x, char *y, int tpr,
-                         hipStream_t st) {
+                         int *ws, hipStream_t st) {
         const dim3 gr(grid_for((long long)m * tpr, L)), bl(BLK);
-        if (pass == 0)
-            hipLaunchKernelGGL((k_spmv<N, L, !SPLIT, false, FMT>), gr, bl, 0, st, c, m, A, AoSRef<N>{(char *)x},
-                               AoSRef<N>{y}, tpr);
-        else if constexpr (SPLIT)
-            hipLaunchKernelGGL((k_spmv<N, L, true, true, FMT>), gr, bl, 0, st, c, m, A, AoSRef<N>{(char *)x},
-                               AoSRef<N>{y}, tpr);
+        const AoSRef<N> X{(char *)x}, Y{y};
+        int *lcnt = ws, *rcnt = ws + 1, *hist = ws + 2, *off = hist + SP_NB, *cur = off + SP_NB;
+        int *perm = cur + SP_NB, *llist = perm + m, *rlist = llist + m;
+        if (pass == 0 && SPLIT) {
+            hipLaunchKernelGGL((k_fill_int<N, L>), dim3(1), bl, 0, st, ws, 2 + 3 * SP_NB, 0);
+            const int *pm = nullptr;
+            if (FMT == SP_CSR) {
+                const dim3 gm((unsigned)(((long long)m + (long long)BLK * SP_RPT - 1) / ((long long)BLK * SP_RPT)));
+                hipLaunchKernelGGL((k_sp_bin_count<N, L>), gm, bl, 0, st, m, A.ptr, hist);
+                hipLaunchKernelGGL((k_sp_bin_scan<N, L>), dim3(1), dim3(64), 0, st, (const int *)hist, off);
+                hipLaunchKernelGGL((k_sp_bin_scatter<N, L>), gm, bl, 0, st, m, A.ptr, (const int *)off, cur, perm);
+                pm = perm;
+            }
+            hipLaunchKernelGGL((k_spmv_chain<N, FMT>), dim3(grid_for((long long)m * tpr, 1)), bl, 0, st, c, m, A, X,
+                               Y, tpr, pm, rcnt, rlist);
+            hipLaunchKernelGGL((k_spmv_resid<N, FMT>), dim3(grid_for((long long)m * (N / SGR) * SP_TS, 1)), bl, 0, st, c, m,
+                               A, X, Y, pm, lcnt, FMT == SP_CSR ? llist : nullptr);
+            if (FMT == SP_CSR)
+                hipLaunchKernelGGL((k_spmv_resid_long<N>), dim3(m < 4096 ? m : 4096), dim3(SPL_T), 0, st, c, A, X, Y,
+                                   (const int *)lcnt, (const int *)llist);
+        } else if (pass == 0) {
+            hipLaunchKernelGGL((k_spmv<N, L, !SPLIT, false, FMT>), gr, bl, 0, st, c, m, A, X, Y, tpr, nullptr,
+                               nullptr);
+        } else if constexpr (SPLIT) {
+            const unsigned g1 = grid_for((long long)m * tpr, L);
+            hipLaunchKernelGGL((k_spmv<N, L, true, true, FMT>), dim3(g1 < 2048 ? g1 : 2048), bl, 0, st, c, m, A, X, Y,
+                               tpr, (const int *)rcnt, (const int *)rlist);
+        }
     }
     static hipError_t spmv(const DevConsts *c, int pass, int fmt, int m, SpMat A, const char *x, char *y, int tpr,
-                           hipStream_t st) {
+                           int *ws, hipStream_t st) {
         if (m <= 0) return hipSuccess;
         if (tpr > BLK / L) return hipErrorInvalidValue;
+        if (SPLIT && !ws) return hipErrorInvalidValue;
         switch (fmt) {
-            case SP_CSR: spmv_fmt<SP_CSR>(c, pass, m, A, x, y, tpr, st); break;
-            case SP_ELL: spmv_fmt<SP_ELL>(c, pass, m, A, x, y, tpr, st); break;
-            case SP_JAD: spmv_fmt<SP_JAD>(c, pass, m, A, x, y, tpr, st); break;
-            default: spmv_fmt<SP_DIA>(c, pass, m, A, x, y, tpr, st); break;
+            case SP_CSR: spmv_fmt<SP_CSR>(c, pass, m, A, x, y, tpr, ws, st); break;
+            case SP_ELL: spmv_fmt<SP_ELL>(c, pass, m, A, x, y, tpr, ws, st); break;
+            case SP_JAD: spmv_fmt<SP_JAD>(c, pass, m, A, x, y, tpr, ws, st); break;
+            default: spmv_fmt<SP_DIA>(c, pass, m, A, x, y, tpr, ws, st); break;
         }
         return hipGetLastError();
     }

@@ -158,6 +158,15 @@ MPH_HD double mul_ru(double a, double b) {
     return (e > 0) ? step_up(p) : p;
 #endif
 }
+// lo = a * b rounded down, hi = c * d rounded up
+MPH_HD void mul_rd_ru(double a, double b, double c, double d, double &lo, double &hi) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    hw_mul_pair(a, b, c, d, lo, hi);
+#else
+    lo = mul_rd(a, b);
+    hi = mul_ru(c, d);
+#endif
+}
 MPH_HD double div_rd(double a, double b) {
     double q = a / b;
     double r = fma(-q, b, a);

@@ -25,6 +25,7 @@ struct SoA {
 
 // a sparse matrix in one of the storage formats of the SpMV kernels (kernels.hpp)
 enum { SP_CSR = 0, SP_ELL = 1, SP_JAD = 2, SP_DIA = 3 };
+constexpr int SP_NB = 16;  // length classes of the split-form SpMV's row grouping (kernels.hpp)
 struct SpMat {
     const double *as;
     const int *ja;      // CSR, ELL, JAD: column indices
@@ -89,7 +90,7 @@ struct Ops {
                             int, char *, hipStream_t);
     size_t (*gemm_mfma_ws)(int, int, int, int);
     // SpMV over a storage format (SP_CSR, SP_ELL, SP_JAD, SP_DIA of kernels.hpp): pass 0 main, 1 fixup
-    hipError_t (*spmv)(const DevConsts *, int, int, int, SpMat, const char *, char *, int, hipStream_t);
+    hipError_t (*spmv)(const DevConsts *, int, int, int, SpMat, const char *, char *, int, int *, hipStream_t);
     // the rest of the BLAS surface (kernels_blas.hpp)
     hipError_t (*v1map)(const DevConsts *, int, int, int, int, SoA, int, SoA, int, int, SoA, int, int, SoA, int, int,
                         hipStream_t);

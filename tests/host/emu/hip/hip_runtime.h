@@ -38,7 +38,7 @@ inline int __ffs(int v) { return __builtin_ffs(v); }
 struct EmuIdx {
     unsigned x, y, z;
 };
-inline thread_local EmuIdx blockIdx, threadIdx;
+inline thread_local EmuIdx blockIdx, threadIdx, gridDim{1, 1, 1};
 inline void (*emu_barrier)() = nullptr;
 
 inline unsigned __lane_id() { return threadIdx.x & 63u; }
@@ -46,6 +46,8 @@ template <class T>
 inline T __shfl_xor(T v, int, int = 64) { return v; }
 template <class T>
 inline T __shfl(T v, int, int = 64) { return v; }
+template <class T>
+inline T __shfl_down(T v, unsigned, int = 64) { return v; }
 inline int atomicMin(int *p, int v) {
     const int o = *p;
     if (v < o) *p = v;

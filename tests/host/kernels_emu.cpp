@@ -64,6 +64,25 @@ void run_wg(long long groups, F &&f) {
     emu_barrier = nullptr;
 }
 
+// one workgroup (block b of nb, T threads) of a kernel that synchronises, its threads as
+// concurrent host threads with a barrier behind __syncthreads
+template <class F>
+void run_block(unsigned b, unsigned nb, unsigned T, F &&f) {
+    emu_barrier = bar_wait;
+    pthread_barrier_init(&g_bar, nullptr, T);
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; t++)
+        th.emplace_back([&, t] {
+            blockIdx = EmuIdx{b, 0, 0};
+            threadIdx = EmuIdx{t, 0, 0};
+            gridDim = EmuIdx{nb, 1, 1};
+            f();
+        });
+    for (auto &t : th) t.join();
+    pthread_barrier_destroy(&g_bar);
+    emu_barrier = nullptr;
+}
+
 template <int N>
 int gemm_v2_n(const Ctx &x, int ta, int tb, int m, int n, int k, char *alpha, char *A, int lda, char *B, int ldb,
               char *beta, char *Cm, int ldc, char *ws, int split) {
@@ -161,9 +180,47 @@ int dot_v2_pass1_n(const Ctx &x, long long cnt, long long nvt, char *X, char *Y,
     return 0;
 }
 
+// CSR SpMV of the 16/32-moduli split form with one lane per row (the tree has no
+// levels): scalar chains, residues, then the fused kernel's fixup of marked rows
+template <int N>
+int spmv_split_n(const Ctx &x, int m, const int *irp, const int *ja, const double *as, char *X, char *Y,
+                 int *counts) {
+    const SpMat A{as, ja, irp, nullptr, nullptr, 0, 0};
+    const AoSRef<N> xa{X}, ya{Y};
+    std::vector<int> ws(2 + 3 * SP_NB + 3 * (size_t)m, 0);
+    int *lcnt = ws.data(), *rcnt = lcnt + 1, *hist = lcnt + 2, *off = hist + SP_NB, *cur = off + SP_NB;
+    int *perm = cur + SP_NB, *llist = perm + m, *rlist = llist + m;
+    const unsigned gm = (unsigned)((m + BLK * SP_RPT - 1) / (BLK * SP_RPT));
+    for (unsigned b = 0; b < gm; b++) run_block(b, gm, BLK, [&] { k_sp_bin_count<N, 1>(m, irp, hist); });
+    run(1, [&] { k_sp_bin_scan<N, 1>(hist, off); });
+    for (unsigned b = 0; b < gm; b++) run_block(b, gm, BLK, [&] { k_sp_bin_scatter<N, 1>(m, irp, off, cur, perm); });
+    run(m, [&] { k_spmv_chain<N, SP_CSR>(&x.c, m, A, xa, ya, 1, perm, rcnt, rlist); });
+    run((long long)m * (N / SGR) * SP_TS, [&] { k_spmv_resid<N, SP_CSR>(&x.c, m, A, xa, ya, perm, lcnt, llist); });
+    // the long-row kernel's workgroups of SPL_T threads, one after another
+    const int nb = *lcnt < 3 ? *lcnt : 3;  // fewer workgroups than rows: each takes several
+    for (int b = 0; b < nb; b++) run_block(b, nb, SPL_T, [&] { k_spmv_resid_long<N>(&x.c, A, xa, ya, lcnt, llist); });
+    // the fixup over the marked rows, two workgroups (each takes several list entries)
+    for (unsigned b = 0; b < 2; b++) run_block(b, 2, BLK, [&] { k_spmv<N, 1, true, true, SP_CSR>(&x.c, m, A, xa, ya, 1, rcnt, rlist); });
+    counts[0] = *lcnt;
+    counts[1] = *rcnt;
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+// counts: the long-row and marked-row list lengths
+int emu_spmv_split(int set_id, int m, const int *irp, const int *ja, const double *as, void *X, void *Y, int *counts) {
+    Ctx x;
+    if (!make_ctx(set_id, x)) return -1;
+    switch (x.c.n) {
+        case 16: return spmv_split_n<16>(x, m, irp, ja, as, (char *)X, (char *)Y, counts);
+        case 32: return spmv_split_n<32>(x, m, irp, ja, as, (char *)X, (char *)Y, counts);
+    }
+    return -1;
+}
+
 
 // v2 gemm (AoS) on the CPU; ws holds m*n mp_float_t; split = 1: fast kernel + fixup epilogue
 int emu_gemm_v2(int set_id, int ta, int tb, int m, int n, int k, void *alpha, void *A, int lda, void *B, int ldb,

@@ -31,7 +31,8 @@ BUILDS = {
 @pytest.fixture(scope="module", params=[b for b in BUILDS if b == "g++" or os.path.exists(CLANG)])
 def emu(request, tmp_path_factory):
     out = tmp_path_factory.mktemp("emu") / "libkemu.so"
-    cmd = BUILDS[request.param] + ["-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-pthread",
+    # one lane per thread: the SpMV residue kernel's term slots (combined by shuffles) become one
+    cmd = BUILDS[request.param] + ["-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-pthread", "-DMPH_SP_TS=1",
            "-I", os.path.join(ROOT, "tests", "host", "emu"),
            os.path.join(ROOT, "tests", "host", "kernels_emu.cpp"),
            os.path.join(ROOT, "mpres-blas_amd", "csrc", "consts.cpp"), "-o", str(out)]
@@ -120,3 +121,34 @@ def test_gemm_v2_split_form(emu, name, lo, hi, ta, tb, cancel):
     got, ws = Cm.copy(), o.empty(m * n)
     assert emu.emu_gemm_v2_split(sid, ta, tb, m, n, k, P(alpha), P(A), lda, P(B), ldb, P(beta), P(got), m, P(ws)) == 0
     assert got.tobytes() == want.tobytes()
+
+
+@pytest.mark.parametrize("name,lo,hi", [("16_4double", -30, 30), ("16_4double", -200, 200), ("32_8double", -30, 30),
+                                        ("32_8double", -400, 400), ("16_32bit", -40, 40)])
+def test_spmv_split_form(emu, name, lo, hi):
+    """Split-form CSR SpMV (scalar chains, residues against the row exponent, fixup of the
+    marked rows) on a power-law matrix with zero values, zero x entries, cancelling
+    pairs and wide exponents, against the oracle's CSR kernel"""
+    o = po.Oracle(name)
+    sid = po.set_names().index(name)
+    m = 400
+    irp, ja, as_ = gen.powerlaw_csr(81, m, kmax=300)
+    assert (np.diff(irp) > 64).sum() >= 5  # rows for the long-row residue kernel
+    as_[::13] = 0.0
+    xs = gen.uexp(82, m, lo, hi)
+    xs[::17] = 0.0
+    # rows whose first two terms cancel exactly
+    for r in range(0, m, 9):
+        a, b = irp[r], irp[r + 1]
+        if b - a >= 2:
+            ja[a + 1] = ja[a]
+            as_[a + 1] = -as_[a]
+    x = o.set_d(xs, D)
+    want = o.spmv(m, irp, ja, as_, x, 1, D)
+    got = o.empty(m)
+    counts = np.zeros(2, np.int32)
+    assert emu.emu_spmv_split(sid, m, P(np.ascontiguousarray(irp, np.int32)), P(np.ascontiguousarray(ja, np.int32)),
+                              P(np.ascontiguousarray(as_)), P(x), P(got), P(counts)) == 0
+    assert got.tobytes() == want.tobytes(), name
+    # the cancelling pairs reach the fixup; with narrow exponents long rows reach their kernel
+    assert counts[1] > 0 and (hi > 100 or counts[0] > 0), counts
