@@ -377,24 +377,61 @@ __global__ __launch_bounds__(BLK) void k_fold(const DevConsts *__restrict__ C, l
     const long long v = group_id<L>();
     if (v >= nvt) return;
     if (FIX && *out.sign(v) != REDO) return;
+    constexpr int R = N / L;
     Grp<N, L> g(C);
-    MP<N / L> acc, p;
+    MP<R> acc, p;
     g.zero(acc);
     bool ok = true;
-    for (long long i = v; i < cnt; i += nvt) {
-        if (MODE == 0) {
-            ok = g.template fma_step<FULL>(acc, xs, bidx(i, cnt, incx), ys, bidx(i, cnt, incy));
-        } else if (MODE == 1) {
-            ok = g.template fma_step<FULL>(acc, xa, i, ya, i);
-        } else if (MODE == 3) {
-            load_soa(g, p, xs, i * incx);
-            p.sign = 0;
-            g.add(acc, acc, p);
-        } else {
-            load_aos(g, p, xa, i);
+    if constexpr (MODE == 0 || MODE == 1) {
+        // products x_i * y_i: the operands of term i + nvt are loaded while step i runs, so
+        // the chain's steps do not wait on memory (Grp::fma_step's arithmetic, unchanged)
+        struct Op {
+            int xs, xe, ys, ye;
+            Er x0, x1, y0, y1;
+            int dx[R], dy[R];
+        };
+        auto load = [&](long long i) {
+            Op o;
+            const int *px, *py;
+            if constexpr (MODE == 0) {
+                const long long ix = bidx(i, cnt, incx), iy = bidx(i, cnt, incy);
+                scalars_of<N>(xs, ix, o.xs, o.xe, o.x0, o.x1);
+                scalars_of<N>(ys, iy, o.ys, o.ye, o.y0, o.y1);
+                px = digits_of<N>(xs, ix);
+                py = digits_of<N>(ys, iy);
+            } else {
+                scalars_of<N>(xa, i, o.xs, o.xe, o.x0, o.x1);
+                scalars_of<N>(ya, i, o.ys, o.ye, o.y0, o.y1);
+                px = digits_of<N>(xa, i);
+                py = digits_of<N>(ya, i);
+            }
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                o.dx[r] = px[g.ix(r)];
+                o.dy[r] = py[g.ix(r)];
+            }
+            return o;
+        };
+        if (v < cnt) {
+            Op cur = load(v);
+            for (long long i = v; i < cnt; i += nvt) {
+                const Op nxt = load(i + nvt < cnt ? i + nvt : i);
+                g.mul_scalars(p, cur.xs, cur.xe, cur.x0, cur.x1, cur.ys, cur.ye, cur.y0, cur.y1);
+                ok = g.template step_tail<FULL>(acc, p, cur.dx, cur.dy, 0.0, false);
+                if (!ok) break;
+                cur = nxt;
+            }
+        }
+    } else {
+        for (long long i = v; i < cnt; i += nvt) {
+            if (MODE == 3) {
+                load_soa(g, p, xs, i * incx);
+                p.sign = 0;
+            } else {
+                load_aos(g, p, xa, i);
+            }
             g.add(acc, acc, p);
         }
-        if (!ok) break;
     }
     if (ok)
         store_aos(g, acc, out, v);
