@@ -141,6 +141,30 @@ MPH_HD void big_mul_add_u32(Big<W> &a, uint32_t m, uint32_t c) {
     }
 }
 // a mod m for a modulus below 2^31
+// a * b, exactly
+template <int A, int B>
+MPH_HD Big<A + B> big_mul(const Big<A> &a, const Big<B> &b) {
+    Big<A + B> r;
+    big_zero(r);
+    for (int i = 0; i < A; i++) {
+        uint64_t c = 0;
+        for (int j = 0; j < B; j++) {
+            const uint64_t t = (uint64_t)a.l[i] * b.l[j] + r.l[i + j] + c;
+            r.l[i + j] = (uint32_t)t;
+            c = t >> 32;
+        }
+        r.l[i + B] = (uint32_t)c;
+    }
+    return r;
+}
+// a in a wider (or equal) type
+template <int WB, int WA>
+MPH_HD Big<WB> big_widen(const Big<WA> &a) {
+    static_assert(WB >= WA, "widen");
+    Big<WB> r;
+    for (int i = 0; i < WB; i++) r.l[i] = i < WA ? a.l[i] : 0u;
+    return r;
+}
 template <int W>
 MPH_HD uint32_t big_mod_u32(const Big<W> &a, uint32_t m) {
     uint64_t r = 0;

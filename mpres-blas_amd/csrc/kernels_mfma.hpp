@@ -339,21 +339,14 @@ __device__ __forceinline__ void mf_pair_value(const Grp<N, L> &g, MP<N / L> &v, 
 // row and column (no low bands) this is the single exact sum rounded once.
 // Band arrays: E* exponents, B* bit bounds, *H high and *L low band, *A rows, *B columns.
 template <int N, int L>
-__global__ __launch_bounds__(BLK) void k_mf_finish(const DevConsts *__restrict__ C_, int m, int n, int lk,
-                                                   const int *__restrict__ S, const int *__restrict__ EHA,
-                                                   const int *__restrict__ BHA, const int *__restrict__ ELA,
-                                                   const int *__restrict__ BLA, const int *__restrict__ EHB,
-                                                   const int *__restrict__ BHB, const int *__restrict__ ELB,
-                                                   const int *__restrict__ BLB, SoA alpha, SoA beta, SoA Cm, int ldc) {
-    alpha = rd(alpha);
-    beta = rd(beta);
-    Cm = rd(Cm);
+__device__ __forceinline__ void mf_finish_rounded(const Grp<N, L> &g, long long e, long long mn, int row, int col,
+                                                  int lk, const int *__restrict__ S, const int *__restrict__ EHA,
+                                                  const int *__restrict__ BHA, const int *__restrict__ ELA,
+                                                  const int *__restrict__ BLA, const int *__restrict__ EHB,
+                                                  const int *__restrict__ BHB, const int *__restrict__ ELB,
+                                                  const int *__restrict__ BLB, const SoA &alpha, const SoA &beta,
+                                                  const SoA &Cm, int ldc) {
     constexpr int R = N / L;
-    const long long e = group_id<L>();
-    const long long mn = (long long)m * n;
-    if (e >= mn) return;
-    const int row = (int)(e % m), col = (int)(e / m);
-    Grp<N, L> g(C_);
     MP<R> sum, v, a, b, cv, s2, c2, res;
     const bool la = BLA[row] != MF_NONE, lb = BLB[col] != MF_NONE;
     bool first = true;
@@ -377,6 +370,173 @@ __global__ __launch_bounds__(BLK) void k_mf_finish(const DevConsts *__restrict__
     g.mul(s2, sum, a);
     g.mul(c2, cv, b);
     g.add(res, s2, c2);
+    store_soa(g, res, Cm, ic);
+}
+template <int N, int L>
+__global__ __launch_bounds__(BLK) void k_mf_finish(const DevConsts *__restrict__ C_, int m, int n, int lk,
+                                                   const int *__restrict__ S, const int *__restrict__ EHA,
+                                                   const int *__restrict__ BHA, const int *__restrict__ ELA,
+                                                   const int *__restrict__ BLA, const int *__restrict__ EHB,
+                                                   const int *__restrict__ BHB, const int *__restrict__ ELB,
+                                                   const int *__restrict__ BLB, SoA alpha, SoA beta, SoA Cm, int ldc) {
+    alpha = rd(alpha);
+    beta = rd(beta);
+    Cm = rd(Cm);
+    const long long e = group_id<L>();
+    const long long mn = (long long)m * n;
+    if (e >= mn) return;
+    Grp<N, L> g(C_);
+    mf_finish_rounded<N, L>(g, e, mn, (int)(e % m), (int)(e / m), lk, S, EHA, BHA, ELA, BLA, EHB, BHB, ELB, BLB,
+                            alpha, beta, Cm, ldc);
+}
+
+// alpha * (band-pair sums) + beta * C of one entry, exactly in an accumulator of W limbs that
+// holds every live term aligned at emin, rounded once (see k_mf_finish_exact)
+template <int N, int W>
+__device__ __forceinline__ void mf_exact_sum(const Grp<N, 1> &g, const DevConsts *__restrict__ C_, long long e,
+                                             long long mn, const int *__restrict__ S, const MP<N> &a, const MP<N> &b,
+                                             const MP<N> &cv, const bool *live, const int *ex, int emin, MP<N> &res) {
+    constexpr int WX = big_limbs<N>();
+    static_assert(2 * WX + 1 <= W, "accumulator");
+    BigN<N> Mb;
+    big_zero(Mb);
+    Mb.l[0] = 1u;
+#pragma unroll
+    for (int i = 0; i < N; i++) big_mul_add_u32(Mb, (uint32_t)C_->m[i], 0u);
+    const BigN<N> Xa = residues_to_big<N>(C_, a.d);
+    Big<W> P, Q;  // the positive and the negative terms, aligned at emin
+    big_zero(P);
+    big_zero(Q);
+#pragma unroll 1
+    for (int t = 0; t < 5; t++) {
+        if (!live[t]) continue;
+        BigN<N> X, Y;
+        int sg;
+        if (t < 4) {
+            // the band pair's exact sum: residues of the signed sum, |sum| < M / 4
+            int d[N];
+#pragma unroll
+            for (int i = 0; i < N; i++) d[i] = S[(size_t)t * N * mn + (size_t)i * mn + e];
+            X = residues_to_big<N>(C_, d);
+            sg = big_cmp(big_shl(X, 1), Mb) >= 0;
+            if (sg) {
+                BigN<N> t2 = Mb;
+                big_sub(t2, X);
+                X = t2;
+            }
+            sg ^= a.sign;
+            Y = Xa;
+        } else {
+            X = residues_to_big<N>(C_, cv.d);
+            Y = residues_to_big<N>(C_, b.d);
+            sg = cv.sign ^ b.sign;
+        }
+        const Big<W> w = big_shl(big_widen<W>(big_mul(X, Y)), ex[t] - emin);
+        if (sg)
+            big_add(Q, w);
+        else
+            big_add(P, w);
+    }
+    int sign = 0;
+    if (big_cmp(P, Q) < 0) {
+        big_sub(Q, P);
+        P = Q;
+        sign = 1;
+    } else {
+        big_sub(P, Q);
+    }
+    if (big_is_zero(P)) {
+        g.zero(res);
+        return;
+    }
+    int exp = emin;
+    big_round(P, exp, C_->mp_precision);
+    const int tz = big_ctz(P);
+    P = big_shr(P, tz);
+    exp += tz;
+#pragma unroll
+    for (int i = 0; i < N; i++) res.d[i] = (int)big_mod_u32(P, (uint32_t)C_->m[i]);
+    res.sign = sign;
+    res.exp = exp;
+    g.eval_full(res.d, res.ev0, res.ev1);
+}
+
+// The finish at 8 moduli with one rounding per entry: alpha * (the exact band-pair sums)
+// + beta * C is formed exactly in integers (one thread per entry) and rounded once to
+// MP_PRECISION bits, to nearest (ties to even); the result is that significand without
+// trailing zeros, its residues and the full interval evaluation. An entry whose terms
+// span more than the accumulator holds takes the rounded finish above.
+constexpr int MF_XS = 24, MF_XW = 48;  // 32-bit limbs of the exact accumulation (common, widest)
+template <int N, int L>
+__global__ __launch_bounds__(BLK) void k_mf_finish_exact(const DevConsts *__restrict__ C_, int m, int n, int lk,
+                                                         const int *__restrict__ S, const int *__restrict__ EHA,
+                                                         const int *__restrict__ BHA, const int *__restrict__ ELA,
+                                                         const int *__restrict__ BLA, const int *__restrict__ EHB,
+                                                         const int *__restrict__ BHB, const int *__restrict__ ELB,
+                                                         const int *__restrict__ BLB, SoA alpha, SoA beta, SoA Cm,
+                                                         int ldc) {
+    alpha = rd(alpha);
+    beta = rd(beta);
+    Cm = rd(Cm);
+    const long long e = (long long)blockIdx.x * BLK + threadIdx.x;
+    const long long mn = (long long)m * n;
+    if (e >= mn) return;
+    const int row = (int)(e % m), col = (int)(e / m);
+    const long long ic = (long long)col * ldc + row;
+    Grp<N, 1> g(C_);
+    MP<N> a, b, cv;
+    load_soa(g, a, alpha, 0);
+    load_soa(g, b, beta, 0);
+    load_soa(g, cv, Cm, ic);
+    // the terms: the band pairs present times alpha, and C times beta, with their exponents
+    const bool la = BLA[row] != MF_NONE, lb = BLB[col] != MF_NONE;
+    bool live[5];
+    int ex[5];
+    auto nonzero = [](const int *d) {
+        int o = 0;
+#pragma unroll
+        for (int i = 0; i < N; i++) o |= d[i];
+        return o != 0;
+    };
+    const bool anz = nonzero(a.d);
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        const int ba_ = p >> 1, bb_ = p & 1;
+        live[p] = anz && !((!ba_ && !la) || (!bb_ && !lb));
+        if (live[p]) {
+            int d[N];
+#pragma unroll
+            for (int i = 0; i < N; i++) d[i] = S[(size_t)p * N * mn + (size_t)i * mn + e];
+            live[p] = nonzero(d);
+        }
+        ex[p] = (ba_ ? EHA[row] : ELA[row]) + (bb_ ? EHB[col] : ELB[col]) + a.exp;
+    }
+    live[4] = nonzero(b.d) && nonzero(cv.d);
+    ex[4] = cv.exp + b.exp;
+    int emin = 0x7fffffff, emax = -0x7fffffff;
+#pragma unroll
+    for (int t = 0; t < 5; t++)
+        if (live[t]) {
+            emin = ex[t] < emin ? ex[t] : emin;
+            emax = ex[t] > emax ? ex[t] : emax;
+        }
+    if (emin == 0x7fffffff) {  // every term is zero
+        MP<N> z;
+        g.zero(z);
+        store_soa(g, z, Cm, ic);
+        return;
+    }
+    // a product of two numbers below M has at most 2 log2 M + 2 bits: the narrowest
+    // accumulator that holds the aligned terms, or the rounded finish
+    const long long need = (long long)emax - emin + 2 * (C_->log2m + 1) + 2;
+    MP<N> res;
+    if (need <= 32 * MF_XS)
+        mf_exact_sum<N, MF_XS>(g, C_, e, mn, S, a, b, cv, live, ex, emin, res);
+    else if (need <= 32 * MF_XW)
+        mf_exact_sum<N, MF_XW>(g, C_, e, mn, S, a, b, cv, live, ex, emin, res);
+    else
+        return mf_finish_rounded<N, 1>(g, e, mn, row, col, lk, S, EHA, BHA, ELA, BLA, EHB, BHB, ELB, BLB, alpha, beta,
+                                       Cm, ldc);
     store_soa(g, res, Cm, ic);
 }
 

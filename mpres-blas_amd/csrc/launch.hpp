@@ -399,8 +399,12 @@ struct Launch {
             const int *fa = ba ? nullptr : BL, *fb = bb ? nullptr : BL + m;
             mf_products(c, nl, m, n, MP, NP, KP, pa, pb, fa, fb, S + (size_t)p * N * m * n, st);
         }
-        hipLaunchKernelGGL((k_mf_finish<N, L>), dim3(grid_for((long long)m * n, L)), bl, 0, st, c, m, n, lk, S, EH, BH,
-                           EL, BL, EH + m, BH + m, EL + m, BL + m, alpha, beta, Cm, ldc);
+        if constexpr (N == 8)  // one rounding per entry (exact integer epilogue)
+            hipLaunchKernelGGL((k_mf_finish_exact<N, L>), dim3(grid_for((long long)m * n, 1)), bl, 0, st, c, m, n, lk,
+                               S, EH, BH, EL, BL, EH + m, BH + m, EL + m, BL + m, alpha, beta, Cm, ldc);
+        else
+            hipLaunchKernelGGL((k_mf_finish<N, L>), dim3(grid_for((long long)m * n, L)), bl, 0, st, c, m, n, lk, S, EH,
+                               BH, EL, BL, EH + m, BH + m, EL + m, BL + m, alpha, beta, Cm, ldc);
         return hipGetLastError();
     }
     // pass 0: the row results, or REDO marks (16 and 32 moduli: the split form: CSR rows

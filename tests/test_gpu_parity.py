@@ -862,6 +862,8 @@ def mfma_reference(o, log2m, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, Cm, l
             if live[e]:
                 acc = sums[e:e + 1] if acc is None else o.add(acc, sums[e:e + 1], D)
         total[e] = acc[0]
+    if len(mm) == 8:
+        return mfma_exact_epilogue(o, mm, m, n, k, ba_, bb_, av, bv, pairs, alpha, beta, Cm, ldc), total
     want = np.array(Cm, copy=True)
     for col in range(n):
         for row in range(m):
@@ -870,6 +872,63 @@ def mfma_reference(o, log2m, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, Cm, l
             c2 = o.mul(Cm[col * ldc + row:col * ldc + row + 1], beta, D)
             want[col * ldc + row] = o.add(s2, c2, D)[0]
     return want, total
+
+
+def mfma_exact_epilogue(o, mm, m, n, k, ba_, bb_, av, bv, pairs, alpha, beta, Cm, ldc):
+    """The 8-moduli finish: alpha times the exact band-pair sums plus beta times C, formed
+    exactly and rounded once to MP_PRECISION bits to nearest (ties to even); the significand
+    without trailing zeros, its residues and the oracle's full evaluation"""
+    prec = o.mp_consts()[0]
+    M = exact.moduli_product(mm)
+
+    def val(r):
+        return (-1) ** int(r["sign"]) * exact.crt(r["digits"], mm), int(r["exp"])
+
+    xa, ea = val(alpha[0])
+    xb, eb = val(beta[0])
+    want = np.array(Cm, copy=True)
+    recs, where = [], []
+    for col in range(n):
+        for row in range(m):
+            e = row + m * col
+            terms = []
+            for p, (_, live) in enumerate(pairs):
+                pa, pb = p >> 1, p & 1
+                if not live[e] or xa == 0:
+                    continue
+                S = sum(av[row][pa][i] * bv[col][pb][i] for i in range(k))
+                if S:
+                    terms.append((S * xa, ba_[row][pa][0] + bb_[col][pb][0] + ea))
+            xc, ec = val(Cm[col * ldc + row])
+            if xb and xc:
+                terms.append((xc * xb, ec + eb))
+            r = o.empty(1)[0]
+            if terms:
+                emin = min(t[1] for t in terms)
+                T = sum(v << (x - emin) for v, x in terms)
+                if T:
+                    q, sh = abs(T), 0
+                    if q.bit_length() > prec:
+                        sh = q.bit_length() - prec
+                        low, q = q & ((1 << sh) - 1), q >> sh
+                        half = 1 << (sh - 1)
+                        if low > half or (low == half and q & 1):
+                            q += 1
+                    tz = (q & -q).bit_length() - 1
+                    q >>= tz
+                    assert q < M
+                    r["digits"] = [q % v for v in mm]
+                    r["sign"] = int(T < 0)
+                    r["exp"] = emin + sh + tz
+            recs.append(r)
+            where.append(col * ldc + row)
+    recs = np.array(recs)
+    lo, up = o.eval(np.ascontiguousarray(recs["digits"]), fast=False, policy=D)
+    recs["eval"][:, 0] = lo
+    recs["eval"][:, 1] = up
+    for r, w in zip(recs, where):
+        want[w] = r
+    return want
 
 
 @pytest.mark.parametrize("name,lo,hi", [("8_2double", -8, 8), ("8_2double", -60, 60), ("8_2double", -150, 150),
