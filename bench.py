@@ -716,7 +716,7 @@ def counter_run(ctx, mp, A, x, y, alpha, beta, Ag, Bg, Cg, alpha_g, beta_g, ws, 
 def counter_ops_run(mp):
     """for hardware-counter runs of the other operations: one warm-up and one measured call of
     mp_dot and mp_axpy (n = 2^24, 8 moduli), the CSR and CSR-vector SpMV (10^6 rows, 16 moduli,
-    16 threads per row), the precision-form gemm (2048^3, 8 moduli, e in [-8, 8]) and the
+    4 threads per row), the precision-form gemm (2048^3, 8 moduli, e in [-8, 8]) and the
     batched form (1024 problems of the batching workload), then the 256 MiB calibration copies"""
     ctx = mp.Context("8_2double")
     L = ctx.L
@@ -748,7 +748,7 @@ def counter_ops_run(mp):
     for _ in range(2):
         mp.check(L.mpres_spmv_csr(ctx.h, m, C.byref(csr), xs.ptr, ys.ptr, None))
     for _ in range(2):
-        mp.check(L.mpres_spmv_csrv(ctx.h, m, C.byref(csr), 16, xs.ptr, ys.ptr, None))
+        mp.check(L.mpres_spmv_csrv(ctx.h, m, C.byref(csr), 4, xs.ptr, ys.ptr, None))
     mp.synchronize()
     wl = gen.batch_workload(ctx, mp, 1024)
     b = mp.Batch(ctx, wl["problems"], dict(lanes=4, wg_per_cu=4, gemv_rows=128))

@@ -395,67 +395,73 @@ __global__ __launch_bounds__(BLK) void k_mf_finish(const DevConsts *__restrict__
 template <int N, int W>
 __device__ __forceinline__ void mf_exact_sum(const Grp<N, 1> &g, const DevConsts *__restrict__ C_, long long e,
                                              long long mn, const int *__restrict__ S, const MP<N> &a, const MP<N> &b,
-                                             const MP<N> &cv, const bool *live, const int *ex, int emin, MP<N> &res) {
+                                             const MP<N> &cv, int live, const int (&ex)[5], int emin, MP<N> &res) {
     constexpr int WX = big_limbs<N>();
     static_assert(2 * WX + 1 <= W, "accumulator");
+    // Every term is below 2^(need - 2) (k_mf_finish_exact), their sum below 2^(need - 1)
+    // <= 2^(32 W - 1): one accumulator in two's complement holds every partial sum.
+    Big<W> A;
+    big_zero(A);
+    if ((live >> 4) & 1) {  // C beta first: C and beta are not needed after it
+        const Big<W> w = big_shl(
+            big_widen<W>(big_mul(residues_to_big<N>(C_, cv.d), residues_to_big<N>(C_, b.d))), ex[4] - emin);
+        if (cv.sign ^ b.sign)
+            big_sub(A, w);
+        else
+            big_add(A, w);
+    }
     BigN<N> Mb;
     big_zero(Mb);
     Mb.l[0] = 1u;
 #pragma unroll
     for (int i = 0; i < N; i++) big_mul_add_u32(Mb, (uint32_t)C_->m[i], 0u);
     const BigN<N> Xa = residues_to_big<N>(C_, a.d);
-    Big<W> P, Q;  // the positive and the negative terms, aligned at emin
-    big_zero(P);
-    big_zero(Q);
 #pragma unroll 1
-    for (int t = 0; t < 5; t++) {
-        if (!live[t]) continue;
-        BigN<N> X, Y;
-        int sg;
-        if (t < 4) {
-            // the band pair's exact sum: residues of the signed sum, |sum| < M / 4
-            int d[N];
+    for (int t = 0; t < 4; t++) {
+        if (!((live >> t) & 1)) continue;
+        // the term's exponent, selected without indexing the array with t
+        int ext = ex[0];
 #pragma unroll
-            for (int i = 0; i < N; i++) d[i] = S[(size_t)t * N * mn + (size_t)i * mn + e];
-            X = residues_to_big<N>(C_, d);
-            sg = big_cmp(big_shl(X, 1), Mb) >= 0;
-            if (sg) {
-                BigN<N> t2 = Mb;
-                big_sub(t2, X);
-                X = t2;
-            }
-            sg ^= a.sign;
-            Y = Xa;
-        } else {
-            X = residues_to_big<N>(C_, cv.d);
-            Y = residues_to_big<N>(C_, b.d);
-            sg = cv.sign ^ b.sign;
+        for (int u = 1; u < 4; u++) ext = t == u ? ex[u] : ext;
+        // the band pair's exact sum: residues of the signed sum, |sum| < M / 4
+        int d[N];
+#pragma unroll
+        for (int i = 0; i < N; i++) d[i] = S[(size_t)t * N * mn + (size_t)i * mn + e];
+        BigN<N> X = residues_to_big<N>(C_, d);
+        int sg = big_cmp(big_shl(X, 1), Mb) >= 0;
+        if (sg) {
+            BigN<N> t2 = Mb;
+            big_sub(t2, X);
+            X = t2;
         }
-        const Big<W> w = big_shl(big_widen<W>(big_mul(X, Y)), ex[t] - emin);
+        sg ^= a.sign;
+        const Big<W> w = big_shl(big_widen<W>(big_mul(X, Xa)), ext - emin);
         if (sg)
-            big_add(Q, w);
+            big_sub(A, w);
         else
-            big_add(P, w);
+            big_add(A, w);
     }
-    int sign = 0;
-    if (big_cmp(P, Q) < 0) {
-        big_sub(Q, P);
-        P = Q;
-        sign = 1;
-    } else {
-        big_sub(P, Q);
+    const int sign = (int)(A.l[W - 1] >> 31);
+    if (sign) {  // |A|
+#pragma unroll
+        for (int i = 0; i < W; i++) A.l[i] = ~A.l[i];
+        big_add_u32(A, 1u);
     }
-    if (big_is_zero(P)) {
+    if (big_is_zero(A)) {
         g.zero(res);
         return;
     }
     int exp = emin;
-    big_round(P, exp, C_->mp_precision);
-    const int tz = big_ctz(P);
-    P = big_shr(P, tz);
+    big_round(A, exp, C_->mp_precision);
+    const int tz = big_ctz(A);
+    A = big_shr(A, tz);
     exp += tz;
+    // the rounded significand has at most MP_PRECISION + 1 < log2 M bits: WX limbs hold it
+    BigN<N> R;
 #pragma unroll
-    for (int i = 0; i < N; i++) res.d[i] = (int)big_mod_u32(P, (uint32_t)C_->m[i]);
+    for (int i = 0; i < WX; i++) R.l[i] = A.l[i];
+#pragma unroll
+    for (int i = 0; i < N; i++) res.d[i] = big_mod_d(R, C_->md[i], C_->minv[i]);
     res.sign = sign;
     res.exp = exp;
     g.eval_full(res.d, res.ev0, res.ev1);
@@ -464,17 +470,68 @@ __device__ __forceinline__ void mf_exact_sum(const Grp<N, 1> &g, const DevConsts
 // The finish at 8 moduli with one rounding per entry: alpha * (the exact band-pair sums)
 // + beta * C is formed exactly in integers (one thread per entry) and rounded once to
 // MP_PRECISION bits, to nearest (ties to even); the result is that significand without
-// trailing zeros, its residues and the full interval evaluation. An entry whose terms
-// span more than the accumulator holds takes the rounded finish above.
+// trailing zeros, its residues and the full interval evaluation.
 constexpr int MF_XS = 24, MF_XW = 48;  // 32-bit limbs of the exact accumulation (common, widest)
+
+// One entry's terms: alpha, beta and C loaded, the live terms (bit t: band pair t, then
+// C beta) with their exponents; returns the accumulator bits they need, or 0 when every
+// term is zero.
+template <int N>
+__device__ __forceinline__ long long mf_exact_terms(const Grp<N, 1> &g, const DevConsts *__restrict__ C_, long long e,
+                                                    long long mn, int row, int col, const int *__restrict__ S,
+                                                    const int *__restrict__ EHA, const int *__restrict__ BLA,
+                                                    const int *__restrict__ ELA, const int *__restrict__ EHB,
+                                                    const int *__restrict__ BLB, const int *__restrict__ ELB,
+                                                    const SoA &alpha, const SoA &beta, const SoA &Cm, long long ic,
+                                                    MP<N> &a, MP<N> &b, MP<N> &cv, int &live, int (&ex)[5],
+                                                    int &emin) {
+    load_soa(g, a, alpha, 0);
+    load_soa(g, b, beta, 0);
+    load_soa(g, cv, Cm, ic);
+    const bool la = BLA[row] != MF_NONE, lb = BLB[col] != MF_NONE;
+    auto nonzero = [](const int *d) {
+        int o = 0;
+#pragma unroll
+        for (int i = 0; i < N; i++) o |= d[i];
+        return o != 0;
+    };
+    const bool anz = nonzero(a.d);
+    live = 0;
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        const int ba_ = p >> 1, bb_ = p & 1;
+        if (anz && !((!ba_ && !la) || (!bb_ && !lb))) {
+            int d[N];
+#pragma unroll
+            for (int i = 0; i < N; i++) d[i] = S[(size_t)p * N * mn + (size_t)i * mn + e];
+            live |= nonzero(d) ? 1 << p : 0;
+        }
+        ex[p] = (ba_ ? EHA[row] : ELA[row]) + (bb_ ? EHB[col] : ELB[col]) + a.exp;
+    }
+    live |= nonzero(b.d) && nonzero(cv.d) ? 1 << 4 : 0;
+    ex[4] = cv.exp + b.exp;
+    emin = 0x7fffffff;
+    int emax = -0x7fffffff;
+#pragma unroll
+    for (int t = 0; t < 5; t++)
+        if ((live >> t) & 1) {
+            emin = ex[t] < emin ? ex[t] : emin;
+            emax = ex[t] > emax ? ex[t] : emax;
+        }
+    if (!live) return 0;
+    // a product of two numbers below M has at most 2 log2 M + 2 bits
+    return (long long)emax - emin + 2 * (C_->log2m + 1) + 2;
+}
+
+// The common case: entries whose aligned terms fit MF_XS limbs. The others are appended
+// to a list (wcnt, wlist) for k_mf_finish_wide, and their C entries are left untouched.
 template <int N, int L>
-__global__ __launch_bounds__(BLK) void k_mf_finish_exact(const DevConsts *__restrict__ C_, int m, int n, int lk,
+__global__ __launch_bounds__(BLK) void k_mf_finish_exact(const DevConsts *__restrict__ C_, int m, int n,
                                                          const int *__restrict__ S, const int *__restrict__ EHA,
-                                                         const int *__restrict__ BHA, const int *__restrict__ ELA,
-                                                         const int *__restrict__ BLA, const int *__restrict__ EHB,
-                                                         const int *__restrict__ BHB, const int *__restrict__ ELB,
+                                                         const int *__restrict__ ELA, const int *__restrict__ BLA,
+                                                         const int *__restrict__ EHB, const int *__restrict__ ELB,
                                                          const int *__restrict__ BLB, SoA alpha, SoA beta, SoA Cm,
-                                                         int ldc) {
+                                                         int ldc, int *wcnt, int *wlist) {
     alpha = rd(alpha);
     beta = rd(beta);
     Cm = rd(Cm);
@@ -484,60 +541,56 @@ __global__ __launch_bounds__(BLK) void k_mf_finish_exact(const DevConsts *__rest
     const int row = (int)(e % m), col = (int)(e / m);
     const long long ic = (long long)col * ldc + row;
     Grp<N, 1> g(C_);
-    MP<N> a, b, cv;
-    load_soa(g, a, alpha, 0);
-    load_soa(g, b, beta, 0);
-    load_soa(g, cv, Cm, ic);
-    // the terms: the band pairs present times alpha, and C times beta, with their exponents
-    const bool la = BLA[row] != MF_NONE, lb = BLB[col] != MF_NONE;
-    bool live[5];
-    int ex[5];
-    auto nonzero = [](const int *d) {
-        int o = 0;
-#pragma unroll
-        for (int i = 0; i < N; i++) o |= d[i];
-        return o != 0;
-    };
-    const bool anz = nonzero(a.d);
-#pragma unroll
-    for (int p = 0; p < 4; p++) {
-        const int ba_ = p >> 1, bb_ = p & 1;
-        live[p] = anz && !((!ba_ && !la) || (!bb_ && !lb));
-        if (live[p]) {
-            int d[N];
-#pragma unroll
-            for (int i = 0; i < N; i++) d[i] = S[(size_t)p * N * mn + (size_t)i * mn + e];
-            live[p] = nonzero(d);
-        }
-        ex[p] = (ba_ ? EHA[row] : ELA[row]) + (bb_ ? EHB[col] : ELB[col]) + a.exp;
-    }
-    live[4] = nonzero(b.d) && nonzero(cv.d);
-    ex[4] = cv.exp + b.exp;
-    int emin = 0x7fffffff, emax = -0x7fffffff;
-#pragma unroll
-    for (int t = 0; t < 5; t++)
-        if (live[t]) {
-            emin = ex[t] < emin ? ex[t] : emin;
-            emax = ex[t] > emax ? ex[t] : emax;
-        }
-    if (emin == 0x7fffffff) {  // every term is zero
-        MP<N> z;
-        g.zero(z);
-        store_soa(g, z, Cm, ic);
+    MP<N> a, b, cv, res;
+    int live, ex[5], emin;
+    const long long need =
+        mf_exact_terms<N>(g, C_, e, mn, row, col, S, EHA, BLA, ELA, EHB, BLB, ELB, alpha, beta, Cm, ic, a, b, cv, live,
+                          ex, emin);
+    if (need > 32 * MF_XS) {
+        wlist[atomicAdd(wcnt, 1)] = (int)e;
         return;
     }
-    // a product of two numbers below M has at most 2 log2 M + 2 bits: the narrowest
-    // accumulator that holds the aligned terms, or the rounded finish
-    const long long need = (long long)emax - emin + 2 * (C_->log2m + 1) + 2;
-    MP<N> res;
-    if (need <= 32 * MF_XS)
-        mf_exact_sum<N, MF_XS>(g, C_, e, mn, S, a, b, cv, live, ex, emin, res);
-    else if (need <= 32 * MF_XW)
-        mf_exact_sum<N, MF_XW>(g, C_, e, mn, S, a, b, cv, live, ex, emin, res);
+    if (need == 0)
+        g.zero(res);
     else
-        return mf_finish_rounded<N, 1>(g, e, mn, row, col, lk, S, EHA, BHA, ELA, BLA, EHB, BHB, ELB, BLB, alpha, beta,
-                                       Cm, ldc);
+        mf_exact_sum<N, MF_XS>(g, C_, e, mn, S, a, b, cv, live, ex, emin, res);
     store_soa(g, res, Cm, ic);
+}
+
+// The listed entries: exactly in MF_XW limbs, or, when the terms span more than that, the
+// rounded finish (mf_finish_rounded). Grid-stride over the list.
+template <int N, int L>
+__global__ __launch_bounds__(BLK) void k_mf_finish_wide(const DevConsts *__restrict__ C_, int m, int n, int lk,
+                                                        const int *__restrict__ S, const int *__restrict__ EHA,
+                                                        const int *__restrict__ BHA, const int *__restrict__ ELA,
+                                                        const int *__restrict__ BLA, const int *__restrict__ EHB,
+                                                        const int *__restrict__ BHB, const int *__restrict__ ELB,
+                                                        const int *__restrict__ BLB, SoA alpha, SoA beta, SoA Cm,
+                                                        int ldc, const int *__restrict__ wcnt,
+                                                        const int *__restrict__ wlist) {
+    alpha = rd(alpha);
+    beta = rd(beta);
+    Cm = rd(Cm);
+    const long long mn = (long long)m * n;
+    const int cnt = *wcnt;
+    Grp<N, 1> g(C_);
+    for (int w = blockIdx.x * BLK + threadIdx.x; w < cnt; w += gridDim.x * BLK) {
+        const long long e = wlist[w];
+        const int row = (int)(e % m), col = (int)(e / m);
+        const long long ic = (long long)col * ldc + row;
+        MP<N> a, b, cv, res;
+        int live, ex[5], emin;
+        const long long need =
+            mf_exact_terms<N>(g, C_, e, mn, row, col, S, EHA, BLA, ELA, EHB, BLB, ELB, alpha, beta, Cm, ic, a, b, cv,
+                              live, ex, emin);
+        if (need <= 32 * MF_XW) {
+            mf_exact_sum<N, MF_XW>(g, C_, e, mn, S, a, b, cv, live, ex, emin, res);
+            store_soa(g, res, Cm, ic);
+        } else {
+            mf_finish_rounded<N, 1>(g, e, mn, row, col, lk, S, EHA, BHA, ELA, BLA, EHB, BHB, ELB, BLB, alpha, beta, Cm,
+                                    ldc);
+        }
+    }
 }
 
 }  // namespace mph

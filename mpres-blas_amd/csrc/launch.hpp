@@ -337,8 +337,10 @@ struct Launch {
     static size_t gemm_mfma_ws(int bits, int m, int n, int k) {
         const int nl = gemm_limbs(bits);
         const size_t MPd = rup(m, 64), NPd = rup(n, 64), KPd = rup(k > 0 ? k : 1, MF_T);
-        return (4 * mf_ints(m, n) + 255) / 256 * 256 + 2 * (size_t)N * nl * (MPd + NPd) * KPd;
+        return (4 * mf_ints(m, n) + 255) / 256 * 256 + 2 * (size_t)N * nl * (MPd + NPd) * KPd + mf_wide_bytes(m, n);
     }
+    // 8 moduli: the count and list of the entries of the wide exact finish (k_mf_finish_wide)
+    static size_t mf_wide_bytes(int m, int n) { return N == 8 ? 4 * (64 + (size_t)m * n) : 0; }
     // residue products of limb planes on INT8 MFMA (k_mf_gemm), one launch
     static void mf_products(const DevConsts *c, int nl, int m, int n, int MP, int NP, int KP, const signed char *pa,
                             const signed char *pb, const int *fa, const int *fb, int *S, hipStream_t st) {
@@ -399,10 +401,16 @@ struct Launch {
             const int *fa = ba ? nullptr : BL, *fb = bb ? nullptr : BL + m;
             mf_products(c, nl, m, n, MP, NP, KP, pa, pb, fa, fb, S + (size_t)p * N * m * n, st);
         }
-        if constexpr (N == 8)  // one rounding per entry (exact integer epilogue)
-            hipLaunchKernelGGL((k_mf_finish_exact<N, L>), dim3(grid_for((long long)m * n, 1)), bl, 0, st, c, m, n, lk,
-                               S, EH, BH, EL, BL, EH + m, BH + m, EL + m, BL + m, alpha, beta, Cm, ldc);
-        else
+        if constexpr (N == 8) {  // one rounding per entry (exact integer epilogue)
+            int *wcnt = (int *)(PB + 2 * bandB), *wlist = wcnt + 64;
+            hipLaunchKernelGGL((k_fill_int<N, L>), dim3(1), bl, 0, st, wcnt, 1, 0);
+            hipLaunchKernelGGL((k_mf_finish_exact<N, L>), dim3(grid_for((long long)m * n, 1)), bl, 0, st, c, m, n, S,
+                               EH, EL, BL, EH + m, EL + m, BL + m, alpha, beta, Cm, ldc, wcnt, wlist);
+            const long long gw = grid_for((long long)m * n, 1);
+            hipLaunchKernelGGL((k_mf_finish_wide<N, L>), dim3((unsigned)(gw < 1024 ? gw : 1024)), bl, 0, st, c, m, n,
+                               lk, S, EH, BH, EL, BL, EH + m, BH + m, EL + m, BL + m, alpha, beta, Cm, ldc,
+                               (const int *)wcnt, (const int *)wlist);
+        } else
             hipLaunchKernelGGL((k_mf_finish<N, L>), dim3(grid_for((long long)m * n, L)), bl, 0, st, c, m, n, lk, S, EH,
                                BH, EL, BL, EH + m, BH + m, EL + m, BL + m, alpha, beta, Cm, ldc);
         return hipGetLastError();

@@ -4,6 +4,7 @@
 
 #include "../../mpres-blas_amd/csrc/internal.hpp"
 #include "../../mpres-blas_amd/csrc/numerics.hpp"
+#include "../../mpres-blas_amd/csrc/bigint.hpp"
 
 extern "C" {
 double nh_add(int up, double a, double b) { return up ? mph::add_ru(a, b) : mph::add_rd(a, b); }
@@ -43,5 +44,39 @@ void nh_er_adjust(double f, int e, double *rf, int *re) {
     mph::Er r = mph::er_adjust(f, e);
     *rf = r.frac;
     *re = r.exp;
+}
+// fixed-width integers (bigint.hpp) at 24 limbs, the exact finish's common accumulator
+constexpr int BW = 24;
+static mph::Big<BW> big_in(const uint32_t *a) {
+    mph::Big<BW> r;
+    for (int i = 0; i < BW; i++) r.l[i] = a[i];
+    return r;
+}
+static void big_out(const mph::Big<BW> &r, uint32_t *o) {
+    for (int i = 0; i < BW; i++) o[i] = r.l[i];
+}
+void nh_big_shl(const uint32_t *a, int s, uint32_t *o) { big_out(mph::big_shl(big_in(a), s), o); }
+void nh_big_shr(const uint32_t *a, int s, uint32_t *o) { big_out(mph::big_shr(big_in(a), s), o); }
+// bit length, trailing zeros, bit i, any bit below i
+void nh_big_bits(const uint32_t *a, int i, int *o) {
+    const mph::Big<BW> x = big_in(a);
+    o[0] = mph::big_bitlen(x);
+    o[1] = mph::big_ctz(x);
+    o[2] = mph::big_bit(x, i);
+    o[3] = mph::big_any_below(x, i) ? 1 : 0;
+}
+int nh_big_cmp(const uint32_t *a, const uint32_t *b) { return mph::big_cmp(big_in(a), big_in(b)); }
+int nh_big_round(const uint32_t *a, int p, uint32_t *o) {
+    mph::Big<BW> x = big_in(a);
+    int e = 0;
+    mph::big_round(x, e, p);
+    big_out(x, o);
+    return e;
+}
+// a mod m through FP64 steps (big_mod_d) and through 64-bit integer division (big_mod_u32)
+void nh_big_mod(const uint32_t *a, int m, int *o) {
+    const mph::Big<BW> x = big_in(a);
+    o[0] = mph::big_mod_d(x, (double)m, 1.0 / (double)m);
+    o[1] = (int)mph::big_mod_u32(x, (uint32_t)m);
 }
 }

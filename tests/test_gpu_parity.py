@@ -768,7 +768,7 @@ def _log2m(c):
     return v[4].value
 
 
-def mfma_reference(o, log2m, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc):
+def mfma_reference(o, log2m, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc, stats=None):
     """The precision form restated with exact integers and the oracle's primitives.
     Per row of op(A) / column of op(B), over its nonzero elements: emin, the highest bit
     top = max(eval_up.exp + 2 + log2m + exp); the high band's exponent eh = max(emin,
@@ -862,22 +862,31 @@ def mfma_reference(o, log2m, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, Cm, l
             if live[e]:
                 acc = sums[e:e + 1] if acc is None else o.add(acc, sums[e:e + 1], D)
         total[e] = acc[0]
+    def rounded(e, row, col):
+        s2 = o.mul(total[e:e + 1], alpha, D)
+        c2 = o.mul(Cm[col * ldc + row:col * ldc + row + 1], beta, D)
+        return o.add(s2, c2, D)[0]
+
     if len(mm) == 8:
-        return mfma_exact_epilogue(o, mm, m, n, k, ba_, bb_, av, bv, pairs, alpha, beta, Cm, ldc), total
+        return mfma_exact_epilogue(o, mm, log2m, m, n, k, ba_, bb_, av, bv, pairs, alpha, beta, Cm, ldc,
+                                   rounded, stats), total
     want = np.array(Cm, copy=True)
     for col in range(n):
         for row in range(m):
-            e = row + m * col
-            s2 = o.mul(total[e:e + 1], alpha, D)
-            c2 = o.mul(Cm[col * ldc + row:col * ldc + row + 1], beta, D)
-            want[col * ldc + row] = o.add(s2, c2, D)[0]
+            want[col * ldc + row] = rounded(row + m * col, row, col)
     return want, total
 
 
-def mfma_exact_epilogue(o, mm, m, n, k, ba_, bb_, av, bv, pairs, alpha, beta, Cm, ldc):
+MF_XW_BITS = 32 * 48  # the widest exact accumulation of the 8-moduli finish
+
+
+def mfma_exact_epilogue(o, mm, log2m, m, n, k, ba_, bb_, av, bv, pairs, alpha, beta, Cm, ldc, rounded, stats=None):
     """The 8-moduli finish: alpha times the exact band-pair sums plus beta times C, formed
     exactly and rounded once to MP_PRECISION bits to nearest (ties to even); the significand
-    without trailing zeros, its residues and the oracle's full evaluation"""
+    without trailing zeros, its residues and the oracle's full evaluation. An entry whose
+    terms span more than MF_XW_BITS (exponent span + 2 (log2 M + 1) + 2 bits) takes the
+    rounded finish (rounded(e, row, col)). stats, when given, counts the entries by the
+    accumulator they need: [common (768 bits), wide (1536 bits), rounded]."""
     prec = o.mp_consts()[0]
     M = exact.moduli_product(mm)
 
@@ -905,6 +914,12 @@ def mfma_exact_epilogue(o, mm, m, n, k, ba_, bb_, av, bv, pairs, alpha, beta, Cm
             r = o.empty(1)[0]
             if terms:
                 emin = min(t[1] for t in terms)
+                need = max(t[1] for t in terms) - emin + 2 * (log2m + 1) + 2
+                if stats is not None:
+                    stats[0 if need <= 32 * 24 else 1 if need <= MF_XW_BITS else 2] += 1
+                if need > MF_XW_BITS:
+                    want[col * ldc + row] = rounded(e, row, col)
+                    continue
                 T = sum(v << (x - emin) for v, x in terms)
                 if T:
                     q, sh = abs(T), 0
@@ -970,6 +985,33 @@ def test_gemm_mfma(name, lo, hi, ta, tb):
                 worst = max(worst, float(abs(got - ex) / abs(ex)))
     if hi <= 60:
         assert worst <= 2.0 ** (-prec + 3), worst
+
+
+def test_gemm_mfma_wide_finish():
+    """8 moduli: C entries far above alpha op(A) op(B) make terms that span more than the
+    common accumulator (768 bits: the wide exact finish over the list of such entries) or
+    more than 1536 bits (the rounded finish); each kind is present"""
+    name = "8_2double"
+    c, o = ctx(name), orc(name)
+    m, n, k = 23, 19, 17
+    As, Bs = gen.uexp(131, m * k, -8, 8), gen.uexp(132, k * n, -8, 8)
+    Cs = gen.uexp(133, m * n, -8, 8)
+    Cs[1::3] *= 2.0 ** 400   # about 600 bits above the products: wide
+    Cs[2::3] *= 2.0 ** 900   # about 1100 bits above: rounded
+    Cs[7] = 0.0
+    A, B, Cm = o.set_d(As, D), o.set_d(Bs, D), o.set_d(Cs, D)
+    alpha, beta = o.set_d([2.0 ** -200 * 1.375], D), o.set_d([0.75], D)
+    stats = [0, 0, 0]
+    want, _ = mfma_reference(o, _log2m(c), 0, 0, m, n, k, alpha, A, m, B, k, beta, Cm, m, stats)
+    assert all(v > 0 for v in stats), stats
+    note("gemm_mfma wide finish", stats)
+    aA, aB, aC = c.array_from_host(A), c.array_from_host(B), c.array_from_host(Cm)
+    aa, ab = c.array_from_host(alpha), c.array_from_host(beta)
+    for rep in range(2):  # the second call: the list count starts from zero again
+        aC = c.array_from_host(Cm)
+        mp.check(c.L.mpres_gemm_mfma(c.h, mp.NO_TRANS, mp.NO_TRANS, m, n, k, C.byref(aa), C.byref(aA), m,
+                                     C.byref(aB), k, C.byref(ab), C.byref(aC), m, None, 0, None))
+        same(c.array_to_host(aC, m * n), want, f"gemm_mfma wide finish call {rep}")
 
 
 def _batch_case(c, o, name, seed0):

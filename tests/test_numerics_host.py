@@ -37,6 +37,12 @@ def nh():
     L.nh_ceil_log2.argtypes = [C.c_double]
     L.nh_mulmod.argtypes = [C.c_int, C.c_int, C.c_int]
     L.nh_modd.argtypes = [C.c_double, C.c_int]
+    u32p, ip = C.POINTER(C.c_uint32), C.POINTER(C.c_int)
+    L.nh_big_shl.argtypes = L.nh_big_shr.argtypes = [u32p, C.c_int, u32p]
+    L.nh_big_bits.argtypes = [u32p, C.c_int, ip]
+    L.nh_big_cmp.argtypes = [u32p, u32p]
+    L.nh_big_round.argtypes = [u32p, C.c_int, u32p]
+    L.nh_big_mod.argtypes = [u32p, C.c_int, ip]
     L.nh_er_add.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_double),
                             C.POINTER(C.c_int)]
     return L
@@ -160,3 +166,87 @@ def test_er_adjust_matches_bit_definition(nh):
         nh.nh_er_adjust(float(f), e, C.byref(rf), C.byref(re_))
         wf, we = ref(float(f), e)
         assert bits(rf.value) == bits(wf) and re_.value == we, (f, e, rf.value, re_.value, wf, we)
+
+
+BW = 24  # limbs of the host build's fixed-width integers
+
+
+def _big(v):
+    a = np.array([(v >> (32 * i)) & 0xffffffff for i in range(BW)], np.uint32)
+    return a, a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def _int(a):
+    return sum(int(x) << (32 * i) for i, x in enumerate(a))
+
+
+def _big_values(rng, count):
+    """integers of every length up to 24 limbs, sparse and dense bit patterns, single bits,
+    all-ones runs, zero"""
+    out = [0, 1, (1 << (32 * BW)) - 1, 1 << (32 * BW - 1), (1 << 32) - 1, 1 << 32]
+    for _ in range(count):
+        bl = int(rng.integers(1, 32 * BW + 1))
+        kind = int(rng.integers(0, 3))
+        if kind == 0:
+            v = int.from_bytes(rng.bytes(BW * 4), "little") >> (32 * BW - bl)
+        elif kind == 1:
+            v = (1 << bl) - 1 - ((1 << int(rng.integers(0, bl))) - 1)
+        else:
+            v = 1 << (bl - 1) | (1 << int(rng.integers(0, bl)))
+        out.append(v)
+    return out
+
+
+def test_bigint_shifts_and_bits(nh):
+    """big_shl / big_shr (barrel of whole limbs, then bits), bit length, trailing zeros, single
+    bits, the sticky test and comparison, against Python integers at shifts across and beyond
+    the width"""
+    rng = np.random.default_rng(21)
+    mask = (1 << (32 * BW)) - 1
+    out = np.zeros(BW, np.uint32)
+    op = out.ctypes.data_as(C.POINTER(C.c_uint32))
+    bits4 = np.zeros(4, np.int32)
+    vals = _big_values(rng, 400)
+    for t, v in enumerate(vals):
+        a, ap = _big(v)
+        for s in [0, 1, 31, 32, 33, 63, 64, 100, 255, 256, 500, 767, 768, 800, 1000, int(rng.integers(0, 800))]:
+            nh.nh_big_shl(ap, s, op)
+            assert _int(out) == (v << s) & mask, (v, s)
+            nh.nh_big_shr(ap, s, op)
+            assert _int(out) == v >> s, (v, s)
+        i = int(rng.integers(-2, 32 * BW + 3))
+        nh.nh_big_bits(ap, i, bits4.ctypes.data_as(C.POINTER(C.c_int)))
+        tz = (v & -v).bit_length() - 1 if v else 0
+        bit = (v >> i) & 1 if 0 <= i < 32 * BW else 0
+        below = int(i > 0 and (v & ((1 << i) - 1)) != 0)
+        assert list(bits4) == [v.bit_length(), tz, bit, below], (v, i, list(bits4))
+        w = vals[(t * 7 + 3) % len(vals)] if t % 3 else v
+        b, bp = _big(w)
+        assert nh.nh_big_cmp(ap, bp) == (v > w) - (v < w)
+
+
+def test_bigint_round_and_mod(nh):
+    """big_round (to nearest, ties to even, the exponent growing by the bits dropped) and
+    a mod m through FP64 steps (big_mod_d) and through integer division (big_mod_u32),
+    against Python integers; ties are built on purpose"""
+    rng = np.random.default_rng(22)
+    out = np.zeros(BW, np.uint32)
+    op = out.ctypes.data_as(C.POINTER(C.c_uint32))
+    r2 = np.zeros(2, np.int32)
+    vals = _big_values(rng, 300)
+    for p in (53, 106, 212):
+        vals += [((1 << p) + 1) << 5 | 1 << 4, ((1 << p) | 1) << 7 | 1 << 6, ((1 << p) - 1) << 3 | 1 << 2]
+    moduli = [2147483647, 2147483629, 1073741789, 65521, 3, 2, 1000003, 2147418113]
+    for v in vals:
+        a, ap = _big(v)
+        for p in (53, 106, 212):
+            e = nh.nh_big_round(ap, p, op)
+            sh = max(v.bit_length() - p, 0)
+            q, low = v >> sh, v & ((1 << sh) - 1)
+            if sh and (low > 1 << (sh - 1) or (low == 1 << (sh - 1) and q & 1)):
+                q += 1
+            assert (_int(out), e) == (q, sh), (v, p)
+        for m in moduli:
+            nh.nh_big_mod(ap, m, r2.ctypes.data_as(C.POINTER(C.c_int)))
+            assert list(r2) == [v % m, v % m], (v, m)
+
