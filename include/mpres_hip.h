@@ -468,7 +468,9 @@ int mpres_batch_destroy(mpres_batch_t *plan);
  * events and returns the accumulated kernel time and launch count of one
  * operation class (same op numbering as mpres_set_lanes; op 6: the residue
  * products of gemm over 16 and 32 moduli, whose op 3 time is the scalar chains;
- * op 7: mpres_gemm_mfma, all its kernels; op 8: mpres_batch_run, the whole graph). */
+ * op 7: mpres_gemm_mfma, all its kernels; op 8: mpres_batch_run, the whole graph; op 9:
+ * gemv over 16 and 32 moduli, its residue kernel (op 2 time, on a second stream) and its
+ * scalar-chain kernel together, from the fork to the join). */
 int mpres_profile(mpres_ctx_t *ctx, int enable);
 int mpres_profile_read(mpres_ctx_t *ctx, int op, double *ms, long *launches, int reset);
 

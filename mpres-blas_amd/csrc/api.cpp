@@ -60,8 +60,9 @@ namespace {
 // operation classes of mpres_set_lanes; 6 is a timing slot only (gemm residue products)
 enum OpKind { OP_ELEM = 0, OP_DOT = 1, OP_GEMV = 2, OP_GEMM = 3, OP_SPMV = 4, OP_AXPY = 5, OP_MFMA = 7, OP_COUNT = 8 };
 bool lanes_op(int op) { return op >= 0 && op < OP_COUNT && op != 6; }
-// extra timing slot: the residue products of the split-form gemm
-constexpr int PROF_GEMM_RESID = 6, PROF_GEMM_MFMA = 7, PROF_BATCH = 8, PROF_SLOTS = 9;
+// extra timing slots: the residue products of the split-form gemm, the precision form,
+// batches, and the split-form gemv's residue and chain kernels together (two streams)
+constexpr int PROF_GEMM_RESID = 6, PROF_GEMM_MFMA = 7, PROF_BATCH = 8, PROF_GEMV_PAIR = 9, PROF_SLOTS = 10;
 
 const Ops *find_ops(int n, int L) { return find_variant(n, L); }
 
@@ -865,11 +866,25 @@ int mpres_gemv(mpres_ctx_t *c, int trans, int m, int n, const mp_array_t *alpha,
     // buffer1 = alpha * x, y = beta * y (src/blas/gemv.cuh:175-190)
     HIPCHK(o->scal(c->dev, lx, soa_of(alpha), soa_of(x), incx, bx, false, strm(st)));
     HIPCHK(o->scal(c->dev, ly, soa_of(beta), ys, incy, nullptr, true, strm(st)));
-    // per-output strided sums of A .* buffer1 and the halving tree (:199-218, :44-76)
-    for (int pass = 0; pass < 3; pass++) {
-        ProfScope ps(c, pass == 1 ? OP_GEMV : -1, strm(st));
-        HIPCHK(o->gemv_fold(c->dev, tr, pass, m, n, soa_of(A), lda, bx, b3, (long long)ly * b3, part, aux, strm(st)));
+    // per-output strided sums of A .* buffer1 and the halving tree (:199-218, :44-76).
+    // Split form: the residues (pass 1) run on the side stream while the scalar chains
+    // (pass 2) run on st.
+    const long long nvt = (long long)ly * b3;
+    HIPCHK(o->gemv_fold(c->dev, tr, 0, m, n, soa_of(A), lda, bx, b3, nvt, part, aux, strm(st)));
+    if (o->gemv_split(tr, b3)) {
+        ProfScope pair(c, PROF_GEMV_PAIR, strm(st));
+        Fork fk(c, strm(st));
+        {
+            ProfScope ps(c, OP_GEMV, fk.s1);
+            HIPCHK(o->gemv_fold(c->dev, tr, 1, m, n, soa_of(A), lda, bx, b3, nvt, part, aux, fk.s1));
+        }
+        HIPCHK(o->gemv_fold(c->dev, tr, 2, m, n, soa_of(A), lda, bx, b3, nvt, part, aux, strm(st)));
+        HIPCHK(fk.join());
+    } else {
+        ProfScope ps(c, OP_GEMV, strm(st));
+        HIPCHK(o->gemv_fold(c->dev, tr, 1, m, n, soa_of(A), lda, bx, b3, nvt, part, aux, strm(st)));
     }
+    HIPCHK(o->gemv_fold(c->dev, tr, 3, m, n, soa_of(A), lda, bx, b3, nvt, part, aux, strm(st)));
     HIPCHK(o->tree(c->dev, part, b3, next_pow2((unsigned)b3), ly, sums, strm(st)));
     HIPCHK(o->gemv_final(c->dev, ly, ys, incy, sums, strm(st)));
     return MPRES_OK;

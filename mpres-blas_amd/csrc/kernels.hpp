@@ -1112,7 +1112,10 @@ __global__ __launch_bounds__(BLK) void k_gemv_chain(const DevConsts *__restrict_
 }
 
 constexpr int GV_REDUCE = 1 << 20;  // terms between reductions of the double accumulators
-// The residues of the partials whose scalar chains k_gemv_chain ran. The N residues of
+// The residues of the partials, aligned at the row's minimum exponent EA[o] + EB[0]: the
+// residues of the exact signed sum 2^-(EA[o] + EB[0]) sum_j A[o, j] bx[j]. They do not
+// depend on the scalar chains (k_gemv_chain), so the two kernels may run at the same
+// time; k_gemv_scale then moves them to the chain's exponent and sign. The N residues of
 // a row are spread over N / GR threads (GR residues each) so that m * b3 * N / GR
 // threads hide the memory latency that one thread per row cannot at these grid sizes. Thread q of a workgroup holds
 // residues (q % TPR) * GR .. + GR - 1 of row q / TPR: a wave's two 16-byte loads per
@@ -1132,13 +1135,9 @@ __global__ __launch_bounds__(GV_RT) void k_gemv_resid(const DevConsts *__restric
     const int t = blockIdx.y;
     const int g0 = (threadIdx.x % TPR) * GR;
     const long long o = (long long)blockIdx.x * RW + threadIdx.x / TPR;
-    const bool live = o < m;
+    const bool ok = o < m;
     const long long v = o * b3 + t;
-    // the chain's result (k_gemv_chain): sign and exponent, or REDO
-    const int csign = live ? *part.sign(v) : REDO;
-    const int cexp = live ? *part.exp(v) : 0;
-    const bool ok = live && csign != REDO;
-    const int ea = live ? EA[o] : 0;
+    const int ea = ok ? EA[o] : 0;
     double md[GR], mv[GR], acc[GR];
 #pragma unroll
     for (int r = 0; r < GR; r++) {
@@ -1206,16 +1205,39 @@ __global__ __launch_bounds__(GV_RT) void k_gemv_resid(const DevConsts *__restric
         }
     }
     if (!ok) return;
-    const int dz = ea + EB[0] - cexp;
-    int out[GR];
-#pragma unroll
-    for (int r = 0; r < GR; r++) {
-        const int w = mulmod((int)fmod_exact(acc[r], md[r], mv[r]), pow2mod(C, dz, g0 + r), md[r], mv[r]);
-        out[r] = (csign && w) ? C->m[g0 + r] - w : w;
-    }
     int *dig = part.digits(v) + g0;
 #pragma unroll
-    for (int r = 0; r < GR; r++) dig[r] = out[r];
+    for (int r = 0; r < GR; r++) dig[r] = (int)fmod_exact(acc[r], md[r], mv[r]);
+}
+// The residues of k_gemv_resid (the signed sum aligned at EA[o] + EB[0]) moved to the
+// chain's result: times 2^(EA[o] + EB[0] - exp), and m - w for a negative sum (digits
+// hold the magnitude). Partials whose chains were abandoned (REDO) are left to the fixup,
+// which runs after this kernel. GR residues per thread; nvt = m * b3 partials.
+template <int N, int L>
+__global__ __launch_bounds__(BLK) void k_gemv_scale(const DevConsts *__restrict__ C, long long nvt, int b3,
+                                                    const int *__restrict__ EA, const int *__restrict__ EB,
+                                                    AoSRef<N> part) {
+    constexpr int TPR = N / GR;
+    const long long q = (long long)blockIdx.x * BLK + threadIdx.x;
+    const long long v = q / TPR;
+    if (v >= nvt) return;
+    const int g0 = (int)(q % TPR) * GR;
+    const int csign = *part.sign(v);
+    if (csign == REDO) return;
+    const int dz = EA[v / b3] + EB[0] - *part.exp(v);
+    int2 *p = (int2 *)(part.digits(v) + g0);  // records are 8-byte aligned
+#pragma unroll
+    for (int h = 0; h < GR / 2; h++) {
+        const int2 x = p[h];
+        int w[2] = {x.x, x.y};
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int r = g0 + 2 * h + u;
+            w[u] = mulmod(w[u], pow2mod(C, dz, r), C->md[r], C->minv[r]);
+            w[u] = (csign && w[u]) ? C->m[r] - w[u] : w[u];
+        }
+        p[h] = make_int2(w[0], w[1]);
+    }
 }
 
 // ---------------------------------------------------------------- SpMV

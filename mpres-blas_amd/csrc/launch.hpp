@@ -167,8 +167,11 @@ struct Launch {
         return hipGetLastError();
     }
     // gemv passes: 0 preparation (split form: row minima of A's exponents, minimum
-    // and pre-scaled residues of bx), 1 main kernel, 2 fixup of marked partials.
-    // aux: the split form's bxp (N*n ints), EA (m ints), EB (1 int).
+    // and pre-scaled residues of bx), 1 main kernel (split form: the residues), 2 split
+    // form: the scalar chains (independent of pass 1, so a caller may run the two on two
+    // streams), 3 split form: the residues moved to the chains' results, then the fixup of
+    // marked partials. aux: the split form's bxp (N*n ints), EA (m ints), EB (1 int).
+    static bool gemv_split(bool trans, int b3) { return SPLIT && !trans && b3 <= 65535; }
     static size_t gemv_aux(bool trans, int m, int n) {
         return (SPLIT && !trans) ? 4 * ((size_t)N * n + m + 1) : 0;
     }
@@ -176,7 +179,7 @@ struct Launch {
     static void gemv_pass(int pass, const DevConsts *c, int m, int n, SoA A, int lda, AoSRef<N> bx, int b3,
                           long long nvt, AoSRef<N> part, char *aux, hipStream_t st) {
         const dim3 gr(grid_for(nvt, L)), bl(BLK);
-        const bool split = SPLIT && !TR && b3 <= 65535;
+        const bool split = gemv_split(TR, b3);
         int *bxp = (int *)aux, *EA = bxp + (size_t)N * n, *EB = EA + m;
         if (pass == 0) {
             if (split) {
@@ -188,15 +191,6 @@ struct Launch {
                 hipLaunchKernelGGL((k_gemm_emin<N, L, AoSRef<N>, 1>), dim3(1, kc), bl, 0, st, 1, n, 0, bx, 0, EB);
                 hipLaunchKernelGGL((k_gemm_prescale<N, L, AoSRef<N>, 1>), dim3(grid_for(n, 1)), bl, 0, st, c, 1, n, 0,
                                    bx, 0, EB, bxp);
-                const dim3 gc(grid_for((long long)m * b3, 1));
-                const bool narrow = (long long)lda * n <= (1LL << 28);
-                if (narrow && m % 64 == 0)
-                    hipLaunchKernelGGL((k_gemv_chain<N, L, true, true>), gc, bl, 0, st, c, m, n, A, lda, bx, b3, part);
-                else if (narrow)
-                    hipLaunchKernelGGL((k_gemv_chain<N, L, true, false>), gc, bl, 0, st, c, m, n, A, lda, bx, b3, part);
-                else
-                    hipLaunchKernelGGL((k_gemv_chain<N, L, false, false>), gc, bl, 0, st, c, m, n, A, lda, bx, b3,
-                                       part);
             }
         } else if (pass == 1) {
             if (split) {
@@ -210,7 +204,22 @@ struct Launch {
             else
                 hipLaunchKernelGGL((k_gemv_fold<N, L, TR, !SPLIT, false>), gr, bl, 0, st, c, m, n, A, lda, bx, b3,
                                    nvt, part);
+        } else if (pass == 2) {
+            if (split) {
+                const dim3 gc(grid_for((long long)m * b3, 1));
+                const bool narrow = (long long)lda * n <= (1LL << 28);
+                if (narrow && m % 64 == 0)
+                    hipLaunchKernelGGL((k_gemv_chain<N, L, true, true>), gc, bl, 0, st, c, m, n, A, lda, bx, b3, part);
+                else if (narrow)
+                    hipLaunchKernelGGL((k_gemv_chain<N, L, true, false>), gc, bl, 0, st, c, m, n, A, lda, bx, b3, part);
+                else
+                    hipLaunchKernelGGL((k_gemv_chain<N, L, false, false>), gc, bl, 0, st, c, m, n, A, lda, bx, b3,
+                                       part);
+            }
         } else if constexpr (SPLIT) {
+            if (split)
+                hipLaunchKernelGGL((k_gemv_scale<N, L>), dim3(grid_for(nvt * (N / GR), 1)), bl, 0, st, c, nvt, b3,
+                                   (const int *)EA, (const int *)EB, part);
             hipLaunchKernelGGL((k_gemv_fold<N, L, TR, true, true>), gr, bl, 0, st, c, m, n, A, lda, bx, b3, nvt, part);
         }
     }
@@ -585,6 +594,7 @@ struct Launch {
         o.syrk_v2 = &syrk_v2;
         o.scal = &scal;
         o.gemv_fold = &gemv_fold;
+        o.gemv_split = &gemv_split;
         o.gemv_final = &gemv_final;
         o.gemm = &gemm;
         o.gemm_ws = &gemm_ws;

@@ -66,7 +66,7 @@ struct Ops {
     hipError_t (*cmp)(const DevConsts *, int, int, const char *, const char *, const int *, const int *, int *,
                       hipStream_t);
     // the reduction launchers take a pass: 0 = main kernel, 1 = fixup kernel (see kernels.hpp);
-    // gemv_fold runs passes 0..2 and gemm / gemm_v2 passes 0..4 (see launch.hpp)
+    // gemv_fold runs passes 0..3 and gemm / gemm_v2 passes 0..4 (see launch.hpp)
     hipError_t (*fold)(const DevConsts *, int, int, long long, long long, SoA, int, SoA, int, const char *,
                        const char *, char *, hipStream_t);
     hipError_t (*tree)(const DevConsts *, char *, int, unsigned, int, char *, hipStream_t);
@@ -79,6 +79,7 @@ struct Ops {
     hipError_t (*scal)(const DevConsts *, int, SoA, SoA, int, char *, bool, hipStream_t);
     hipError_t (*gemv_fold)(const DevConsts *, bool, int, int, int, SoA, int, const char *, int, long long, char *,
                             char *, hipStream_t);
+    bool (*gemv_split)(bool trans, int b3);  // passes 1 and 2 of gemv_fold are independent
     size_t (*gemv_aux)(bool, int, int);
     hipError_t (*gemv_final)(const DevConsts *, int, SoA, int, const char *, hipStream_t);
     hipError_t (*gemm)(const DevConsts *, int, int, int, int, int, SoA, SoA, int, SoA, int, SoA, SoA, int, char *,
