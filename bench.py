@@ -39,7 +39,7 @@ PROF_GEMV_PAIR = 9         # library timing slot of the gemv residue and chain k
 # wave-instructions per launch of the gemm chains (SQ_INSTS_VALU). They are not measured
 # in this run, and the JSON line names them as profile values.
 PROFILE = "profiles/bench_424bit_counters.md"
-TRAFFIC_BYTES = {"chain": 149.3e6, "gemv": 2336.7e6, "gemv_chain": 685.8e6}
+TRAFFIC_BYTES = {"chain": 148.2e6, "gemv": 2319.9e6, "gemv_chain": 685.8e6}
 CHAIN_VALU_INSTS = 3.38e8
 # VALU issue peak: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction
 VALU_ISSUE_PEAK = 614.4e9
