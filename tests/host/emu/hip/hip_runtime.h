@@ -50,8 +50,15 @@ inline T __shfl(T v, int, int = 64) { return v; }
 template <class T>
 inline T __shfl_down(T v, unsigned, int = 64) { return v; }
 inline int atomicMin(int *p, int v) {
-    const int o = *p;
-    if (v < o) *p = v;
+    int o = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+    while (v < o && !__atomic_compare_exchange_n(p, &o, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {
+    }
+    return o;
+}
+inline int atomicMax(int *p, int v) {
+    int o = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+    while (v > o && !__atomic_compare_exchange_n(p, &o, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {
+    }
     return o;
 }
 inline int atomicAdd(int *p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }

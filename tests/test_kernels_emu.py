@@ -152,3 +152,31 @@ def test_spmv_split_form(emu, name, lo, hi):
     assert got.tobytes() == want.tobytes(), name
     # the cancelling pairs reach the fixup; with narrow exponents long rows reach their kernel
     assert counts[1] > 0 and (hi > 100 or counts[0] > 0), counts
+
+
+@pytest.mark.parametrize("name", ["16_4double", "32_8double"])
+@pytest.mark.parametrize("ncols_of", [lambda m: 3 * m, lambda m: m // 2], ids=["wide", "tall"])
+def test_spmv_split_rectangular(emu, name, ncols_of):
+    """Split-form CSR SpMV on a rectangular matrix: x has ncols records. The gathers read the
+    packed copy of x for the columns below min(m, largest column + 1) and x itself for the
+    others (wide); a tall matrix's x has fewer records than the matrix has rows, and only the
+    columns it uses are packed"""
+    o = po.Oracle(name)
+    sid = po.set_names().index(name)
+    m = 300
+    ncols = ncols_of(m)
+    irp, ja, as_ = gen.powerlaw_csr(83, m, kmax=200)
+    rng = np.random.default_rng(84)
+    ja = rng.integers(0, ncols, ja.size).astype(np.int32)  # CSR order kept, columns unsorted
+    ja[irp[1]] = ncols - 1  # the largest column is used
+    as_[::11] = 0.0
+    xs = gen.uexp(85, ncols, -30, 30)
+    xs[::7] = 0.0
+    x = o.set_d(xs, D)
+    want = o.spmv(m, irp, ja, as_, x, 1, D)
+    got = o.empty(m)
+    counts = np.zeros(2, np.int32)
+    assert emu.emu_spmv_split(sid, m, P(np.ascontiguousarray(irp, np.int32)), P(np.ascontiguousarray(ja, np.int32)),
+                              P(np.ascontiguousarray(as_)), P(x), P(got), P(counts)) == 0
+    assert got.tobytes() == want.tobytes(), name
+    assert counts[0] > 0  # long rows reached their kernel
