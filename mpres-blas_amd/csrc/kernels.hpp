@@ -1383,6 +1383,7 @@ constexpr int SGR = 4;        // residues per thread in the residue kernels
 constexpr int SP_LONG = 64;   // CSR rows with more terms take the per-workgroup residue kernel
 constexpr int SPL_T = 256;    // threads of k_spmv_resid_long
 constexpr int SP_PF = 4;      // terms of a chain whose x fields are in flight ahead of the step
+constexpr int SP_PF1 = 8;     // the same for one chain per row (CSR scalar kernel: long serial chains)
 #ifndef MPH_SP_TS
 #define MPH_SP_TS 4
 #endif
@@ -1401,7 +1402,7 @@ __device__ __forceinline__ XScal load_xscal(const AoSRef<N> &x, int col) {
 
 // perm (CSR): rows by decreasing length class (k_sp_bin_*), so that the chains of one
 // wavefront have similar lengths; rows marked REDO are appended to rlist (count *rcnt)
-template <int N, int FMT>
+template <int N, int FMT, int PF = SP_PF>
 __global__ __launch_bounds__(BLK) void k_spmv_chain(const DevConsts *__restrict__ C, int m, SpMat A, AoSRef<N> x,
                                                     AoSRef<N> y, int tpr, const int *__restrict__ perm, int *rcnt,
                                                     int *rlist) {
@@ -1417,7 +1418,7 @@ __global__ __launch_bounds__(BLK) void k_spmv_chain(const DevConsts *__restrict_
         return sc_muld_step(K, acc, o.se.x, o.se.y, Er{o.l.frac, (int)o.l.exp}, Er{o.u.frac, (int)o.u.exp}, v);
     };
     if constexpr (FMT == SP_CSR) {
-        // terms i, i + tpr, ... in blocks of SP_PF: x's fields of a block are gathered one
+        // terms i, i + tpr, ... in blocks of PF: x's fields of a block are gathered one
         // block ahead, its columns and values two blocks ahead, into fixed registers (a
         // ring that is never shifted, so a step waits only for its own operands)
         if (live) {
@@ -1425,24 +1426,24 @@ __global__ __launch_bounds__(BLK) void k_spmv_chain(const DevConsts *__restrict_
             int i = A.ptr[row] + lane;
             auto col = [&](int j) { return j < re ? A.ja[j] : 0; };
             auto val = [&](int j) { return j < re ? A.as[j] : 0.0; };
-            XScal b[SP_PF];
-            double vb[SP_PF], vn[SP_PF];
-            int cn[SP_PF];
+            XScal b[PF];
+            double vb[PF], vn[PF];
+            int cn[PF];
 #pragma unroll
-            for (int k = 0; k < SP_PF; k++) {
+            for (int k = 0; k < PF; k++) {
                 b[k] = load_xscal(x, col(i + k * tpr));
                 vb[k] = val(i + k * tpr);
-                cn[k] = col(i + (SP_PF + k) * tpr);
-                vn[k] = val(i + (SP_PF + k) * tpr);
+                cn[k] = col(i + (PF + k) * tpr);
+                vn[k] = val(i + (PF + k) * tpr);
             }
-            for (; i < re && ok; i += SP_PF * tpr) {
+            for (; i < re && ok; i += PF * tpr) {
 #pragma unroll
-                for (int k = 0; k < SP_PF; k++) {
+                for (int k = 0; k < PF; k++) {
                     if (ok && i + k * tpr < re) ok = step(b[k], vb[k]);
                     b[k] = load_xscal(x, cn[k]);
                     vb[k] = vn[k];
-                    cn[k] = col(i + (2 * SP_PF + k) * tpr);
-                    vn[k] = val(i + (2 * SP_PF + k) * tpr);
+                    cn[k] = col(i + (2 * PF + k) * tpr);
+                    vn[k] = val(i + (2 * PF + k) * tpr);
                 }
             }
         }

@@ -446,8 +446,12 @@ struct Launch {
                 hipLaunchKernelGGL((k_sp_bin_scatter<N, L>), gm, bl, 0, st, m, A.ptr, (const int *)off, cur, perm);
                 pm = perm;
             }
-            hipLaunchKernelGGL((k_spmv_chain<N, FMT>), dim3(grid_for((long long)m * tpr, 1)), bl, 0, st, c, m, A, X,
-                               Y, tpr, pm, rcnt, rlist);
+            if (FMT == SP_CSR && tpr == 1)  // long serial chains: a deeper ring of gathers
+                hipLaunchKernelGGL((k_spmv_chain<N, FMT, SP_PF1>), dim3(grid_for((long long)m, 1)), bl, 0, st, c, m,
+                                   A, X, Y, tpr, pm, rcnt, rlist);
+            else
+                hipLaunchKernelGGL((k_spmv_chain<N, FMT>), dim3(grid_for((long long)m * tpr, 1)), bl, 0, st, c, m, A,
+                                   X, Y, tpr, pm, rcnt, rlist);
             hipLaunchKernelGGL((k_spmv_resid<N, FMT>), dim3(grid_for((long long)m * (N / SGR) * SP_TS, 1)), bl, 0, st, c, m,
                                A, X, Y, pm, lcnt, FMT == SP_CSR ? llist : nullptr);
             if (FMT == SP_CSR)

@@ -194,7 +194,7 @@ int spmv_split_n(const Ctx &x, int m, const int *irp, const int *ja, const doubl
     for (unsigned b = 0; b < gm; b++) run_block(b, gm, BLK, [&] { k_sp_bin_count<N, 1>(m, irp, hist); });
     run(1, [&] { k_sp_bin_scan<N, 1>(hist, off); });
     for (unsigned b = 0; b < gm; b++) run_block(b, gm, BLK, [&] { k_sp_bin_scatter<N, 1>(m, irp, off, cur, perm); });
-    run(m, [&] { k_spmv_chain<N, SP_CSR>(&x.c, m, A, xa, ya, 1, perm, rcnt, rlist); });
+    run(m, [&] { k_spmv_chain<N, SP_CSR, SP_PF1>(&x.c, m, A, xa, ya, 1, perm, rcnt, rlist); });
     run((long long)m * (N / SGR) * SP_TS, [&] { k_spmv_resid<N, SP_CSR>(&x.c, m, A, xa, ya, perm, lcnt, llist); });
     // the long-row kernel's workgroups of SPL_T threads, one after another
     const int nb = *lcnt < 3 ? *lcnt : 3;  // fewer workgroups than rows: each takes several
