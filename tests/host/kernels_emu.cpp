@@ -206,9 +206,57 @@ int spmv_split_n(const Ctx &x, int m, const int *irp, const int *ja, const doubl
     return 0;
 }
 
+// the v1 (SoA) chain kernel and the generic AoS chain kernel over the same operands (A m x k,
+// B k x n, column-major): W1 from k_gemm_chain, W2 from k_gemm_chain_nn (UB or not); the
+// sums' sign, exponent and evaluation, or REDO
+template <int N>
+int chain_nn_n(const Ctx &x, int m, int n, int k, char *Ab, char *Bb, char *W1, char *W2, int ub) {
+    const AoSRef<N> a{Ab}, b{Bb}, w1{W1}, w2{W2};
+    struct Soa {
+        std::vector<int> d, s, e;
+        std::vector<ErMem> ev;
+        int len;
+        SoA view() { return SoA{d.data(), s.data(), e.data(), ev.data(), &len, len}; }
+    };
+    auto soa = [&](const AoSRef<N> &v, int cnt, Soa &o) {
+        o.len = cnt;
+        o.d.resize((size_t)cnt * N);
+        o.s.resize(cnt);
+        o.e.resize(cnt);
+        o.ev.resize(2 * (size_t)cnt);
+        for (int i = 0; i < cnt; i++) {
+            for (int r = 0; r < N; r++) o.d[(size_t)i * N + r] = v.digits(i)[r];
+            o.s[i] = *v.sign(i);
+            o.e[i] = *v.exp(i);
+            o.ev[i] = v.eval(i)[0];
+            o.ev[(size_t)cnt + i] = v.eval(i)[1];
+        }
+    };
+    Soa sa, sb;
+    soa(a, m * k, sa);
+    soa(b, k * n, sb);
+    const SoA SA = sa.view(), SB = sb.view();
+    run((long long)m * n, [&] { k_gemm_chain<N, 1, AoSRef<N>>(&x.c, 0, 0, m, n, k, a, m, b, k, w1); });
+    if (ub)
+        run((long long)m * n, [&] { k_gemm_chain_nn<N, 1, true>(&x.c, m, n, k, SA, m, SB, k, w2); });
+    else
+        run((long long)m * n, [&] { k_gemm_chain_nn<N, 1, false>(&x.c, m, n, k, SA, m, SB, k, w2); });
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+int emu_gemm_chain_nn(int set_id, int m, int n, int k, void *A, void *B, void *W1, void *W2, int ub) {
+    Ctx x;
+    if (!make_ctx(set_id, x)) return -1;
+    switch (x.c.n) {
+        case 16: return chain_nn_n<16>(x, m, n, k, (char *)A, (char *)B, (char *)W1, (char *)W2, ub);
+        case 32: return chain_nn_n<32>(x, m, n, k, (char *)A, (char *)B, (char *)W1, (char *)W2, ub);
+    }
+    return -1;
+}
 
 // counts: the long-row and marked-row list lengths
 int emu_spmv_split(int set_id, int m, const int *irp, const int *ja, const double *as, void *X, void *Y, int *counts) {

@@ -180,3 +180,30 @@ def test_spmv_split_rectangular(emu, name, ncols_of):
                               P(np.ascontiguousarray(as_)), P(x), P(got), P(counts)) == 0
     assert got.tobytes() == want.tobytes(), name
     assert counts[0] > 0  # long rows reached their kernel
+
+
+@pytest.mark.parametrize("name,lo,hi", [("16_4double", -30, 30), ("32_8double", -30, 30), ("32_8double", -400, 400),
+                                        ("16_32bit", -40, 40)])
+@pytest.mark.parametrize("n", [6, 7])
+@pytest.mark.parametrize("ub", [0, 1])
+def test_gemm_chain_nn(emu, name, lo, hi, n, ub):
+    """The v1 gemm's chain kernel (SoA operands at 32-bit offsets, B's fields wave-uniform
+    when UB) against the generic chain kernel on the same operands: every entry's sign,
+    exponent and evaluation, or its REDO mark, agree; zeros and exact cancellations abandon
+    some chains"""
+    o = po.Oracle(name)
+    sid = po.set_names().index(name)
+    m, k = 9, 21
+    As, Bs = gen.uexp(91, m * k, lo, hi), gen.uexp(92, k * n, lo, hi)
+    As[4] = 0.0
+    Bs[::5] = 0.0
+    # entry (2, 1): its first two products cancel exactly
+    As[2 + m * 0], As[2 + m * 1] = 1.5, 1.5
+    Bs[0 + k * 1], Bs[1 + k * 1] = 0.75, -0.75
+    A, B = o.set_d(As, D), o.set_d(Bs, D)
+    w1, w2 = o.empty(m * n), o.empty(m * n)
+    assert emu.emu_gemm_chain_nn(sid, m, n, k, P(A), P(B), P(w1), P(w2), ub) == 0
+    assert w1.tobytes() == w2.tobytes()
+    redo = (w1["sign"] != 0) & (w1["sign"] != 1)
+    assert redo.any() and (hi > 100 or not redo.all())  # wide exponents abandon every chain
+
