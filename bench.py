@@ -460,6 +460,38 @@ def exact_dot(a, b):
     return Fraction(s) * Fraction(2) ** e0
 
 
+def exact_dot_fast(a, b):
+    """exact_dot for long vectors: each product p q 2^e (p, q integers below 2^53) is split
+    into 16-bit pieces of its three 27-bit partial products (made nonnegative by a bias),
+    the pieces are summed per exponent with np.bincount (sums below 2^40: exact in float64),
+    and the per-exponent sums are recombined as Python integers"""
+    ma, ea = np.frexp(np.asarray(a, np.float64))
+    mb, eb = np.frexp(np.asarray(b, np.float64))
+    ia, ib = np.ldexp(ma, 53).astype(np.int64), np.ldexp(mb, 53).astype(np.int64)
+    e = ea.astype(np.int64) + eb.astype(np.int64) - 106
+    nz = (ia != 0) & (ib != 0)
+    if not nz.any():
+        return Fraction(0)
+    ia, ib, e = ia[nz], ib[nz], e[nz]
+    e0 = int(e.min())
+    k = e - e0
+    ph, qh = ia >> 27, ib >> 27
+    pl, ql = ia - (ph << 27), ib - (qh << 27)
+    bias = 1 << 55
+    nk = int(k.max()) + 1
+    cnt = np.bincount(k, minlength=nk)
+    total = 0
+    for t, sh in ((ph * qh, 54), (ph * ql + pl * qh, 27), (pl * ql, 0)):
+        u = t + bias  # 0 <= u < 2^56
+        parts = [np.bincount(k, weights=((u >> (16 * j)) & 0xFFFF).astype(np.float64), minlength=nk)
+                 for j in range(4)]
+        for g in range(nk):
+            if cnt[g]:
+                sg = sum(int(parts[j][g]) << (16 * j) for j in range(4)) - bias * int(cnt[g])
+                total += sg << (sh + g)
+    return Fraction(total) * Fraction(2) ** e0
+
+
 def rel_err(v, exact):
     if exact == 0:
         return 0.0 if v == 0 else float("inf")
@@ -486,7 +518,13 @@ def plain_lines(mp, threads, cpu=True):
         x, y, r = ctx.array_set_d(hx), ctx.array_set_d(hy), ctx.array(1)
         ms = time_call(mp, lambda: mp.check(L.mpres_dot(ctx.h, n, C.byref(x), 1, C.byref(y), 1, C.byref(r), 256, 64,
                                                         None, 0, None)), reps)
-        err = rel_err(exact_value(ctx.array_to_host(r, 1)[0], m_), exact_dot(hx, hy)) if n <= 65536 else None
+        if n <= 65536:
+            want = exact_dot(hx, hy)
+            if exact_dot_fast(hx, hy) != want:
+                raise RuntimeError("exact_dot_fast disagrees with exact_dot")
+        else:
+            want = exact_dot_fast(hx, hy)
+        err = rel_err(exact_value(ctx.array_to_host(r, 1)[0], m_), want)
         b = 2.0 * n * S
         line = {"op": "mp_dot", "moduli_set": "8_2double", "bits": ctx.precision, "n": n,
                 "order": "gridDim3=256, blockDim3=64", "lanes_per_number": L.mpres_get_lanes(ctx.h, mp.K_DOT),
