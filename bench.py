@@ -420,7 +420,7 @@ def mfma_lines(mp, size=2048, samples=((0, 0), (1000, 7), (2047, 2047), (5, 1500
             err_pl = sample_error(ctx, mp, A, B, C0, Cm, al, be, size, samples, m_)
             fmas = float(size) ** 3
             out.append({"moduli_set": name, "bits": ctx.precision, "exponents": [lo, hi], "size": size,
-                        "lanes_per_number": ctx.L.mpres_get_lanes(ctx.h, mp.K_MFMA),
+                        "lanes_per_number": ctx.L.mpres_get_lanes_for(ctx.h, mp.K_MFMA, size ** 3),
                         "mfma_ms": t_mf, "mfma_mpfma_per_s": fmas / (t_mf / 1e3),
                         "plain_ms": t_pl, "plain_mpfma_per_s": fmas / (t_pl / 1e3), "speedup": t_pl / t_mf,
                         "mfma_max_rel_err": err_mf, "mfma_product_only_max_rel_err": err_prod,
@@ -527,7 +527,7 @@ def plain_lines(mp, threads, cpu=True):
         err = rel_err(exact_value(ctx.array_to_host(r, 1)[0], m_), want)
         b = 2.0 * n * S
         line = {"op": "mp_dot", "moduli_set": "8_2double", "bits": ctx.precision, "n": n,
-                "order": "gridDim3=256, blockDim3=64", "lanes_per_number": L.mpres_get_lanes(ctx.h, mp.K_DOT),
+                "order": "gridDim3=256, blockDim3=64", "lanes_per_number": L.mpres_get_lanes_for(ctx.h, mp.K_DOT, n),
                 "ms": ms, "mpfma_per_s": n / (ms / 1e3),
                 "bytes": b, "gbs": b / (ms / 1e3) / 1e9, "hbm_frac": b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
                 "max_rel_err": err}
@@ -560,7 +560,7 @@ def plain_lines(mp, threads, cpu=True):
         got = exact_value(fetch_range(ctx, mp, y1, i, 1)[0], m_)
         aerr = max(aerr, rel_err(got, Fraction(float(ha)) * Fraction(float(hx[i])) + Fraction(float(hy[i]))))
     out.append({"op": "mp_axpy", "moduli_set": "8_2double", "bits": ctx.precision, "n": n, "ms": ms,
-                "lanes_per_number": L.mpres_get_lanes(ctx.h, mp.K_AXPY),
+                "lanes_per_number": L.mpres_get_lanes_for(ctx.h, mp.K_AXPY, n),
                 "mpfma_per_s": n / (ms / 1e3), "bytes": b, "gbs": b / (ms / 1e3) / 1e9,
                 "hbm_frac": b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, "max_rel_err_sampled": aerr,
                 "error_samples": "5 elements of y = alpha*x + y after one call, against the exact value"})
@@ -582,7 +582,7 @@ def plain_lines(mp, threads, cpu=True):
     rows = (0, 1, 12345, m - 1)
     want = {i: exact_dot(as_[irp[i]:irp[i + 1]], hx[ja[irp[i]:irp[i + 1]]]) for i in rows}
     b = nnz * 12.0 + (m + 1) * 4.0 + 2.0 * m * S
-    for tpr in (None, 4, 16, 32):
+    for tpr in (None, 4, 8, 16, 32):
         if tpr is None:
             fn = lambda: mp.check(L.mpres_spmv_csr(ctx.h, m, C.byref(csr), xs.ptr, ys.ptr, None))  # noqa: E731
         else:
@@ -592,7 +592,7 @@ def plain_lines(mp, threads, cpu=True):
         err = max(rel_err(exact_value(yh[i], m_), want[i]) for i in rows)
         out.append({"op": "mp_spmv_csr" if tpr is None else "mp_spmv_csrv", "threads_per_row": tpr,
                     "moduli_set": "16_4double", "bits": ctx.precision, "rows": m, "nnz": nnz, "ms": ms,
-                    "lanes_per_number": L.mpres_get_lanes(ctx.h, mp.K_SPMV),
+                    "lanes_per_number": L.mpres_get_lanes_for(ctx.h, mp.K_SPMV, m),
                     "mpfma_per_s": nnz / (ms / 1e3), "bytes": b, "gbs": b / (ms / 1e3) / 1e9,
                     "hbm_frac": b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, "max_rel_err_sampled_rows": err})
     if cpu:
@@ -819,11 +819,11 @@ def ctx_moduli(ctx):
 
 
 def gemv_lanes(ctx, mp):
-    return ctx.L.mpres_get_lanes(ctx.h, mp.K_GEMV)
+    return ctx.L.mpres_get_lanes_for(ctx.h, mp.K_GEMV, M_GEMV * N_GEMV)
 
 
 def gemm_lanes(ctx, mp):
-    return ctx.L.mpres_get_lanes(ctx.h, mp.K_GEMM)
+    return ctx.L.mpres_get_lanes_for(ctx.h, mp.K_GEMM, M_GEMM * N_GEMM * K_GEMM)
 
 
 def accuracy(ctx, mp, A, x, y0, y, alpha, beta, Ag, Bg, C0, Cg, al_g, be_g, m):

@@ -480,10 +480,25 @@ int mpres_profile_read(mpres_ctx_t *ctx, int op, double *ms, long *launches, int
  * Lanes per multiple-precision number used by the kernels of an operation
  * (0 = the built-in choice). op: 0 elementwise, 1 dot, 2 gemv, 3 gemm, 4 spmv, 5 axpy,
  * 7 precision-form gemm (mpres_gemm_mfma).
- * Only values compiled into the library are accepted (mpres_lanes_supported). */
+ * Only values compiled into the library are accepted (mpres_lanes_supported).
+ * The built-in choice for the 8_2double, 16_4double and 32_8double sets is a table by call
+ * size measured on MI355X (mpres-blas_amd/tuning/lanes_gfx950.json) for elementwise, dot,
+ * gemv, spmv and axpy; mpres_set_lanes with a nonzero count replaces it by that fixed count,
+ * with 0 restores it. mpres_get_lanes: the choice without a table. */
 int mpres_lanes_supported(const mpres_ctx_t *ctx, int lanes);
 int mpres_set_lanes(mpres_ctx_t *ctx, int op, int lanes);
 int mpres_get_lanes(const mpres_ctx_t *ctx, int op);  /* lanes in use, or a negative error */
+/* A table of lane counts by call size, standing in for the reference's compile-time launch
+ * constants (src/kernel_config.cuh) chosen per size: a call of operation op whose size is at
+ * most bounds[i] (the first such i; bounds increasing) uses lanes[i], a larger one
+ * lanes[count - 1]; count = 0 removes the table (mpres_set_lanes' choice applies again).
+ * Size: the call's multiply-accumulate count (dot and axpy n, gemv m n, gemm and the
+ * precision form m n k), the row count for SpMV, the element count for mpres_elementwise.
+ * No reference counterpart; results do not depend on it. mpres-blas_amd/lanes_tune.py
+ * measures a table. mpres_get_lanes_for: the lanes a call of that size uses. */
+#define MPRES_LANES_TABLE_MAX 8
+int mpres_set_lanes_table(mpres_ctx_t *ctx, int op, int count, const long long *bounds, const int *lanes);
+int mpres_get_lanes_for(const mpres_ctx_t *ctx, int op, long long size);
 
 #ifdef __cplusplus
 }

@@ -72,6 +72,34 @@ const Ops *find_ops(int n, int L) { return find_variant(n, L); }
 // 8 and 32 moduli); mpres_set_lanes overrides. Element-parallel work with
 // enough numbers prefers one lane per number (no lane repeats a number's scalar
 // logic); reductions with few independent chains prefer more lanes.
+// Lanes by call size measured on MI355X (mpres-blas_amd/lanes_tune.py, the table in
+// mpres-blas_amd/tuning/lanes_gfx950.json): a call of size <= bound[i] uses lanes[i], a
+// larger one the last entry. Installed at context creation for these sets; mpres_set_lanes
+// replaces an operation's table by a fixed choice, mpres_set_lanes(op, 0) restores it.
+struct BuiltinLanes {
+    const char *set;
+    int op, count;
+    long long bound[4];
+    int lanes[4];
+};
+const BuiltinLanes kBuiltinLanes[] = {
+    {"8_2double", OP_DOT, 4, {16384LL, 262144LL, 4194304LL, 16777216LL}, {8, 8, 8, 8}},
+    {"8_2double", OP_AXPY, 4, {16384LL, 262144LL, 4194304LL, 16777216LL}, {8, 2, 1, 1}},
+    {"8_2double", OP_GEMV, 4, {16384LL, 262144LL, 4194304LL, 16777216LL}, {8, 8, 8, 1}},
+    {"8_2double", OP_SPMV, 3, {31622LL, 316227LL, 1000000LL}, {8, 8, 8}},
+    {"8_2double", OP_ELEM, 3, {16384LL, 262144LL, 1048576LL}, {8, 2, 2}},
+    {"16_4double", OP_DOT, 4, {16384LL, 262144LL, 4194304LL, 16777216LL}, {16, 16, 4, 4}},
+    {"16_4double", OP_AXPY, 4, {16384LL, 262144LL, 4194304LL, 16777216LL}, {16, 4, 1, 1}},
+    {"16_4double", OP_GEMV, 4, {16384LL, 262144LL, 4194304LL, 16777216LL}, {16, 16, 16, 16}},
+    {"16_4double", OP_SPMV, 3, {31622LL, 316227LL, 1000000LL}, {2, 4, 4}},
+    {"16_4double", OP_ELEM, 3, {16384LL, 262144LL, 1048576LL}, {16, 4, 4}},
+    {"32_8double", OP_DOT, 4, {16384LL, 262144LL, 4194304LL, 16777216LL}, {32, 8, 8, 8}},
+    {"32_8double", OP_AXPY, 4, {16384LL, 262144LL, 4194304LL, 16777216LL}, {8, 8, 1, 1}},
+    {"32_8double", OP_GEMV, 4, {16384LL, 262144LL, 4194304LL, 16777216LL}, {32, 32, 32, 32}},
+    {"32_8double", OP_SPMV, 3, {31622LL, 316227LL, 1000000LL}, {1, 2, 4}},
+    {"32_8double", OP_ELEM, 3, {16384LL, 262144LL, 1048576LL}, {32, 8, 8}},
+};
+
 int default_lanes(int n, int op) {
     switch (n) {
         case 8: return (op == OP_AXPY || op == OP_MFMA) ? 1 : 8;
@@ -90,6 +118,13 @@ struct mpres_ctx {
     int *dpow2 = nullptr;
     int bits = 0;  // bit length of the largest modulus
     const Ops *ops[OP_COUNT] = {};
+    // per operation, an optional table of lane counts by call size (mpres_set_lanes_table):
+    // entry i serves calls of size <= bound[i], the last entry larger ones
+    struct LanesTable {
+        int count = 0;
+        long long bound[MPRES_LANES_TABLE_MAX] = {};
+        const Ops *ops[MPRES_LANES_TABLE_MAX] = {};
+    } lt[OP_COUNT];
     void *ws = nullptr;
     size_t ws_bytes = 0;
     // a second stream (non-blocking, default priority) for work of one call that does not
@@ -111,6 +146,30 @@ struct mpres_ctx {
 namespace {
 
 size_t mpsize(const mpres_ctx *c) { return (size_t)(4 * c->host.n + 40); }
+// the built-in lanes table of operation op for the context's moduli set, if there is one
+// (every listed lane count is compiled; an entry that is not leaves the table out)
+void install_builtin_lanes(mpres_ctx *c, int op) {
+    auto &t = c->lt[op];
+    t.count = 0;
+    const char *name = get_set(c->set_id)->name;
+    for (const BuiltinLanes &b : kBuiltinLanes) {
+        if (b.op != op || std::strcmp(b.set, name) != 0) continue;
+        for (int i = 0; i < b.count; i++) {
+            t.bound[i] = b.bound[i];
+            t.ops[i] = find_ops(c->host.n, b.lanes[i]);
+            if (!t.ops[i]) return;
+        }
+        t.count = b.count;
+        return;
+    }
+}
+// the kernel variant of operation op for a call of the given size (mpres_set_lanes_table)
+const Ops *sized(const mpres_ctx *c, int op, long long size) {
+    const auto &t = c->lt[op];
+    for (int i = 0; i < t.count; i++)
+        if (size <= t.bound[i]) return t.ops[i];
+    return t.count ? t.ops[t.count - 1] : c->ops[op];
+}
 // the split-form SpMV's workspace: counts, 3 x SP_NB class counters, the row order and two row lists
 size_t spmv_ws_bytes(int m) { return 4 * (2 + 3 * (size_t)SP_NB + 3 * (size_t)m); }
 
@@ -282,6 +341,7 @@ int mpres_ctx_create(mpres_ctx_t **out, int set_id, int device) {
             delete c;
             return fail(MPRES_ERR_UNSUPPORTED, "no kernels compiled for %d moduli", s->n);
         }
+        install_builtin_lanes(c, op);
     }
     if (hipMalloc(&c->dpow2, pow2.size() * sizeof(int)) != hipSuccess ||
         hipMalloc(&c->dev, sizeof(DevConsts)) != hipSuccess) {
@@ -523,8 +583,8 @@ int mpres_elementwise(mpres_ctx_t *c, int op, int count, const void *x, const vo
     if (!c || op < 0 || op > 4) return fail(MPRES_ERR_ARG, "invalid operation %d", op);
     if (count <= 0) return MPRES_OK;
     if (!x || !y || !r) return fail(MPRES_ERR_ARG, "null argument");
-    HIPCHK(c->ops[OP_ELEM]->elem(c->dev, op, count, (const char *)x, (const char *)y, (const double *)y, (char *)r,
-                                 strm(st)));
+    HIPCHK(sized(c, OP_ELEM, count)->elem(c->dev, op, count, (const char *)x, (const char *)y, (const double *)y,
+                                          (char *)r, strm(st)));
     return MPRES_OK;
 }
 
@@ -593,7 +653,7 @@ static int dot_common(mpres_ctx_t *c, WsUse &wu, int mode, int n, SoA xs, int in
     if (rc) return rc;
     const long long G = (long long)g3 * b3;
     char *p1 = w, *p2 = p1 + S * G, *p3 = p2 + S * g3, *p4 = p3 + S * b3;
-    const Ops *o = c->ops[OP_DOT];
+    const Ops *o = sized(c, OP_DOT, n);
     const unsigned P = next_pow2((unsigned)b3);
     // pass 1: strided fold over the products, halving tree per block of b3 (src/mpreduct.cuh:38-74)
     {
@@ -780,15 +840,15 @@ int mpres_axpy(mpres_ctx_t *c, int n, const mp_array_t *alpha, const mp_array_t 
     if (!c || !alpha || !x || !y) return fail(MPRES_ERR_ARG, "null argument");
     if (n <= 0) return MPRES_OK;  // src/blas/axpy.cuh:49-51
     ProfScope ps(c, OP_AXPY, strm(st));
-    HIPCHK(c->ops[OP_AXPY]->axpy(c->dev, n, soa_of(alpha), soa_of(x), incx, soa_of(y), incy, strm(st)));
+    HIPCHK(sized(c, OP_AXPY, n)->axpy(c->dev, n, soa_of(alpha), soa_of(x), incx, soa_of(y), incy, strm(st)));
     return MPRES_OK;
 }
 
 int mpres_axpy_v2(mpres_ctx_t *c, int n, const void *alpha, const void *x, const void *y, void *r, void *st) {
     if (!c || !alpha || !x || !y || !r) return fail(MPRES_ERR_ARG, "null argument");
     if (n <= 0) return MPRES_OK;
-    HIPCHK(c->ops[OP_AXPY]->axpy_v2(c->dev, n, (const char *)alpha, (const char *)x, (const char *)y, (char *)r,
-                                     false, strm(st)));
+    HIPCHK(sized(c, OP_AXPY, n)->axpy_v2(c->dev, n, (const char *)alpha, (const char *)x, (const char *)y,
+                                         (char *)r, false, strm(st)));
     return MPRES_OK;
 }
 
@@ -796,8 +856,8 @@ int mpres_axpy_v2(mpres_ctx_t *c, int n, const void *alpha, const void *x, const
 int mpres_maxpy_v2(mpres_ctx_t *c, int n, const void *alpha, const void *x, const void *y, void *r, void *st) {
     if (!c || !alpha || !x || !y || !r) return fail(MPRES_ERR_ARG, "null argument");
     if (n <= 0) return MPRES_OK;
-    HIPCHK(c->ops[OP_AXPY]->axpy_v2(c->dev, n, (const char *)alpha, (const char *)x, (const char *)y, (char *)r,
-                                     true, strm(st)));
+    HIPCHK(sized(c, OP_AXPY, n)->axpy_v2(c->dev, n, (const char *)alpha, (const char *)x, (const char *)y,
+                                         (char *)r, true, strm(st)));
     return MPRES_OK;
 }
 
@@ -861,7 +921,7 @@ int mpres_gemv(mpres_ctx_t *c, int trans, int m, int n, const mp_array_t *alpha,
     int rc = wu.get(ws, wsb, mpres_gemv_workspace(c, trans, m, n, b3), &w);
     if (rc) return rc;
     char *bx = w, *part = bx + S * lx, *sums = part + S * (size_t)ly * b3, *aux = sums + S * (size_t)ly;
-    const Ops *o = c->ops[OP_GEMV];
+    const Ops *o = sized(c, OP_GEMV, (long long)m * n);
     SoA ys = soa_of(y);
     // buffer1 = alpha * x, y = beta * y (src/blas/gemv.cuh:175-190)
     HIPCHK(o->scal(c->dev, lx, soa_of(alpha), soa_of(x), incx, bx, false, strm(st)));
@@ -914,7 +974,7 @@ int mpres_gemm(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const mp_arr
     WsUse wu(c, strm(st));
     int rc = wu.get(ws, wsb, mpres_gemm_workspace(c, m, n, k), &w);
     if (rc) return rc;
-    const Ops *o = c->ops[OP_GEMM];
+    const Ops *o = sized(c, OP_GEMM, (long long)m * n * k);
     // the residue path (pass 0) runs on the side stream while the chains (pass 1) run on st
     Fork fk(c, strm(st));
     {
@@ -955,7 +1015,7 @@ int mpres_gemm_mfma(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const m
     int rc = wu.get(ws, wsb, mpres_gemm_mfma_workspace(c, m, n, k), &w);
     if (rc) return rc;
     ProfScope ps(c, PROF_GEMM_MFMA, strm(st));
-    HIPCHK(c->ops[OP_MFMA]->gemm_mfma(c->dev, c->bits, c->host.log2m, tra, trb, m, n, k, soa_of(alpha), soa_of(A), lda,
+    HIPCHK(sized(c, OP_MFMA, (long long)m * n * k)->gemm_mfma(c->dev, c->bits, c->host.log2m, tra, trb, m, n, k, soa_of(alpha), soa_of(A), lda,
                                       soa_of(B), ldb, soa_of(beta), soa_of(C), ldc, w, strm(st)));
     return MPRES_OK;
 }
@@ -971,10 +1031,11 @@ int mpres_gemm_v2(mpres_ctx_t *c, int ta, int tb, int m, int n, int k, const voi
     if (rc) return rc;
     const int tra = ta == MPRES_TRANS || ta == MPRES_CONJ_TRANS, trb = tb == MPRES_TRANS || tb == MPRES_CONJ_TRANS;
     // as in mpres_gemm: the residue path (pass 0) on the side stream, the chains on st
+    const Ops *o = sized(c, OP_GEMM, (long long)m * n * k);
     Fork fk(c, strm(st));
     for (int pass = 0; pass < 5; pass++) {
         if (pass == 2) HIPCHK(fk.join());
-        HIPCHK(c->ops[OP_GEMM]->gemm_v2(c->dev, pass, c->bits, tra, trb, m, n, k, (const char *)alpha, (const char *)A,
+        HIPCHK(o->gemm_v2(c->dev, pass, c->bits, tra, trb, m, n, k, (const char *)alpha, (const char *)A,
                                         lda, (const char *)B, ldb, (const char *)beta, (char *)C, ldc, w,
                                         pass == 0 ? fk.s1 : strm(st)));
     }
@@ -988,7 +1049,7 @@ static int spmv_common(mpres_ctx_t *c, int fmt, int m, const SpMat &A, int tpr, 
                        int *list) {
     if (m <= 0) return MPRES_OK;
     if (tpr < 1 || tpr > 32 || (tpr & (tpr - 1))) return fail(MPRES_ERR_ARG, "threadsPerRow %d not in 1..32", tpr);
-    const Ops *o = c->ops[OP_SPMV];
+    const Ops *o = sized(c, OP_SPMV, m);
     if (tpr > 256 / o->L) {
         // the tree of a row must fit one workgroup: fall back to a narrower group
         for (int L = o->L / 2; L >= 1; L /= 2) {
@@ -1528,12 +1589,42 @@ int mpres_set_lanes(mpres_ctx_t *c, int op, int lanes) {
     const Ops *o = find_ops(c->host.n, lanes == 0 ? default_lanes(c->host.n, op) : lanes);
     if (!o) return fail(MPRES_ERR_UNSUPPORTED, "%d lanes per number not compiled for %d moduli", lanes, c->host.n);
     c->ops[op] = o;
+    // a fixed choice replaces the size table; 0 restores the built-in one
+    if (lanes == 0)
+        install_builtin_lanes(c, op);
+    else
+        c->lt[op].count = 0;
     return MPRES_OK;
 }
 
 int mpres_get_lanes(const mpres_ctx_t *c, int op) {
     if (!c || !lanes_op(op)) return fail(MPRES_ERR_ARG, "invalid operation %d", op);
     return c->ops[op]->L;
+}
+
+int mpres_set_lanes_table(mpres_ctx_t *c, int op, int count, const long long *bounds, const int *lanes) {
+    if (!c || !lanes_op(op)) return fail(MPRES_ERR_ARG, "invalid operation %d", op);
+    if (count < 0 || count > MPRES_LANES_TABLE_MAX) return fail(MPRES_ERR_ARG, "table of %d entries", count);
+    if (count > 0 && (!bounds || !lanes)) return fail(MPRES_ERR_ARG, "null argument");
+    const Ops *ops[MPRES_LANES_TABLE_MAX];
+    for (int i = 0; i < count; i++) {
+        if (i > 0 && bounds[i] <= bounds[i - 1]) return fail(MPRES_ERR_ARG, "bounds must increase");
+        ops[i] = find_ops(c->host.n, lanes[i]);
+        if (!ops[i])
+            return fail(MPRES_ERR_UNSUPPORTED, "%d lanes per number not compiled for %d moduli", lanes[i], c->host.n);
+    }
+    auto &t = c->lt[op];
+    t.count = count;
+    for (int i = 0; i < count; i++) {
+        t.bound[i] = bounds[i];
+        t.ops[i] = ops[i];
+    }
+    return MPRES_OK;
+}
+
+int mpres_get_lanes_for(const mpres_ctx_t *c, int op, long long size) {
+    if (!c || !lanes_op(op)) return fail(MPRES_ERR_ARG, "invalid operation %d", op);
+    return sized(c, op, size)->L;
 }
 
 }  // extern "C"

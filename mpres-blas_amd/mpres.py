@@ -14,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmpres_hip.so")
 
 MPRES_OK = 0
+MPRES_ERR_ARG, MPRES_ERR_HIP, MPRES_ERR_WORKSPACE, MPRES_ERR_UNSUPPORTED = -1, -2, -3, -4
 NO_TRANS, TRANS, CONJ_TRANS = 111, 112, 113
 OP_ADD, OP_SUB, OP_MUL, OP_MUL_D, OP_VADD = 0, 1, 2, 3, 4
 # operation classes of mpres_set_lanes
@@ -211,6 +212,8 @@ def lib():
         L.mpres_lanes_supported.argtypes = [vp, i32]
         L.mpres_set_lanes.argtypes = [vp, i32, i32]
         L.mpres_get_lanes.argtypes = [vp, i32]
+        L.mpres_set_lanes_table.argtypes = [vp, i32, i32, C.POINTER(C.c_longlong), C.POINTER(C.c_int)]
+        L.mpres_get_lanes_for.argtypes = [vp, i32, C.c_longlong]
         _lib = L
     return _lib
 
@@ -401,6 +404,16 @@ class Context:
 
     def set_lanes(self, op, lanes):
         check(self.L.mpres_set_lanes(self.h, op, lanes))
+
+    def set_lanes_table(self, op, table):
+        """table: [(bound, lanes), ...] with increasing bounds; [] removes the table"""
+        n = len(table)
+        b = (C.c_longlong * max(n, 1))(*[int(t[0]) for t in table])
+        l_ = (C.c_int * max(n, 1))(*[int(t[1]) for t in table])
+        check(self.L.mpres_set_lanes_table(self.h, op, n, b, l_))
+
+    def lanes_for(self, op, size):
+        return self.L.mpres_get_lanes_for(self.h, op, int(size))
 
     # ---- mp_array_t (SoA) ----
     def array(self, size):

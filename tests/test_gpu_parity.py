@@ -206,6 +206,54 @@ def test_dot(name, n, g3, b3):
         assert abs(got - ex) / abs(ex) < 2.0 ** (-c.precision + 12)
 
 
+@pytest.mark.parametrize("name", ["8_2double", "32_8double"])
+def test_lanes_table(name):
+    """mpres_set_lanes_table: calls pick their lanes by size (first bound at or above the size,
+    the last entry beyond), results stay those of the oracle, bad tables are refused and an
+    empty table restores the per-operation choice"""
+    c, o = ctx(name), orc(name)
+    lanes = lanes_of(c)
+    small, large = lanes[0], lanes[-1]
+    default = c.L.mpres_get_lanes(c.h, mp.K_DOT)
+    try:
+        c.set_lanes_table(mp.K_DOT, [(1000, small), (100000, large)])
+        assert (c.lanes_for(mp.K_DOT, 1), c.lanes_for(mp.K_DOT, 1000), c.lanes_for(mp.K_DOT, 1001),
+                c.lanes_for(mp.K_DOT, 10 ** 12)) == (small, small, large, large)
+        for n, seed in ((700, 61), (5000, 62)):
+            note("lanes table dot", name, n)
+            x, y = o.set_d(gen.uexp(seed, n, -20, 20), D), o.set_d(gen.uexp(seed + 1, n, -20, 20), D)
+            ax, ay, r = c.array_from_host(x), c.array_from_host(y), c.array(1)
+            mp.check(c.L.mpres_dot(c.h, n, C.byref(ax), 1, C.byref(ay), 1, C.byref(r), 16, 32, None, 0, None))
+            same(c.array_to_host(r, 1), o.dot(x, y, 16, 32, D), f"dot {name} n={n} with a lanes table")
+        B = C.c_longlong * 2
+        Lx = C.c_int * 2
+        assert c.L.mpres_set_lanes_table(c.h, mp.K_DOT, 2, B(10, 10), Lx(small, small)) == mp.MPRES_ERR_ARG
+        assert c.L.mpres_set_lanes_table(c.h, mp.K_DOT, 1, B(10, 0), Lx(3, 0)) == mp.MPRES_ERR_UNSUPPORTED
+        assert c.L.mpres_set_lanes_table(c.h, mp.K_DOT, 9, B(1, 2), Lx(small, small)) == mp.MPRES_ERR_ARG
+        assert c.L.mpres_set_lanes_table(c.h, 6, 1, B(1, 2), Lx(small, small)) == mp.MPRES_ERR_ARG
+        c.set_lanes_table(mp.K_DOT, [])
+        assert c.lanes_for(mp.K_DOT, 1000) == c.lanes_for(mp.K_DOT, 10 ** 9) == default
+        c.set_lanes(mp.K_DOT, small)  # a fixed choice replaces any table
+        assert c.lanes_for(mp.K_DOT, 10 ** 9) == small
+    finally:
+        c.set_lanes(mp.K_DOT, 0)  # the built-in choice again
+
+
+def test_builtin_lanes_tables():
+    """the measured lanes tables installed at context creation (424-bit gemv of 4096^2 on 32
+    lanes, 106-bit gemv of 4096^2 on one), set_lanes overriding them and 0 restoring them"""
+    c = ctx("32_8double")
+    c.set_lanes(mp.K_GEMV, 0)
+    assert c.lanes_for(mp.K_GEMV, 4096 * 4096) == 32
+    c.set_lanes(mp.K_GEMV, 8)
+    assert c.lanes_for(mp.K_GEMV, 4096 * 4096) == 8
+    c.set_lanes(mp.K_GEMV, 0)
+    assert c.lanes_for(mp.K_GEMV, 4096 * 4096) == 32
+    c8 = ctx("8_2double")
+    c8.set_lanes(mp.K_GEMV, 0)
+    assert (c8.lanes_for(mp.K_GEMV, 64 * 64), c8.lanes_for(mp.K_GEMV, 4096 * 4096)) == (8, 1)
+
+
 def _get_d_exact(rec, m):
     """(-1)^sign X 2^exp as the reference's MPFR conversion gives it: X rounded to 53 bits to
     nearest (ties to even), then to a double (C ldexp: one more rounding when subnormal)"""
