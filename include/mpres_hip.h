@@ -483,7 +483,7 @@ int mpres_profile_read(mpres_ctx_t *ctx, int op, double *ms, long *launches, int
  * Only values compiled into the library are accepted (mpres_lanes_supported).
  * The built-in choice for the 8_2double, 16_4double and 32_8double sets is a table by call
  * size measured on MI355X (mpres-blas_amd/tuning/lanes_gfx950.json) for elementwise, dot,
- * gemv, spmv and axpy; mpres_set_lanes with a nonzero count replaces it by that fixed count,
+ * gemv, gemm, spmv and axpy; mpres_set_lanes with a nonzero count replaces it by that fixed count,
  * with 0 restores it. mpres_get_lanes: the choice without a table. */
 int mpres_lanes_supported(const mpres_ctx_t *ctx, int lanes);
 int mpres_set_lanes(mpres_ctx_t *ctx, int op, int lanes);

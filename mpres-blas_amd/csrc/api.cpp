@@ -86,16 +86,19 @@ const BuiltinLanes kBuiltinLanes[] = {
     {"8_2double", OP_DOT, 4, {16384LL, 262144LL, 4194304LL, 16777216LL}, {8, 8, 8, 8}},
     {"8_2double", OP_AXPY, 4, {16384LL, 262144LL, 4194304LL, 16777216LL}, {8, 2, 1, 1}},
     {"8_2double", OP_GEMV, 4, {16384LL, 262144LL, 4194304LL, 16777216LL}, {8, 8, 8, 1}},
+    {"8_2double", OP_GEMM, 4, {741455LL, 5931641LL, 47453132LL, 134217728LL}, {8, 8, 2, 1}},
     {"8_2double", OP_SPMV, 3, {31622LL, 316227LL, 1000000LL}, {8, 8, 8}},
     {"8_2double", OP_ELEM, 3, {16384LL, 262144LL, 1048576LL}, {8, 2, 2}},
     {"16_4double", OP_DOT, 4, {16384LL, 262144LL, 4194304LL, 16777216LL}, {16, 16, 4, 4}},
     {"16_4double", OP_AXPY, 4, {16384LL, 262144LL, 4194304LL, 16777216LL}, {16, 4, 1, 1}},
     {"16_4double", OP_GEMV, 4, {16384LL, 262144LL, 4194304LL, 16777216LL}, {16, 16, 16, 16}},
+    {"16_4double", OP_GEMM, 4, {741455LL, 5931641LL, 47453132LL, 134217728LL}, {16, 16, 4, 4}},
     {"16_4double", OP_SPMV, 3, {31622LL, 316227LL, 1000000LL}, {2, 4, 4}},
     {"16_4double", OP_ELEM, 3, {16384LL, 262144LL, 1048576LL}, {16, 4, 4}},
     {"32_8double", OP_DOT, 4, {16384LL, 262144LL, 4194304LL, 16777216LL}, {32, 8, 8, 8}},
     {"32_8double", OP_AXPY, 4, {16384LL, 262144LL, 4194304LL, 16777216LL}, {8, 8, 1, 1}},
-    {"32_8double", OP_GEMV, 4, {16384LL, 262144LL, 4194304LL, 16777216LL}, {32, 32, 32, 32}},
+    {"32_8double", OP_GEMV, 4, {16384LL, 262144LL, 4194304LL, 16777216LL}, {32, 4, 8, 8}},
+    {"32_8double", OP_GEMM, 4, {741455LL, 5931641LL, 47453132LL, 134217728LL}, {8, 8, 8, 8}},
     {"32_8double", OP_SPMV, 3, {31622LL, 316227LL, 1000000LL}, {1, 2, 4}},
     {"32_8double", OP_ELEM, 3, {16384LL, 262144LL, 1048576LL}, {32, 8, 8}},
 };

@@ -5,8 +5,8 @@ The lanes per number of an operation's kernels never change a result, only the
 time, and the fastest count depends on the call's size: few lanes per number
 give more numbers per wave, many lanes shorten each number's critical path. For
 each moduli set, operation and size of a fixed list this script times every
-compiled lane count on synthetic inputs (u 2^e, e in [-20, 20]) and keeps the
-fastest. The table it writes (one entry per size, the bound of each bucket
+compiled lane count on synthetic inputs (u 2^e, e in [-20, 20]; gemv and gemm with
+beta = 0, so that repeated calls do the same work) and keeps the fastest. The table it writes (one entry per size, the bound of each bucket
 halfway, geometrically, to the next size) plays the part of the reference's
 compile-time launch constants (src/kernel_config.cuh), chosen by measurement
 per size; load() installs it in a context.
@@ -30,12 +30,14 @@ SIZES = {
     "axpy": [1 << 12, 1 << 16, 1 << 20, 1 << 24],
     "gemv": [64 * 64, 256 * 256, 1024 * 1024, 4096 * 4096],
     "spmv": [10 ** 4, 10 ** 5, 10 ** 6],
+    "gemm": [64 ** 3, 128 ** 3, 256 ** 3, 512 ** 3],
     "elem": [1 << 12, 1 << 16, 1 << 20],
 }
 
 
 def op_code(mp, op):
-    return {"dot": mp.K_DOT, "axpy": mp.K_AXPY, "gemv": mp.K_GEMV, "spmv": mp.K_SPMV, "elem": mp.K_ELEM}[op]
+    return {"dot": mp.K_DOT, "axpy": mp.K_AXPY, "gemv": mp.K_GEMV, "spmv": mp.K_SPMV, "elem": mp.K_ELEM,
+            "gemm": mp.K_GEMM}[op]
 
 
 def bounds_of(sizes):
@@ -69,10 +71,20 @@ def make_call(mp, gen, ctx, op, size, keep):
         m = int(math.isqrt(size))
         A = ctx.array_set_d(gen.uexp(21, m * m, -20, 20))
         x, y = ctx.array_set_d(gen.uexp(22, m, -20, 20)), ctx.array_set_d(gen.uexp(23, m, -20, 20))
-        al, be = ctx.array_set_d(gen.uexp(24, 1, -20, 20)), ctx.array_set_d(gen.uexp(25, 1, -20, 20))
+        # beta = 0: every repetition computes the same y (an accumulating y would drift in
+        # exponent and change the work from one call to the next)
+        al, be = ctx.array_set_d(gen.uexp(24, 1, -20, 20)), ctx.array_set_d([0.0])
         keep += [A, x, y, al, be]
         return lambda: mp.check(L.mpres_gemv(ctx.h, mp.NO_TRANS, m, m, C.byref(al), C.byref(A), m, C.byref(x), 1,
                                              C.byref(be), C.byref(y), 1, 32, None, 0, None))
+    if op == "gemm":
+        m = round(size ** (1 / 3))
+        A, B = ctx.array_set_d(gen.uexp(51, m * m, -20, 20)), ctx.array_set_d(gen.uexp(52, m * m, -20, 20))
+        Cm = ctx.array_set_d(gen.uexp(53, m * m, -20, 20))
+        al, be = ctx.array_set_d(gen.uexp(54, 1, -20, 20)), ctx.array_set_d([0.0])  # beta = 0, as for gemv
+        keep += [A, B, Cm, al, be]
+        return lambda: mp.check(L.mpres_gemm(ctx.h, mp.NO_TRANS, mp.NO_TRANS, m, m, m, C.byref(al), C.byref(A), m,
+                                             C.byref(B), m, C.byref(be), C.byref(Cm), m, None, 0, None))
     if op == "spmv":
         irp, ja, as_ = gen.powerlaw_csr(31, size)
         xs, ys = ctx.set_d(gen.uexp(32, size, -10, 10)), ctx.aos(count=size)

@@ -240,18 +240,22 @@ def test_lanes_table(name):
 
 
 def test_builtin_lanes_tables():
-    """the measured lanes tables installed at context creation (424-bit gemv of 4096^2 on 32
-    lanes, 106-bit gemv of 4096^2 on one), set_lanes overriding them and 0 restoring them"""
-    c = ctx("32_8double")
-    c.set_lanes(mp.K_GEMV, 0)
-    assert c.lanes_for(mp.K_GEMV, 4096 * 4096) == 32
-    c.set_lanes(mp.K_GEMV, 8)
-    assert c.lanes_for(mp.K_GEMV, 4096 * 4096) == 8
-    c.set_lanes(mp.K_GEMV, 0)
-    assert c.lanes_for(mp.K_GEMV, 4096 * 4096) == 32
+    """the measured lanes tables installed at context creation (106-bit gemv of 64^2 on 8
+    lanes and of 4096^2 on one, gemm of 512^3 on one), set_lanes overriding them and 0
+    restoring them"""
     c8 = ctx("8_2double")
-    c8.set_lanes(mp.K_GEMV, 0)
-    assert (c8.lanes_for(mp.K_GEMV, 64 * 64), c8.lanes_for(mp.K_GEMV, 4096 * 4096)) == (8, 1)
+    try:
+        c8.set_lanes(mp.K_GEMV, 0)
+        assert (c8.lanes_for(mp.K_GEMV, 64 * 64), c8.lanes_for(mp.K_GEMV, 4096 * 4096)) == (8, 1)
+        c8.set_lanes(mp.K_GEMV, 2)
+        assert c8.lanes_for(mp.K_GEMV, 4096 * 4096) == 2
+        c8.set_lanes(mp.K_GEMV, 0)
+        assert c8.lanes_for(mp.K_GEMV, 4096 * 4096) == 1
+        c8.set_lanes(mp.K_GEMM, 0)
+        assert c8.lanes_for(mp.K_GEMM, 512 ** 3) == 1
+    finally:
+        c8.set_lanes(mp.K_GEMV, 0)
+        c8.set_lanes(mp.K_GEMM, 0)
 
 
 def _get_d_exact(rec, m):
