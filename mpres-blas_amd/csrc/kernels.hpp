@@ -1377,11 +1377,10 @@ __global__ __launch_bounds__(BLK) void k_spmv(const DevConsts *__restrict__ C, i
 // and tree on the scalar fields alone (one thread per chain, where the fused kernel
 // repeats the scalar work on every lane of a number) and writes y's sign, exponent and
 // evaluation, or REDO; k_spmv_resid sums the residues of the row's terms against E, in
-// any order (the sum is exact), CSR rows longer than SP_LONG terms by k_spmv_resid_long
-// with a workgroup per row. REDO rows go to the fused kernel's fixup pass.
+// any order (the sum is exact), CSR rows longer than SP_LONG terms by k_spmv_resid_wave
+// with SPW_T threads per row. REDO rows go to the fused kernel's fixup pass.
 constexpr int SGR = 4;        // residues per thread in the residue kernels
-constexpr int SP_LONG = 64;   // CSR rows with more terms take the per-workgroup residue kernel
-constexpr int SPL_T = 256;    // threads of k_spmv_resid_long
+constexpr int SP_LONG = 64;   // CSR rows with more terms take the long-row residue kernel
 constexpr int SP_PF = 4;      // terms of a chain whose x fields are in flight ahead of the step
 constexpr int SP_PF1 = 8;     // the same for one chain per row (CSR scalar kernel: long serial chains)
 #ifndef MPH_SP_TS
@@ -1564,7 +1563,7 @@ __device__ __forceinline__ void sp_resid_slot(const DevConsts *__restrict__ C, c
 // groups by SP_TS term slots per row (slot t takes terms t, t + SP_TS, ...; the slots'
 // sums meet through shuffles), rows in perm's order (CSR). With a long-row list (CSR), a
 // row of more than SP_LONG terms is appended to it (count *lcnt) and left to
-// k_spmv_resid_long.
+// k_spmv_resid_wave.
 template <int N, int FMT>
 __global__ __launch_bounds__(BLK) void k_spmv_resid(const DevConsts *__restrict__ C, int m, SpMat A, AoSRef<N> x,
                                                     AoSRef<N> y, const int *__restrict__ perm, int *lcnt,
@@ -1633,16 +1632,23 @@ __global__ __launch_bounds__(BLK) void k_spmv_resid(const DevConsts *__restrict_
     }
 }
 
-// Residues of the long CSR rows listed by k_spmv_resid: one workgroup per row, SPL_T / (N /
-// SGR) term slots of N / SGR threads; the slots' reduced sums meet in LDS. Workgroups
-// take rows blockIdx.x, + gridDim.x, ... of the list, so any grid covers it.
+// Residues of the long CSR rows listed by k_spmv_resid, a group of SPW_T threads per row:
+// SPW_T / (N / SGR) term slots of N / SGR threads, whose reduced sums meet through shuffles
+// (the group is one wavefront, so no barrier). Groups take rows g, g + (groups of the grid),
+// ... of the list, so any grid covers it. (A workgroup per row, 64 slots joined in LDS,
+// left most slots idle on rows of a few hundred terms.)
+#ifndef MPH_SPW_T
+#define MPH_SPW_T 32
+#endif
+constexpr int SPW_T = MPH_SPW_T;  // threads per long row (half a wavefront; 1 slot per group on the host)
 template <int N>
-__global__ __launch_bounds__(SPL_T) void k_spmv_resid_long(const DevConsts *__restrict__ C, SpMat A, AoSRef<N> x,
-                                                           AoSRef<N> y, const int *__restrict__ lcnt,
-                                                           const int *__restrict__ llist) {
-    constexpr int G = N / SGR, TS = SPL_T / G;
-    __shared__ int part[TS][N + 1];
-    const int q = threadIdx.x % G, sl = threadIdx.x / G, g0 = q * SGR;
+__global__ __launch_bounds__(BLK) void k_spmv_resid_wave(const DevConsts *__restrict__ C, SpMat A, AoSRef<N> x,
+                                                         AoSRef<N> y, const int *__restrict__ lcnt,
+                                                         const int *__restrict__ llist) {
+    constexpr int G = N / SGR, TPR = SPW_T < G ? G : SPW_T, TS = TPR / G;
+    static_assert(64 % TPR == 0 && BLK % TPR == 0, "a row's threads in one wavefront");
+    const int t = threadIdx.x % TPR, q = t % G, sl = t / G, g0 = q * SGR;
+    const int grp = blockIdx.x * (BLK / TPR) + threadIdx.x / TPR, ngrp = gridDim.x * (BLK / TPR);
     double md[SGR], mv[SGR];
 #pragma unroll
     for (int r = 0; r < SGR; r++) {
@@ -1650,10 +1656,10 @@ __global__ __launch_bounds__(SPL_T) void k_spmv_resid_long(const DevConsts *__re
         mv[r] = C->minv[g0 + r];
     }
     const int count = *lcnt;
-    for (int k = blockIdx.x; k < count; k += gridDim.x) {
+    for (int k = grp; k < count; k += ngrp) {  // uniform over the row's group
         const int row = llist[k];
         const int E = *y.exp(row);
-        const int rs = A.ptr[row], re = A.ptr[row + 1];
+        const int rs = A.ptr[row], re = A.ptr[row + 1];  // a listed row has more than SP_LONG terms
         double acc[SGR] = {0.0, 0.0, 0.0, 0.0};
         // a slot's terms in blocks of at most GV_REDUCE, reduced between blocks
         for (int i = rs + sl; i < re; i += GV_REDUCE * TS) {
@@ -1662,17 +1668,23 @@ __global__ __launch_bounds__(SPL_T) void k_spmv_resid_long(const DevConsts *__re
 #pragma unroll
             for (int r = 0; r < SGR; r++) acc[r] = fmod_exact(acc[r], md[r], mv[r]);
         }
+        // the slots' sums (each below m, TS of them add exactly) through shuffles
+        double w[SGR];
 #pragma unroll
-        for (int r = 0; r < SGR; r++) part[sl][g0 + r] = (int)fmod_exact(acc[r], md[r], mv[r]);
-        __syncthreads();
-        if (threadIdx.x < N) {
-            const int r = threadIdx.x;
-            double s = 0.0;  // TS sums below 2^31: exact
-            for (int t = 0; t < TS; t++) s += (double)part[t][r];
-            const int w = (int)fmod_exact(s, C->md[r], C->minv[r]);
-            y.digits(row)[r] = (*y.sign(row) && w) ? C->m[r] - w : w;
+        for (int r = 0; r < SGR; r++) w[r] = fmod_exact(acc[r], md[r], mv[r]);
+#pragma unroll
+        for (int h = G; h < TPR; h <<= 1)
+#pragma unroll
+            for (int r = 0; r < SGR; r++) w[r] += __shfl_xor(w[r], h, 64);
+        if (sl == 0) {
+            const int sg = *y.sign(row);
+            int *dig = y.digits(row) + g0;
+#pragma unroll
+            for (int r = 0; r < SGR; r++) {
+                const int v = (int)fmod_exact(w[r], md[r], mv[r]);
+                dig[r] = (sg && v) ? C->m[g0 + r] - v : v;
+            }
         }
-        __syncthreads();
     }
 }
 

@@ -429,6 +429,11 @@ struct Launch {
     // marked rows recomputed by the fused kernel. ws: spmv_ws_ints(m) ints (split form):
     // long-row and marked-row counts, class counts, offsets and cursors, then the row
     // order, the long-row list and the marked-row list.
+    // workgroups of k_spmv_resid_wave: BLK / SPW_T rows each per sweep, at most 16384
+    static unsigned rows_blocks(int m) {
+        const long long b = ((long long)m + BLK / SPW_T - 1) / (BLK / SPW_T);
+        return (unsigned)(b < 16384 ? b : 16384);
+    }
     template <int FMT>
     static void spmv_fmt(const DevConsts *c, int pass, int m, const SpMat &A, const char *x, char *y, int tpr,
                          int *ws, hipStream_t st) {
@@ -455,7 +460,7 @@ struct Launch {
             hipLaunchKernelGGL((k_spmv_resid<N, FMT>), dim3(grid_for((long long)m * (N / SGR) * SP_TS, 1)), bl, 0, st, c, m,
                                A, X, Y, pm, lcnt, FMT == SP_CSR ? llist : nullptr);
             if (FMT == SP_CSR)
-                hipLaunchKernelGGL((k_spmv_resid_long<N>), dim3(m < 4096 ? m : 4096), dim3(SPL_T), 0, st, c, A, X, Y,
+                hipLaunchKernelGGL((k_spmv_resid_wave<N>), dim3(rows_blocks(m)), dim3(BLK), 0, st, c, A, X, Y,
                                    (const int *)lcnt, (const int *)llist);
         } else if (pass == 0) {
             hipLaunchKernelGGL((k_spmv<N, L, !SPLIT, false, FMT>), gr, bl, 0, st, c, m, A, X, Y, tpr, nullptr,

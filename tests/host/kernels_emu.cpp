@@ -196,9 +196,8 @@ int spmv_split_n(const Ctx &x, int m, const int *irp, const int *ja, const doubl
     for (unsigned b = 0; b < gm; b++) run_block(b, gm, BLK, [&] { k_sp_bin_scatter<N, 1>(m, irp, off, cur, perm); });
     run(m, [&] { k_spmv_chain<N, SP_CSR, SP_PF1>(&x.c, m, A, xa, ya, 1, perm, rcnt, rlist); });
     run((long long)m * (N / SGR) * SP_TS, [&] { k_spmv_resid<N, SP_CSR>(&x.c, m, A, xa, ya, perm, lcnt, llist); });
-    // the long-row kernel's workgroups of SPL_T threads, one after another
-    const int nb = *lcnt < 3 ? *lcnt : 3;  // fewer workgroups than rows: each takes several
-    for (int b = 0; b < nb; b++) run_block(b, nb, SPL_T, [&] { k_spmv_resid_long<N>(&x.c, A, xa, ya, lcnt, llist); });
+    // the long rows, two workgroups (fewer groups than rows: each takes several)
+    for (unsigned b = 0; b < 2; b++) run_block(b, 2, BLK, [&] { k_spmv_resid_wave<N>(&x.c, A, xa, ya, lcnt, llist); });
     // the fixup over the marked rows, two workgroups (each takes several list entries)
     for (unsigned b = 0; b < 2; b++) run_block(b, 2, BLK, [&] { k_spmv<N, 1, true, true, SP_CSR>(&x.c, m, A, xa, ya, 1, rcnt, rlist); });
     counts[0] = *lcnt;

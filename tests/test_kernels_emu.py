@@ -32,7 +32,7 @@ BUILDS = {
 def emu(request, tmp_path_factory):
     out = tmp_path_factory.mktemp("emu") / "libkemu.so"
     # one lane per thread: the SpMV residue kernel's term slots (combined by shuffles) become one
-    cmd = BUILDS[request.param] + ["-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-pthread", "-DMPH_SP_TS=1",
+    cmd = BUILDS[request.param] + ["-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-pthread", "-DMPH_SP_TS=1", "-DMPH_SPW_T=1",
            "-I", os.path.join(ROOT, "tests", "host", "emu"),
            os.path.join(ROOT, "tests", "host", "kernels_emu.cpp"),
            os.path.join(ROOT, "mpres-blas_amd", "csrc", "consts.cpp"), "-o", str(out)]
