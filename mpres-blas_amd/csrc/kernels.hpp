@@ -1380,11 +1380,11 @@ __global__ __launch_bounds__(BLK) void k_spmv(const DevConsts *__restrict__ C, i
 // any order (the sum is exact), CSR rows longer than SP_LONG terms by k_spmv_resid_wave
 // with SPW_T threads per row. REDO rows go to the fused kernel's fixup pass.
 constexpr int SGR = 4;        // residues per thread in the residue kernels
-constexpr int SP_LONG = 64;   // CSR rows with more terms take the long-row residue kernel
+constexpr int SP_LONG = 128;  // CSR rows with more terms take the long-row residue kernel
 constexpr int SP_PF = 4;      // terms of a chain whose x fields are in flight ahead of the step
 constexpr int SP_PF1 = 8;     // the same for one chain per row (CSR scalar kernel: long serial chains)
 #ifndef MPH_SP_TS
-#define MPH_SP_TS 4
+#define MPH_SP_TS 2
 #endif
 constexpr int SP_TS = MPH_SP_TS;  // term slots per row in k_spmv_resid (1 in the one-lane CPU emulation)
 
