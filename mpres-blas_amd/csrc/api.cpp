@@ -173,8 +173,9 @@ const Ops *sized(const mpres_ctx *c, int op, long long size) {
         if (size <= t.bound[i]) return t.ops[i];
     return t.count ? t.ops[t.count - 1] : c->ops[op];
 }
-// the split-form SpMV's workspace: counts, 3 x SP_NB class counters, the row order and two row lists
-size_t spmv_ws_bytes(int m) { return 4 * (2 + 3 * (size_t)SP_NB + 3 * (size_t)m); }
+// the split-form SpMV's workspace: the REDO count, 3 x SP_NB class counters, the row order and
+// the REDO row list
+size_t spmv_ws_bytes(int m) { return 4 * (1 + 3 * (size_t)SP_NB + 2 * (size_t)m); }
 
 SoA soa_of(const mp_array_t *a) { return soa_view(a); }
 
@@ -1074,6 +1075,13 @@ static int spmv_common(mpres_ctx_t *c, int fmt, int m, const SpMat &A, int tpr, 
     {
         ProfScope ps(c, OP_SPMV, strm(st));
         HIPCHK(o->spmv(c->dev, 0, fmt, m, A, (const char *)x, (char *)y, tpr, (int *)w, strm(st)));
+        if (o->spmv_two(fmt)) {
+            // the long rows' residues on the side stream beside the short rows' (disjoint rows of y)
+            Fork fk(c, strm(st));
+            HIPCHK(o->spmv(c->dev, 3, fmt, m, A, (const char *)x, (char *)y, tpr, (int *)w, fk.s1));
+            HIPCHK(o->spmv(c->dev, 2, fmt, m, A, (const char *)x, (char *)y, tpr, (int *)w, strm(st)));
+            HIPCHK(fk.join());
+        }
     }
     HIPCHK(o->spmv(c->dev, 1, fmt, m, A, (const char *)x, (char *)y, tpr, (int *)w, strm(st)));
     return MPRES_OK;

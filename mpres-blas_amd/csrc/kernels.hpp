@@ -1377,10 +1377,13 @@ __global__ __launch_bounds__(BLK) void k_spmv(const DevConsts *__restrict__ C, i
 // and tree on the scalar fields alone (one thread per chain, where the fused kernel
 // repeats the scalar work on every lane of a number) and writes y's sign, exponent and
 // evaluation, or REDO; k_spmv_resid sums the residues of the row's terms against E, in
-// any order (the sum is exact), CSR rows longer than SP_LONG terms by k_spmv_resid_wave
-// with SPW_T threads per row. REDO rows go to the fused kernel's fixup pass.
-constexpr int SGR = 4;        // residues per thread in the residue kernels
-constexpr int SP_LONG = 128;  // CSR rows with more terms take the long-row residue kernel
+// any order (the sum is exact), CSR rows of at least 2^SP_LONG_BITS terms by
+// k_spmv_resid_wave with SPW_T threads per row (the two residue kernels take disjoint rows
+// and may run at once). REDO rows go to the fused kernel's fixup pass.
+constexpr int SGR = 4;           // residues per thread in the residue kernels
+constexpr int SP_LONG_BITS = 8;  // CSR rows of 2^8 = 128 terms or more take the long-row residue kernel
+// the long rows' length classes lead the row order (k_sp_bin_*): off[SP_LONG_CLASS] counts them
+constexpr int SP_LONG_CLASS = SP_NB - SP_LONG_BITS;
 constexpr int SP_PF = 4;      // terms of a chain whose x fields are in flight ahead of the step
 constexpr int SP_PF1 = 8;     // the same for one chain per row (CSR scalar kernel: long serial chains)
 #ifndef MPH_SP_TS
@@ -1561,18 +1564,18 @@ __device__ __forceinline__ void sp_resid_slot(const DevConsts *__restrict__ C, c
 
 // Residues of the rows k_spmv_chain finished: SGR residues per thread, N / SGR residue
 // groups by SP_TS term slots per row (slot t takes terms t, t + SP_TS, ...; the slots'
-// sums meet through shuffles), rows in perm's order (CSR). With a long-row list (CSR), a
-// row of more than SP_LONG terms is appended to it (count *lcnt) and left to
-// k_spmv_resid_wave.
+// sums meet through shuffles), rows in perm's order (CSR). With skip (CSR), the rows from
+// perm[*skip] on: the first *skip, those of 2^SP_LONG_BITS terms or more, are
+// k_spmv_resid_wave's.
 template <int N, int FMT>
 __global__ __launch_bounds__(BLK) void k_spmv_resid(const DevConsts *__restrict__ C, int m, SpMat A, AoSRef<N> x,
-                                                    AoSRef<N> y, const int *__restrict__ perm, int *lcnt,
-                                                    int *llist) {
+                                                    AoSRef<N> y, const int *__restrict__ perm,
+                                                    const int *__restrict__ skip) {
     static_assert(N % SGR == 0 && SGR == 4, "residue groups");
     constexpr int G = N / SGR, TPR = G * SP_TS;
     static_assert(64 % TPR == 0, "a row's threads in one wavefront");
     const long long g = (long long)blockIdx.x * BLK + threadIdx.x;
-    const long long idx = g / TPR;
+    const long long idx = g / TPR + (skip ? *skip : 0);
     const int q = (int)(g % G), sl = (int)((g / G) % SP_TS), g0 = q * SGR;
     bool live = idx < m;
     const long long row = live ? (perm ? perm[idx] : idx) : 0;
@@ -1588,12 +1591,8 @@ __global__ __launch_bounds__(BLK) void k_spmv_resid(const DevConsts *__restrict_
     }
     if constexpr (FMT == SP_CSR) {
         const int rs = live ? A.ptr[row] : 0, re = live ? A.ptr[row + 1] : 0;
-        if (llist && re - rs > SP_LONG) {
-            if (g0 == 0 && sl == 0) llist[atomicAdd(lcnt, 1)] = (int)row;
-            live = false;
-        }
-        // at most SP_LONG terms (fewer than 2^20: no intermediate reduction); the next
-        // two terms' operands are in flight while one is added
+        // fewer than 2^SP_LONG_BITS terms with skip (fewer than 2^20 in any case: no
+        // intermediate reduction); the next two terms' operands are in flight while one is added
         if (live && rs + sl < re) {  // a slot with terms: rs is then a valid index
             const int E = *y.exp(orow);
             sp_resid_slot<N, 2>(C, A, x, g0, md, mv, acc, rs + sl, re, rs, SP_TS, E);
@@ -1632,19 +1631,20 @@ __global__ __launch_bounds__(BLK) void k_spmv_resid(const DevConsts *__restrict_
     }
 }
 
-// Residues of the long CSR rows listed by k_spmv_resid, a group of SPW_T threads per row:
-// SPW_T / (N / SGR) term slots of N / SGR threads, whose reduced sums meet through shuffles
-// (the group is one wavefront, so no barrier). Groups take rows g, g + (groups of the grid),
-// ... of the list, so any grid covers it. (A workgroup per row, 64 slots joined in LDS,
-// left most slots idle on rows of a few hundred terms.)
+// Residues of the long CSR rows (2^SP_LONG_BITS terms or more: the first *nlong rows of the
+// row order perm), a group of SPW_T threads per row: SPW_T / (N / SGR) term slots of N / SGR
+// threads, whose reduced sums meet through shuffles (the group is one wavefront, so no
+// barrier). Groups take rows g, g + (groups of the grid), ... of the list, so any grid covers
+// it; rows the chains marked REDO are left to the fixup pass. (A workgroup per row, 64 slots
+// joined in LDS, left most slots idle on rows of a few hundred terms.)
 #ifndef MPH_SPW_T
 #define MPH_SPW_T 32
 #endif
 constexpr int SPW_T = MPH_SPW_T;  // threads per long row (half a wavefront; 1 slot per group on the host)
 template <int N>
 __global__ __launch_bounds__(BLK) void k_spmv_resid_wave(const DevConsts *__restrict__ C, SpMat A, AoSRef<N> x,
-                                                         AoSRef<N> y, const int *__restrict__ lcnt,
-                                                         const int *__restrict__ llist) {
+                                                         AoSRef<N> y, const int *__restrict__ perm,
+                                                         const int *__restrict__ nlong) {
     constexpr int G = N / SGR, TPR = SPW_T < G ? G : SPW_T, TS = TPR / G;
     static_assert(64 % TPR == 0 && BLK % TPR == 0, "a row's threads in one wavefront");
     const int t = threadIdx.x % TPR, q = t % G, sl = t / G, g0 = q * SGR;
@@ -1655,11 +1655,12 @@ __global__ __launch_bounds__(BLK) void k_spmv_resid_wave(const DevConsts *__rest
         md[r] = C->md[g0 + r];
         mv[r] = C->minv[g0 + r];
     }
-    const int count = *lcnt;
+    const int count = *nlong;
     for (int k = grp; k < count; k += ngrp) {  // uniform over the row's group
-        const int row = llist[k];
+        const int row = perm[k];
+        if (*y.sign(row) == REDO) continue;
         const int E = *y.exp(row);
-        const int rs = A.ptr[row], re = A.ptr[row + 1];  // a listed row has more than SP_LONG terms
+        const int rs = A.ptr[row], re = A.ptr[row + 1];
         double acc[SGR] = {0.0, 0.0, 0.0, 0.0};
         // a slot's terms in blocks of at most GV_REDUCE, reduced between blocks
         for (int i = rs + sl; i < re; i += GV_REDUCE * TS) {

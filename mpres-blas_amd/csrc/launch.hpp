@@ -424,53 +424,64 @@ struct Launch {
                                BH, EL, BL, EH + m, BH + m, EL + m, BL + m, alpha, beta, Cm, ldc);
         return hipGetLastError();
     }
-    // pass 0: the row results, or REDO marks (16 and 32 moduli: the split form: CSR rows
-    // grouped by length, scalar chains, residues of short and of long rows); pass 1: the
-    // marked rows recomputed by the fused kernel. ws: spmv_ws_ints(m) ints (split form):
-    // long-row and marked-row counts, class counts, offsets and cursors, then the row
-    // order, the long-row list and the marked-row list.
     // workgroups of k_spmv_resid_wave: BLK / SPW_T rows each per sweep, at most 16384
     static unsigned rows_blocks(int m) {
         const long long b = ((long long)m + BLK / SPW_T - 1) / (BLK / SPW_T);
         return (unsigned)(b < 16384 ? b : 16384);
     }
+    // Passes of the split form (16 and 32 moduli): 0 the rows' order (CSR rows grouped by
+    // length, longest first) and the scalar chains (and, but for CSR, the residues); 2 the
+    // residues of CSR's short rows, 3 those of its long rows (2 and 3 take disjoint rows, so
+    // they may run at once on two streams); 1 the rows the chains marked REDO, recomputed by
+    // the fused kernel. Without the split form (8 moduli) pass 0 is the fused kernel and the
+    // others do nothing. ws (split form, spmv_ws_bytes(m)): the marked-row count, class
+    // counts, offsets and cursors, then the row order and the marked-row list.
     template <int FMT>
     static void spmv_fmt(const DevConsts *c, int pass, int m, const SpMat &A, const char *x, char *y, int tpr,
                          int *ws, hipStream_t st) {
         const dim3 gr(grid_for((long long)m * tpr, L)), bl(BLK);
         const AoSRef<N> X{(char *)x}, Y{y};
-        int *lcnt = ws, *rcnt = ws + 1, *hist = ws + 2, *off = hist + SP_NB, *cur = off + SP_NB;
-        int *perm = cur + SP_NB, *llist = perm + m, *rlist = llist + m;
-        if (pass == 0 && SPLIT) {
-            hipLaunchKernelGGL((k_fill_int<N, L>), dim3(1), bl, 0, st, ws, 2 + 3 * SP_NB, 0);
-            const int *pm = nullptr;
-            if (FMT == SP_CSR) {
-                const dim3 gm((unsigned)(((long long)m + (long long)BLK * SP_RPT - 1) / ((long long)BLK * SP_RPT)));
-                hipLaunchKernelGGL((k_sp_bin_count<N, L>), gm, bl, 0, st, m, A.ptr, hist);
-                hipLaunchKernelGGL((k_sp_bin_scan<N, L>), dim3(1), dim3(64), 0, st, (const int *)hist, off);
-                hipLaunchKernelGGL((k_sp_bin_scatter<N, L>), gm, bl, 0, st, m, A.ptr, (const int *)off, cur, perm);
-                pm = perm;
-            }
-            if (FMT == SP_CSR && tpr == 1)  // long serial chains: a deeper ring of gathers
-                hipLaunchKernelGGL((k_spmv_chain<N, FMT, SP_PF1>), dim3(grid_for((long long)m, 1)), bl, 0, st, c, m,
-                                   A, X, Y, tpr, pm, rcnt, rlist);
-            else
-                hipLaunchKernelGGL((k_spmv_chain<N, FMT>), dim3(grid_for((long long)m * tpr, 1)), bl, 0, st, c, m, A,
-                                   X, Y, tpr, pm, rcnt, rlist);
-            hipLaunchKernelGGL((k_spmv_resid<N, FMT>), dim3(grid_for((long long)m * (N / SGR) * SP_TS, 1)), bl, 0, st, c, m,
-                               A, X, Y, pm, lcnt, FMT == SP_CSR ? llist : nullptr);
-            if (FMT == SP_CSR)
+        int *rcnt = ws, *hist = ws + 1, *off = hist + SP_NB, *cur = off + SP_NB;
+        int *perm = cur + SP_NB, *rlist = perm + m;
+        const int *nlong = off + SP_LONG_CLASS;  // the long rows lead perm
+        if constexpr (SPLIT) {
+            if (pass == 0) {
+                hipLaunchKernelGGL((k_fill_int<N, L>), dim3(1), bl, 0, st, ws, 1 + 3 * SP_NB, 0);
+                const int *pm = nullptr;
+                if (FMT == SP_CSR) {
+                    const dim3 gm(
+                        (unsigned)(((long long)m + (long long)BLK * SP_RPT - 1) / ((long long)BLK * SP_RPT)));
+                    hipLaunchKernelGGL((k_sp_bin_count<N, L>), gm, bl, 0, st, m, A.ptr, hist);
+                    hipLaunchKernelGGL((k_sp_bin_scan<N, L>), dim3(1), dim3(64), 0, st, (const int *)hist, off);
+                    hipLaunchKernelGGL((k_sp_bin_scatter<N, L>), gm, bl, 0, st, m, A.ptr, (const int *)off, cur,
+                                       perm);
+                    pm = perm;
+                }
+                if (FMT == SP_CSR && tpr == 1)  // long serial chains: a deeper ring of gathers
+                    hipLaunchKernelGGL((k_spmv_chain<N, FMT, SP_PF1>), dim3(grid_for((long long)m, 1)), bl, 0, st,
+                                       c, m, A, X, Y, tpr, pm, rcnt, rlist);
+                else
+                    hipLaunchKernelGGL((k_spmv_chain<N, FMT>), dim3(grid_for((long long)m * tpr, 1)), bl, 0, st, c,
+                                       m, A, X, Y, tpr, pm, rcnt, rlist);
+                if (FMT != SP_CSR)
+                    hipLaunchKernelGGL((k_spmv_resid<N, FMT>), dim3(grid_for((long long)m * (N / SGR) * SP_TS, 1)),
+                                       bl, 0, st, c, m, A, X, Y, nullptr, nullptr);
+            } else if (pass == 2 && FMT == SP_CSR) {
+                hipLaunchKernelGGL((k_spmv_resid<N, FMT>), dim3(grid_for((long long)m * (N / SGR) * SP_TS, 1)), bl,
+                                   0, st, c, m, A, X, Y, (const int *)perm, nlong);
+            } else if (pass == 3 && FMT == SP_CSR) {
                 hipLaunchKernelGGL((k_spmv_resid_wave<N>), dim3(rows_blocks(m)), dim3(BLK), 0, st, c, A, X, Y,
-                                   (const int *)lcnt, (const int *)llist);
+                                   (const int *)perm, nlong);
+            } else if (pass == 1) {
+                const unsigned g1 = grid_for((long long)m * tpr, L);
+                hipLaunchKernelGGL((k_spmv<N, L, true, true, FMT>), dim3(g1 < 2048 ? g1 : 2048), bl, 0, st, c, m, A,
+                                   X, Y, tpr, (const int *)rcnt, (const int *)rlist);
+            }
         } else if (pass == 0) {
-            hipLaunchKernelGGL((k_spmv<N, L, !SPLIT, false, FMT>), gr, bl, 0, st, c, m, A, X, Y, tpr, nullptr,
-                               nullptr);
-        } else if constexpr (SPLIT) {
-            const unsigned g1 = grid_for((long long)m * tpr, L);
-            hipLaunchKernelGGL((k_spmv<N, L, true, true, FMT>), dim3(g1 < 2048 ? g1 : 2048), bl, 0, st, c, m, A, X, Y,
-                               tpr, (const int *)rcnt, (const int *)rlist);
+            hipLaunchKernelGGL((k_spmv<N, L, true, false, FMT>), gr, bl, 0, st, c, m, A, X, Y, tpr, nullptr, nullptr);
         }
     }
+    static bool spmv_two(int fmt) { return SPLIT && fmt == SP_CSR; }
     static hipError_t spmv(const DevConsts *c, int pass, int fmt, int m, SpMat A, const char *x, char *y, int tpr,
                            int *ws, hipStream_t st) {
         if (m <= 0) return hipSuccess;
@@ -612,6 +623,7 @@ struct Launch {
         o.gemm_mfma_ws = &gemm_mfma_ws;
         o.gemm_v2 = &gemm_v2;
         o.spmv = &spmv;
+        o.spmv_two = &spmv_two;
         o.batch = &batch;
         return o;
     }

@@ -90,8 +90,10 @@ struct Ops {
     hipError_t (*gemm_mfma)(const DevConsts *, int, int, int, int, int, int, int, SoA, SoA, int, SoA, int, SoA, SoA,
                             int, char *, hipStream_t);
     size_t (*gemm_mfma_ws)(int, int, int, int);
-    // SpMV over a storage format (SP_CSR, SP_ELL, SP_JAD, SP_DIA of kernels.hpp): pass 0 main, 1 fixup
+    // SpMV over a storage format (SP_CSR, SP_ELL, SP_JAD, SP_DIA of kernels.hpp): pass 0 main, 1 fixup,
+    // 2 and 3 the short and long rows' residues of the split form over CSR (see launch.hpp)
     hipError_t (*spmv)(const DevConsts *, int, int, int, SpMat, const char *, char *, int, int *, hipStream_t);
+    bool (*spmv_two)(int fmt);  // passes 2 and 3 have work (and may run at once on two streams)
     // the rest of the BLAS surface (kernels_blas.hpp)
     hipError_t (*v1map)(const DevConsts *, int, int, int, int, SoA, int, SoA, int, int, SoA, int, int, SoA, int, int,
                         hipStream_t);

@@ -187,20 +187,21 @@ int spmv_split_n(const Ctx &x, int m, const int *irp, const int *ja, const doubl
                  int *counts) {
     const SpMat A{as, ja, irp, nullptr, nullptr, 0, 0};
     const AoSRef<N> xa{X}, ya{Y};
-    std::vector<int> ws(2 + 3 * SP_NB + 3 * (size_t)m, 0);
-    int *lcnt = ws.data(), *rcnt = lcnt + 1, *hist = lcnt + 2, *off = hist + SP_NB, *cur = off + SP_NB;
-    int *perm = cur + SP_NB, *llist = perm + m, *rlist = llist + m;
+    std::vector<int> ws(1 + 3 * SP_NB + 2 * (size_t)m, 0);
+    int *rcnt = ws.data(), *hist = rcnt + 1, *off = hist + SP_NB, *cur = off + SP_NB;
+    int *perm = cur + SP_NB, *rlist = perm + m;
+    const int *nlong = off + SP_LONG_CLASS;
     const unsigned gm = (unsigned)((m + BLK * SP_RPT - 1) / (BLK * SP_RPT));
     for (unsigned b = 0; b < gm; b++) run_block(b, gm, BLK, [&] { k_sp_bin_count<N, 1>(m, irp, hist); });
     run(1, [&] { k_sp_bin_scan<N, 1>(hist, off); });
     for (unsigned b = 0; b < gm; b++) run_block(b, gm, BLK, [&] { k_sp_bin_scatter<N, 1>(m, irp, off, cur, perm); });
     run(m, [&] { k_spmv_chain<N, SP_CSR, SP_PF1>(&x.c, m, A, xa, ya, 1, perm, rcnt, rlist); });
-    run((long long)m * (N / SGR) * SP_TS, [&] { k_spmv_resid<N, SP_CSR>(&x.c, m, A, xa, ya, perm, lcnt, llist); });
+    run((long long)m * (N / SGR) * SP_TS, [&] { k_spmv_resid<N, SP_CSR>(&x.c, m, A, xa, ya, perm, nlong); });
     // the long rows, two workgroups (fewer groups than rows: each takes several)
-    for (unsigned b = 0; b < 2; b++) run_block(b, 2, BLK, [&] { k_spmv_resid_wave<N>(&x.c, A, xa, ya, lcnt, llist); });
+    for (unsigned b = 0; b < 2; b++) run_block(b, 2, BLK, [&] { k_spmv_resid_wave<N>(&x.c, A, xa, ya, perm, nlong); });
     // the fixup over the marked rows, two workgroups (each takes several list entries)
     for (unsigned b = 0; b < 2; b++) run_block(b, 2, BLK, [&] { k_spmv<N, 1, true, true, SP_CSR>(&x.c, m, A, xa, ya, 1, rcnt, rlist); });
-    counts[0] = *lcnt;
+    counts[0] = *nlong;
     counts[1] = *rcnt;
     return 0;
 }
@@ -257,7 +258,7 @@ int emu_gemm_chain_nn(int set_id, int m, int n, int k, void *A, void *B, void *W
     return -1;
 }
 
-// counts: the long-row and marked-row list lengths
+// counts: the number of long rows (2^SP_LONG_BITS terms or more) and the marked-row list length
 int emu_spmv_split(int set_id, int m, const int *irp, const int *ja, const double *as, void *X, void *Y, int *counts) {
     Ctx x;
     if (!make_ctx(set_id, x)) return -1;
