@@ -396,7 +396,8 @@ int mpres_gemm_v2(mpres_ctx_t *ctx, int transa, int transb, int m, int n, int k,
 
 /* ---- sparse: double CSR matrix times mp vector (x, y: mp_float_t arrays) ----
  * Over 16 and 32 moduli every format runs in the split form (scalar chains, then residues)
- * and takes 4 * (50 + 3 m) bytes of the context's pool for its row order and row lists. */
+ * and takes 4 * (49 + 2 m) bytes of the context's pool for its row order and REDO row list; over
+ * CSR the long rows' residues run on the context's second stream, joined back before return. */
 /* cuda::mp_spmv_csr<threads>, src/sparse/spmv/spmv_csr.cuh:48-64 */
 int mpres_spmv_csr(mpres_ctx_t *ctx, int m, const csr_t *csr, const void *x, void *y, void *stream);
 /* cuda::mp_spmv_csrv<threads,threadsPerRow>, src/sparse/spmv/spmv_csrv.cuh:48-87; threads_per_row in {1,2,4,8,16,32} */
