@@ -208,6 +208,7 @@ struct Grp {
         k.C = c;
         k.P2 = c->pow2;
         k.lane = (int)(__lane_id() & (L - 1));
+        k.load();
         if constexpr (CACHE) {
 #pragma unroll
             for (int r = 0; r < R; r++) {

@@ -22,13 +22,58 @@ namespace mph {
 template <int N, int L>
 struct Core {
     static constexpr int R = N / L;
+    // One residue per lane on the inlined paths (8 moduli on 8 lanes, where rounding runs on
+    // almost every step): the lane's constants are read once, by load(), and held in
+    // registers, so that no step waits on a reload of them. Otherwise they are read where used.
+    static constexpr bool KC = (N <= 8 && R == 1);
+    static constexpr int KN = KC ? N - 1 : 1;
     const DevConsts *__restrict__ C;
     const int *__restrict__ P2;
     int lane;
+    double kmd, kmv, krd, kru;
+    int kmi, kw, kmrc[KN];
+    __device__ __forceinline__ void load() {
+        if constexpr (KC) {
+            kmd = C->md[lane];
+            kmv = C->minv[lane];
+            krd = C->rrd[lane];
+            kru = C->rru[lane];
+            kmi = C->m[lane];
+            kw = C->w[lane];
+#pragma unroll
+            for (int j = 0; j < KN; j++) kmrc[j] = C->mrcinv[j][lane];
+        }
+    }
     __device__ __forceinline__ int ix(int r) const { return lane * R + r; }
-    __device__ __forceinline__ double md(int r) const { return C->md[ix(r)]; }
-    __device__ __forceinline__ double mv(int r) const { return C->minv[ix(r)]; }
-    __device__ __forceinline__ int mi(int r) const { return C->m[ix(r)]; }
+    __device__ __forceinline__ double md(int r) const {
+        if constexpr (KC) return kmd;
+        else return C->md[ix(r)];
+    }
+    __device__ __forceinline__ double mv(int r) const {
+        if constexpr (KC) return kmv;
+        else return C->minv[ix(r)];
+    }
+    __device__ __forceinline__ int mi(int r) const {
+        if constexpr (KC) return kmi;
+        else return C->m[ix(r)];
+    }
+    __device__ __forceinline__ int w(int r) const {
+        if constexpr (KC) return kw;
+        else return C->w[ix(r)];
+    }
+    __device__ __forceinline__ double rrd(int r) const {
+        if constexpr (KC) return krd;
+        else return C->rrd[ix(r)];
+    }
+    __device__ __forceinline__ double rru(int r) const {
+        if constexpr (KC) return kru;
+        else return C->rru[ix(r)];
+    }
+    // m_j^-1 mod m_i for j < i = ix(r) (the mixed-radix conversion's step j)
+    __device__ __forceinline__ int mrc(int j, int r) const {
+        if constexpr (KC) return kmrc[j];
+        else return C->mrcinv[j][ix(r)];
+    }
 };
 
 template <int COUNT, bool U, typename F>
@@ -117,7 +162,7 @@ __device__ __forceinline__ void rns_mrc(Core<N, L> k, int *mr, const int *x) {
             if (i > j) {
                 int t = mr[r] - v;
                 t = (mr[r] < v) ? t + k.mi(r) : t;
-                mr[r] = mulmod(t, k.C->mrcinv[j][i], k.md(r), k.mv(r));
+                mr[r] = mulmod(t, k.mrc(j, r), k.md(r), k.mv(r));
             }
         });
     });
@@ -153,9 +198,9 @@ __device__ __forceinline__ void rns_eval(Core<N, L> k, const int *x, Er &lo, Er 
     int s[R];
     double fl[R], fu[R];
     loop<R, U>([&](int r) {
-        s[r] = mulmod(x[r], C->w[k.ix(r)], k.md(r), k.mv(r));
-        fl[r] = mul_rd((double)s[r], C->rrd[k.ix(r)]);
-        fu[r] = mul_ru((double)s[r], C->rru[k.ix(r)]);
+        s[r] = mulmod(x[r], k.w(r), k.md(r), k.mv(r));
+        fl[r] = mul_rd((double)s[r], k.rrd(r));
+        fu[r] = mul_ru((double)s[r], k.rru(r));
     });
     double suml = psum_arr<N, L, false, U>(fl);
     double sumu = psum_arr<N, L, true, U>(fu);
@@ -195,13 +240,13 @@ __device__ __forceinline__ void rns_eval(Core<N, L> k, const int *x, Er &lo, Er 
         const int *row = k.P2 + (size_t)kk * N;
         loop<R, U>([&](int r) {
             s[r] = mulmod(s[r], row[k.ix(r)], k.md(r), k.mv(r));
-            fu[r] = mul_ru((double)s[r], C->rru[k.ix(r)]);
+            fu[r] = mul_ru((double)s[r], k.rru(r));
         });
         sumu = psum_arr<N, L, true, U>(fu);
         sumu = sub_ru(sumu, (double)(unsigned)sumu);
         K += kk;
     }
-    loop<R, U>([&](int r) { fl[r] = mul_rd((double)s[r], C->rrd[k.ix(r)]); });
+    loop<R, U>([&](int r) { fl[r] = mul_rd((double)s[r], k.rrd(r)); });
     suml = psum_arr<N, L, false, U>(fl);
     suml = sub_rd(suml, (double)(unsigned)suml);
     lo = er_set_d(suml);
@@ -216,8 +261,8 @@ __device__ __forceinline__ int rns_rank_full(Core<N, L> k, const int *x, const i
     constexpr int R = N / L;
     double fl[R], fu[R];
     loop<R, U>([&](int r) {
-        fl[r] = mul_rd((double)c[r], k.C->rrd[k.ix(r)]);
-        fu[r] = mul_ru((double)c[r], k.C->rru[k.ix(r)]);
+        fl[r] = mul_rd((double)c[r], k.rrd(r));
+        fu[r] = mul_ru((double)c[r], k.rru(r));
     });
     const unsigned whl = (unsigned)psum_arr<N, L, false, U>(fl), whu = (unsigned)psum_arr<N, L, true, U>(fu);
     if (whl == whu) return (int)whl;
@@ -227,7 +272,7 @@ template <int N, int L, bool U>
 __device__ __forceinline__ int rns_rank_fast(Core<N, L> k, const int *c) {
     constexpr int R = N / L;
     double fu[R];
-    loop<R, U>([&](int r) { fu[r] = mul_ru((double)c[r], k.C->rru[k.ix(r)]); });
+    loop<R, U>([&](int r) { fu[r] = mul_ru((double)c[r], k.rru(r)); });
     return (int)psum_arr<N, L, true, U>(fu);
 }
 // y = (x - (X mod 2^j)) * 2^-j, given the rank kr of x
@@ -249,7 +294,7 @@ __device__ __forceinline__ void rns_scale_step(Core<N, L> k, int *y, int kr, int
 template <int N, int L, bool U>
 __device__ __forceinline__ void rns_xw(Core<N, L> k, int *c, const int *x) {
     constexpr int R = N / L;
-    loop<R, U>([&](int r) { c[r] = mulmod(x[r], k.C->w[k.ix(r)], k.md(r), k.mv(r)); });
+    loop<R, U>([&](int r) { c[r] = mulmod(x[r], k.w(r), k.md(r), k.mv(r)); });
 }
 template <int N, int L, bool U>
 __device__ __forceinline__ void rns_scale2pow(Core<N, L> k, int *x, unsigned D) {
