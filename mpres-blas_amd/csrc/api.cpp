@@ -86,7 +86,7 @@ const BuiltinLanes kBuiltinLanes[] = {
     {"8_2double", OP_DOT, 4, {16384LL, 262144LL, 4194304LL, 16777216LL}, {8, 8, 8, 8}},
     {"8_2double", OP_AXPY, 4, {16384LL, 262144LL, 4194304LL, 16777216LL}, {8, 2, 1, 1}},
     {"8_2double", OP_GEMV, 4, {16384LL, 262144LL, 4194304LL, 16777216LL}, {8, 8, 8, 1}},
-    {"8_2double", OP_GEMM, 4, {741455LL, 5931641LL, 47453132LL, 134217728LL}, {8, 8, 2, 1}},
+    {"8_2double", OP_GEMM, 4, {741455LL, 5931641LL, 47453132LL, 134217728LL}, {8, 8, 8, 1}},
     {"8_2double", OP_SPMV, 3, {31622LL, 316227LL, 1000000LL}, {8, 8, 8}},
     {"8_2double", OP_ELEM, 3, {16384LL, 262144LL, 1048576LL}, {8, 2, 2}},
     {"16_4double", OP_DOT, 4, {16384LL, 262144LL, 4194304LL, 16777216LL}, {16, 16, 4, 4}},
