@@ -88,14 +88,36 @@ __device__ __forceinline__ void loop(F &&f) {
 }
 
 // ---------------- cross-lane sums (balanced tree, see group.hpp) ----------------
-template <int L, bool UP>
+// The value of lane (lane ^ OFF) at one level of a butterfly whose earlier levels made
+// every lane of each block of OFF lanes equal: OFF = 1, 2 by quad permutations, 4 and 8 by
+// the mirror within 8 and 16 lanes (it reaches the other block of OFF lanes, whose lanes
+// all hold its value), all by DPP in registers; 16 and more through a shuffle.
+template <int OFF>
+__device__ __forceinline__ int xchg(int v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (OFF == 1) return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);       // quad_perm 1,0,3,2
+    else if constexpr (OFF == 2) return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
+    else if constexpr (OFF == 4) return __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false); // row_half_mirror
+    else if constexpr (OFF == 8) return __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false); // row_mirror
+    else return __shfl_xor(v, OFF, 64);
+#else
+    return __shfl_xor(v, OFF, 64);
+#endif
+}
+template <int OFF>
+__device__ __forceinline__ double xchg_d(double v) {
+    const uint64_t u = d2u(v);
+    const uint32_t lo = (uint32_t)xchg<OFF>((int)(uint32_t)u), hi = (uint32_t)xchg<OFF>((int)(uint32_t)(u >> 32));
+    return u2d(((uint64_t)hi << 32) | lo);
+}
+template <int L, bool UP, int OFF = 1>
 __device__ __forceinline__ double xlane_sum(double s) {
-#pragma unroll
-    for (int off = 1; off < L; off <<= 1) {
-        const double o = __shfl_xor(s, off, 64);
-        s = UP ? add_ru(s, o) : add_rd(s, o);
+    if constexpr (OFF < L) {
+        const double o = xchg_d<OFF>(s);
+        return xlane_sum<L, UP, 2 * OFF>(UP ? add_ru(s, o) : add_rd(s, o));
+    } else {
+        return s;
     }
-    return s;
 }
 template <int N, int L, bool UP, bool U>
 __device__ __forceinline__ double psum_arr(const double *v) {
@@ -121,20 +143,21 @@ __device__ __forceinline__ double psum_arr(const double *v) {
     }
     return xlane_sum<L, UP>(s);
 }
-template <int L>
+template <int L, int OFF = 1>
 __device__ __forceinline__ uint32_t xlane_usum(uint32_t v) {
-#pragma unroll
-    for (int off = 1; off < L; off <<= 1) v += (uint32_t)__shfl_xor((int)v, off, 64);
-    return v;
+    if constexpr (OFF < L)
+        return xlane_usum<L, 2 * OFF>(v + (uint32_t)xchg<OFF>((int)v));
+    else
+        return v;
 }
-template <int L>
+template <int L, int OFF = 1>
 __device__ __forceinline__ int xlane_imax(int v) {
-#pragma unroll
-    for (int off = 1; off < L; off <<= 1) {
-        const int o = __shfl_xor(v, off, 64);
-        v = o > v ? o : v;
+    if constexpr (OFF < L) {
+        const int o = xchg<OFF>(v);
+        return xlane_imax<L, 2 * OFF>(o > v ? o : v);
+    } else {
+        return v;
     }
-    return v;
 }
 template <int L>
 __device__ __forceinline__ int bcast(int v, int src) {
